@@ -1,0 +1,259 @@
+// Workload generator: stellarator surfaces split into quad charts, each chart
+// into two Vioreanu-Rokhlin triangles with p(p+1)/2 nodes.
+//
+// Restates the geometry the reference's drivers feed to rrq_r64:
+//   surface evaluation          utils/stellarator_geo_mex.c:60-102 (built-in 7-term
+//                               surface and the VMEC Fourier form)
+//   chart list, aspect splits   :124-159 (single pass) and :427-479 (iterated)
+//   chart -> two triangles      :636-749 (tensor Gauss-Legendre grid, spectral
+//                               derivatives, diagonal choice, sw = |t1 x t2| w 4)
+//   exact chart boundary        :555-582 (uv2x_core), used for ichart = 1
+// The 1-D tensor interpolation uses the barycentric form; the reference keeps a
+// monomial solve (BARYCENTRIC 0, :147), which agrees to rounding for p <= 16.
+#include <cmath>
+#include <stdexcept>
+
+#include "qol_host.h"
+
+namespace qol {
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+const double kAspect = 1.65;
+
+void builtin_surface(double t, double p, double* x) {
+  const double ct = std::cos(t), st = std::sin(t);
+  const double c2pt = std::cos(2.0 * p - t), s2pt = std::sin(2.0 * p - t);
+  const double c2p = std::cos(2.0 * p), s2p = std::sin(2.0 * p);
+  const double cp = std::cos(p), sp = std::sin(p);
+  const double cpt = std::cos(p + t), spt = std::sin(p + t);
+  const double cmpt = std::cos(-p + t), smpt = std::sin(-p + t);
+  x[0] = 0.17 * ct * c2pt + 0.11 * ct * c2p + 1.0 * ct * cp + 4.5 * ct - 0.25 * ct * cp + 0.07 * ct * cpt - 0.45 * ct * cmpt;
+  x[1] = 0.17 * st * c2pt + 0.11 * st * c2p + 1.0 * st * cp + 4.5 * st - 0.25 * st * cp + 0.07 * st * cpt - 0.45 * st * cmpt;
+  x[2] = 0.17 * s2pt + 0.11 * s2p + 1.0 * sp + 0.0 + 0.25 * sp + 0.07 * spt - 0.45 * smpt;
+}
+
+void surface_eval(const Surface& s, double t, double p, double* x) {
+  if (s.rc.empty()) { builtin_surface(t, p, x); return; }
+  double R = 0.0, Z = 0.0;
+  for (size_t k = 0; k < s.rc.size(); ++k) {
+    const double ang = s.m[k] * p - (double)s.nfp * s.n[k] * t;
+    R += s.rc[k] * std::cos(ang);
+    Z += s.zs[k] * std::sin(ang);
+  }
+  x[0] = R * std::cos(t);
+  x[1] = R * std::sin(t);
+  x[2] = Z;
+}
+
+void chart_eval(const Surface& s, const Chart& c, double ts1, double ts2, double t1, double t2, double* x) {
+  const double u = ((c.a1 * t1 + c.b1) + 2.0 * c.jj) * ts1;
+  const double v = ((c.a2 * t2 + c.b2) + 2.0 * c.ii) * ts2;
+  surface_eval(s, u, v, x);
+}
+
+double dist3(const double* a, const double* b) {
+  return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+// barycentric Lagrange interpolation matrix L(i,j) = l_j(t_i) on nodes s (p)
+void lagrange_matrix(const double* t, i64 q, const double* s, i64 p, double* L) {
+  std::vector<double> bw(p);
+  for (i64 j = 0; j < p; ++j) {
+    double pr = 1.0;
+    for (i64 k = 0; k < p; ++k) if (k != j) pr *= s[j] - s[k];
+    bw[j] = 1.0 / pr;
+  }
+  for (i64 i = 0; i < q; ++i) {
+    i64 hit = -1;
+    for (i64 j = 0; j < p; ++j) if (t[i] == s[j]) hit = j;
+    if (hit >= 0) {
+      for (i64 j = 0; j < p; ++j) L[i + q * j] = (j == hit) ? 1.0 : 0.0;
+      continue;
+    }
+    double den = 0.0;
+    for (i64 j = 0; j < p; ++j) { L[i + q * j] = bw[j] / (t[i] - s[j]); den += L[i + q * j]; }
+    for (i64 j = 0; j < p; ++j) L[i + q * j] /= den;
+  }
+}
+
+// tensor interpolation weights Mt(m,i), grid node m = a + b p at (t1,t2) = (x0[b], x0[a])
+void tensor_weights(const std::vector<double>& t1, const std::vector<double>& t2, const double* x0, i64 p,
+                    std::vector<double>& Mt) {
+  const i64 q = (i64)t1.size();
+  std::vector<double> L1(q * p), L2(q * p);
+  lagrange_matrix(t1.data(), q, x0, p, L1.data());
+  lagrange_matrix(t2.data(), q, x0, p, L2.data());
+  Mt.assign(p * p * q, 0.0);
+  for (i64 i = 0; i < q; ++i)
+    for (i64 b = 0; b < p; ++b)
+      for (i64 a = 0; a < p; ++a) Mt[(a + b * p) + i * p * p] = L2[i + q * a] * L1[i + q * b];
+}
+
+// the triangle's corner map: side 0/1, diagonal A (d1 >= d2) or B
+void corner_map(bool diagA, i64 side, double u, double v, double* t1, double* t2) {
+  const double up = 2.0 * u - 1.0, vp = 2.0 * v - 1.0;
+  double a = diagA ? up : vp, b = diagA ? vp : -up;
+  if (side == 1) { a = -a; b = -b; }
+  *t1 = a;
+  *t2 = b;
+}
+
+bool chart_diagonal_A(const Surface& s, const Chart& c, double ts1, double ts2, const double* x0, i64 p) {
+  double ll[3], ur[3], ul[3], lr[3];
+  chart_eval(s, c, ts1, ts2, x0[0], x0[0], ll);
+  chart_eval(s, c, ts1, ts2, x0[p - 1], x0[p - 1], ur);
+  chart_eval(s, c, ts1, ts2, x0[0], x0[p - 1], ul);
+  chart_eval(s, c, ts1, ts2, x0[p - 1], x0[0], lr);
+  double d1 = 0.0, d2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    d1 += (ll[d] - ur[d]) * (ll[d] - ur[d]);
+    d2 += (ul[d] - lr[d]) * (ul[d] - lr[d]);
+  }
+  return d1 + 1e-13 > d2;
+}
+
+}  // namespace
+
+std::vector<Chart> build_charts(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect) {
+  std::vector<double> x0(p), w0(p);
+  gl_nodes(p, x0.data(), w0.data(), nullptr);
+  const double ts1 = kPi / (double)np, ts2 = kPi / (double)mp;
+  std::vector<Chart> L;
+  for (i64 j = 1; j <= np; ++j)
+    for (i64 i = 1; i <= mp; ++i) L.push_back(Chart{1.0, 0.0, 1.0, 0.0, (double)j, (double)i});
+  auto aspect = [&](const Chart& c, double& dt, double& dp) {
+    double ll[3], ul[3], lr[3];
+    chart_eval(s, c, ts1, ts2, x0[0], x0[0], ll);
+    chart_eval(s, c, ts1, ts2, x0[0], x0[p - 1], ul);
+    chart_eval(s, c, ts1, ts2, x0[p - 1], x0[0], lr);
+    dt = dist3(ll, lr);
+    dp = dist3(ll, ul);
+  };
+  if (!iterate_aspect) {
+    // one pass; halves are emitted in place (utils/stellarator_geo_mex.c:124-159)
+    std::vector<Chart> out;
+    for (const Chart& c : L) {
+      double dt, dp;
+      aspect(c, dt, dp);
+      if (dt / dp > kAspect) {
+        Chart a = c, b = c;
+        a.a1 = 0.5; a.b1 = -0.5; b.a1 = 0.5; b.b1 = 0.5;
+        out.push_back(a); out.push_back(b);
+      } else if (dp / dt > kAspect) {
+        Chart a = c, b = c;
+        a.a2 = 0.5; a.b2 = -0.5; b.a2 = 0.5; b.b2 = 0.5;
+        out.push_back(a); out.push_back(b);
+      } else out.push_back(c);
+    }
+    return out;
+  }
+  // iterated passes; the second half is appended (:451-479)
+  for (int pass = 0; pass < 8; ++pass) {
+    i64 nsplit = 0;
+    const i64 n0 = (i64)L.size();
+    for (i64 k = 0; k < n0; ++k) {
+      const Chart c = L[k];
+      double dt, dp;
+      aspect(c, dt, dp);
+      if (dt / dp > kAspect) {
+        L[k].a1 = c.a1 / 2.0; L[k].b1 = c.b1 - c.a1 / 2.0;
+        Chart b = c; b.a1 = c.a1 / 2.0; b.b1 = c.b1 + c.a1 / 2.0;
+        L.push_back(b);
+        ++nsplit;
+      } else if (dp / dt > kAspect) {
+        L[k].a2 = c.a2 / 2.0; L[k].b2 = c.b2 - c.a2 / 2.0;
+        Chart b = c; b.a2 = c.a2 / 2.0; b.b2 = c.b2 + c.a2 / 2.0;
+        L.push_back(b);
+        ++nsplit;
+      }
+    }
+    if (!nsplit) break;
+  }
+  return L;
+}
+
+void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, Geometry& g) {
+  const OrderTables& T = order_tables(p);
+  const i64 h = T.h, p2 = p * p;
+  g.p = p; g.h = h; g.mp = mp; g.np = np;
+  g.charts = build_charts(s, mp, np, p, iterate_aspect);
+  const i64 nq = (i64)g.charts.size();
+  g.ntri = 2 * nq;
+  g.nsrc = g.ntri * h;
+  g.sx.assign(3 * g.nsrc, 0.0); g.snx.assign(3 * g.nsrc, 0.0); g.sw.assign(g.nsrc, 0.0);
+  g.rts.assign(3 * g.nsrc, 0.0); g.rps.assign(3 * g.nsrc, 0.0);
+  std::vector<double> x0(p), w0(p), D(p * p);
+  gl_nodes(p, x0.data(), w0.data(), D.data());
+  const double ts1 = kPi / (double)np, ts2 = kPi / (double)mp;
+  // interpolation weights for the four (diagonal, side) combinations
+  std::vector<double> Mt[2][2];
+  for (int dg = 0; dg < 2; ++dg)
+    for (int sd = 0; sd < 2; ++sd) {
+      std::vector<double> t1(h), t2(h);
+      for (i64 i = 0; i < h; ++i) corner_map(dg == 0, sd, T.uvn[2 * i], T.uvn[2 * i + 1], &t1[i], &t2[i]);
+      tensor_weights(t1, t2, x0.data(), p, Mt[dg][sd]);
+    }
+  std::vector<double> grid(3 * p2), d2(3 * p2), d1(3 * p2);
+  for (i64 k = 0; k < nq; ++k) {
+    const Chart& c = g.charts[k];
+    for (i64 b = 0; b < p; ++b)
+      for (i64 a = 0; a < p; ++a) chart_eval(s, c, ts1, ts2, x0[b], x0[a], &grid[3 * (a + b * p)]);
+    // d/dt2 (first grid index) and d/dt1 (second grid index), spectral
+    for (i64 b = 0; b < p; ++b)
+      for (i64 a = 0; a < p; ++a)
+        for (int d = 0; d < 3; ++d) {
+          double s2 = 0.0, s1 = 0.0;
+          for (i64 q = 0; q < p; ++q) s2 += D[a + q * p] * grid[3 * (q + b * p) + d];
+          for (i64 q = 0; q < p; ++q) s1 += D[b + q * p] * grid[3 * (a + q * p) + d];
+          d2[3 * (a + b * p) + d] = s2;
+          d1[3 * (a + b * p) + d] = s1;
+        }
+    const int dg = chart_diagonal_A(s, c, ts1, ts2, x0.data(), p) ? 0 : 1;
+    for (int sd = 0; sd < 2; ++sd) {
+      const std::vector<double>& W = Mt[dg][sd];
+      const i64 off = (2 * k + sd) * h;
+      for (i64 i = 0; i < h; ++i) {
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (i64 m = 0; m < p2; ++m) {
+          const double w = W[m + i * p2];
+          for (int d = 0; d < 3; ++d) {
+            acc[d] += grid[3 * m + d] * w;
+            acc[3 + d] += d2[3 * m + d] * w;
+            acc[6 + d] += d1[3 * m + d] * w;
+          }
+        }
+        // "rts" carries d/dt2, "rps" d/dt1, each times 2; normal = rps x rts
+        const double* xt = acc + 3;
+        const double* xp = acc + 6;
+        double nx[3] = {xp[1] * xt[2] - xp[2] * xt[1], xp[2] * xt[0] - xp[0] * xt[2], xp[0] * xt[1] - xp[1] * xt[0]};
+        const double sp = std::sqrt(nx[0] * nx[0] + nx[1] * nx[1] + nx[2] * nx[2]);
+        for (int d = 0; d < 3; ++d) {
+          g.sx[3 * (off + i) + d] = acc[d];
+          g.snx[3 * (off + i) + d] = nx[d] / sp;
+          g.rts[3 * (off + i) + d] = 2.0 * xt[d];
+          g.rps[3 * (off + i) + d] = 2.0 * xp[d];
+        }
+        g.sw[off + i] = sp * T.wn[i] * 4.0;
+      }
+    }
+  }
+}
+
+void chart_uv2x(const Surface& s, const Geometry& g, i64 itri, i64 nuv, const double* uv, double* x) {
+  if (itri < 0 || itri >= g.ntri) throw std::runtime_error("chart_uv2x: triangle index out of range");
+  const i64 p = g.p;
+  std::vector<double> x0(p), w0(p);
+  gl_nodes(p, x0.data(), w0.data(), nullptr);
+  const double ts1 = kPi / (double)g.np, ts2 = kPi / (double)g.mp;
+  const Chart& c = g.charts[itri / 2];
+  const bool dA = chart_diagonal_A(s, c, ts1, ts2, x0.data(), p);
+  for (i64 i = 0; i < nuv; ++i) {
+    double t1, t2;
+    corner_map(dA, itri % 2, uv[2 * i], uv[2 * i + 1], &t1, &t2);
+    chart_eval(s, c, ts1, ts2, t1, t2, &x[3 * i]);
+  }
+}
+
+}  // namespace qol

@@ -1,0 +1,64 @@
+// Host-side support for the MI355X RRQ close-evaluation library.
+//
+// Per-order tables (Gauss-Legendre, Vioreanu-Rokhlin nodes, the Koornwinder
+// values<->coefficients maps and the four refinement levels of
+// patch_refine_mod.f90:100-421) are built once per patch order on the host and
+// uploaded; everything per panel and per target runs on the GPU.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace qol {
+
+typedef int64_t i64;
+
+struct LevelTable {
+  i64 n = 0, nsub = 0;
+  std::vector<double> uv, wt, interpmat;  // uv(2,n), wt(n), interpmat(n,h) column-major
+  std::vector<i64> suboff, subtyp;        // suboff(nsub+1) 0-based
+};
+
+struct OrderTables {
+  i64 p = 0, h = 0, nq = 0, korder = 0, orderff = 0, ncoef = 0;
+  std::vector<double> tgl, wgl, Dgl, wbc, Legmat;  // nq, nq, nq*nq, nq, nq*nq
+  std::vector<double> uvn, wn;                    // VR nodes of order p-1
+  std::vector<double> umatr, vmatr;               // h*h
+  LevelTable lev[4];
+  i64 ntot = 0, nsubtot = 0;                       // sums over levels
+};
+
+// Thread-safe, cached per order.  Orders 2..20.
+const OrderTables& order_tables(i64 p);
+
+// primitives (also used by the workload generator)
+void gl_nodes(i64 n, double* t, double* w, double* D);
+void koornwinder(double u, double v, i64 nmax, double* f, double* fu, double* fv);
+void vr_table(i64 order, double* uv, double* w);
+void invert(i64 n, double* A);
+
+// ---------------- workload (stellarator surfaces) ----------------
+struct Surface {
+  // nmode == 0: the reference's built-in 7-term surface; otherwise VMEC form
+  // R = sum rc cos(m th - nfp n ph), Z = sum zs sin(m th - nfp n ph)
+  i64 nfp = 1;
+  std::vector<double> m, n, rc, zs;
+};
+
+struct Chart {
+  double a1, b1, a2, b2, jj, ii;
+};
+
+std::vector<Chart> build_charts(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect);
+
+struct Geometry {
+  i64 p = 0, h = 0, ntri = 0, nsrc = 0, mp = 0, np = 0;
+  std::vector<double> sx, snx, sw, rts, rps;  // (3,nsrc) / (nsrc)
+  std::vector<Chart> charts;
+};
+
+void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, Geometry& g);
+// exact chart points of triangle itri (0-based) at reference-triangle (u,v)
+void chart_uv2x(const Surface& s, const Geometry& g, i64 itri, i64 nuv, const double* uv, double* x);
+
+}  // namespace qol

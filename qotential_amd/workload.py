@@ -1,0 +1,88 @@
+"""Stellarator workloads: geometry, near-surface targets, point-charge densities.
+
+Geometry and targets come from the library's host generator (a restatement of
+utils/stellarator_geo_mex.c); densities are closed forms evaluated here.
+"""
+import numpy as np
+
+from ._lib import check, dbl, i64, lib, ptr
+
+# 5-period surface R = 1 + 0.3 cos th + 0.06 cos(th - 5 ph), Z = 0.3 sin th - 0.06 sin(th - 5 ph)
+VMEC_BENCH = np.array([[0, 0, 1.0, 0.0], [1, 0, 0.3, 0.3], [1, 1, 0.06, -0.06]], dtype=np.float64)
+
+
+def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None):
+    """Triangulated surface: dict(sx, snx, sw, rts, rps (nsrc,3|nsrc), ntri, p, h).
+
+    modes=None -> the reference's built-in 7-term surface (single aspect pass);
+    modes=(nmode,4) array of (m, n, Rc, Zs) -> VMEC form with nfp periods.
+    """
+    L = lib()
+    nmode = 0 if modes is None else len(modes)
+    mn = np.zeros(4) if modes is None else np.ascontiguousarray(modes, dtype=np.float64)
+    it = (0 if modes is None else 1) if iterate_aspect is None else int(iterate_aspect)
+    ntri = L.qol_geo_count(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it))
+    if ntri <= 0:
+        raise RuntimeError("geometry chart build failed")
+    h = p * (p + 1) // 2
+    nsrc = ntri * h
+    sx = np.zeros((nsrc, 3)); snx = np.zeros((nsrc, 3)); sw = np.zeros(nsrc)
+    rts = np.zeros((nsrc, 3)); rps = np.zeros((nsrc, 3)); charts = np.zeros((ntri // 2, 6))
+    check(L.qol_geo_build(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it), ptr(sx), ptr(snx),
+                          ptr(sw), ptr(rts), ptr(rps), ptr(charts)), "qol_geo_build")
+    return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=charts,
+                modes=modes, nfp=nfp, iterate_aspect=it)
+
+
+def chart_boundary(geo):
+    """exact chart boundary at the edge Gauss-Legendre nodes (ichart=1 inputs)."""
+    L = lib()
+    p = geo["p"]; nq = p + 2; ntri = geo["ntri"]
+    modes = geo["modes"]
+    nmode = 0 if modes is None else len(modes)
+    mn = np.zeros(4) if modes is None else np.ascontiguousarray(modes, dtype=np.float64)
+    sxbd = np.zeros((ntri, 3 * nq, 3)); rv = np.zeros((ntri, 3, 3))
+    check(L.qol_geo_chart_boundary(i64(geo["mp"]), i64(geo["np"]), i64(p), i64(geo["nfp"]), i64(nmode), ptr(mn),
+                                   i64(geo["iterate_aspect"]), i64(nq), ptr(sxbd), ptr(rv)), "qol_geo_chart_boundary")
+    return sxbd, rv
+
+
+def targets(geo, nt, umin, umax, sides=0, seed=1):
+    """near-surface targets: offset side*h*10^U along the patch normal."""
+    tx = np.zeros((nt, 3)); tri = np.zeros(nt, dtype=np.int64); dist = np.zeros(nt)
+    check(lib().qol_make_targets(i64(geo["p"]), i64(geo["ntri"]), ptr(geo["sx"]), ptr(geo["snx"]), i64(nt),
+                                 dbl(umin), dbl(umax), i64(sides), C_u64(seed), ptr(tx), ptr(tri), ptr(dist)),
+          "qol_make_targets")
+    return tx, tri, dist
+
+
+def C_u64(v):
+    import ctypes
+    return ctypes.c_uint64(v)
+
+
+def point_charges(geo, nchg=8, offset=0.4, seed=2):
+    """nchg charges q~N(0,1) at x_s + offset*n(x_s) for surface nodes x_s drawn uniformly."""
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, geo["sx"].shape[0], size=nchg)
+    pos = geo["sx"][idx] + offset * geo["snx"][idx]
+    q = rng.standard_normal(nchg)
+    return pos, q
+
+
+def charge_field(x, pos, q):
+    """u = sum q/(4 pi |x-p|) and grad u."""
+    d = x[:, None, :] - pos[None, :, :]
+    r = np.sqrt((d * d).sum(-1))
+    u = (q[None, :] / (4 * np.pi * r)).sum(1)
+    g = -(q[None, :, None] * d / (4 * np.pi * r[..., None] ** 3)).sum(1)
+    return u, g
+
+
+def harmonic_exp(x, lam=0.5):
+    """the reference GRF density exp(lam x) cos(lam y) and its gradient
+    (stellarator_grf_core_mod.f90:1089-1103)."""
+    e = np.exp(lam * x[:, 0])
+    u = e * np.cos(lam * x[:, 1])
+    g = np.stack([lam * e * np.cos(lam * x[:, 1]), -lam * e * np.sin(lam * x[:, 1]), np.zeros(len(x))], 1)
+    return u, g
