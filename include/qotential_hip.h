@@ -45,6 +45,94 @@ int qol_make_targets(int64_t p, int64_t ntri, const double* sx, const double* sn
                      int64_t nt, double umin, double umax, int64_t sides, uint64_t seed,
                      double* tx, int64_t* tri, double* dist);
 
+/* ---------------- per-order tables ----------------
+ * qol_simplex_precomp_mex_ replaces simplex_precomp_r64 / qol_simplex_precomp_mex
+ * (src/lap3d_close_mod.f90:115-133, src/lap3d_close_mex.f90:48-59): Gauss-Legendre
+ * nodes/weights/differentiation matrix, barycentric weights, Legendre
+ * values->coefficients, Koornwinder values<->coefficients at the VR nodes. */
+void qol_simplex_precomp_mex_(const int64_t* nquad, const int64_t* korder, const int64_t* kpols,
+                              double* tgl, double* wgl, double* Dgl, double* w_bclag,
+                              double* Legmat, double* umatr, double* vmatr);
+/* level sizes of patch order p: info[0..12] = h, nquad, ncoef, ntot, nsubtot,
+ * nl(1..4), nsub(1..4).  No reference counterpart (the reference keeps them in
+ * the module variable REF, src/patch_refine_mod.f90:51). */
+int qol_order_info(int64_t p, int64_t* info);
+
+/* ---------------- drop-in close evaluation ----------------
+ * qol_rrq_mex_ replaces qol_rrq_mex -> rrq_r64 (src/lap3d_close_mex.f90:18-45,
+ * src/lap3d_close_mod.f90:622-881): As, Ad (m,n), Omega (4h,m),
+ * IalphaAsvestas (m) for m targets against one panel; timeinfo(20) accumulated. */
+void qol_rrq_mex_(const int64_t* m, const double* tx, const int64_t* n, const double* sx,
+                  const double* snx, const double* sw, const double* rts, const double* rps,
+                  const int64_t* order, const int64_t* nquad, const int64_t* orderff,
+                  const double* distff, const int64_t* exterior, const int64_t* isimd,
+                  const int64_t* ichart, const double* sxbd_chart, const double* rv_chart,
+                  const double* tgl, const double* wgl, const double* Dgl, const double* w_bclag,
+                  const double* Legmat, const double* umatr, const double* vmatr, double* As,
+                  double* Ad, double* Omega, double* IalphaAsvestas, double* timeinfo);
+/* qol_getnearquad_lap_rrq_mex_ replaces qol_getnearquad_lap_rrq_mex ->
+ * getnearquad_lap_rrq_r64 (src/lap3d_close_mex.f90:62-81,
+ * src/lap3d_close_mod.f90:884-960): COO triplets of (S - S_naive, K - K_naive),
+ * out index (tcxi(k)-1)*hdim + (j-1)*m + i, 1-based rows/cols. */
+void qol_getnearquad_lap_rrq_mex_(const int64_t* npan, const int64_t* nterms, const int64_t* hdim,
+                                  const int64_t* nquad, const int64_t* nsrc, const double* sx,
+                                  const double* snx, const double* sw, const double* rts,
+                                  const double* rps, const int64_t* ntcx, const double* tcx,
+                                  const int64_t* tcxrow, const int64_t* tcxi,
+                                  const int64_t* orderff, const double* distff,
+                                  const int64_t* iside, const int64_t* isimd, const int64_t* nnz,
+                                  int64_t* sparsei, int64_t* sparsej, double* sparsevs,
+                                  double* sparsevd, double* timeinfo);
+/* handle-based refinement levels, replacing src/patch_refine_mex.f90:198-242
+ * (qol_build_patch_levels_mex, qol_lap3dsdlpmat_levels_mex,
+ * qol_clear_patch_levels_mex, qol_clear_patch_levels_all_mex).  One slot,
+ * generation-tagged handle; ier 0 ok, 1 no free slot, 2 bad handle,
+ * 3 npols mismatch, 5 unsupported level set, 6 device failure. */
+void qol_build_patch_levels_mex_(const int64_t* korder, const int64_t* npols, const double* sx,
+                                 const double* rts, const double* rps, const int64_t* order,
+                                 const double* dist, const int64_t* nlevel, int64_t* lv,
+                                 int64_t* ier);
+void qol_lap3dsdlpmat_levels_mex_(const int64_t* m, const double* tx, const int64_t* npols,
+                                  const int64_t* lv, const int64_t* isimd, double* As, double* Ad,
+                                  int64_t* idxs, int64_t* ms, int64_t* ier);
+void qol_clear_patch_levels_mex_(const int64_t* lv, int64_t* ier);
+void qol_clear_patch_levels_all_mex_(int64_t* ier);
+
+/* ---------------- batched engine (GPU-resident surface) ----------------
+ * No single reference counterpart: one engine does for a whole surface what
+ * stellarator_grf_core_mod.f90:645-899 does panel by panel (ball count, CSR,
+ * rrq_r64 + naive block per panel, apply, fixed-order scatter), with geometry,
+ * targets, near list and weights resident in HBM.  device_ptrs != 0 means the
+ * array arguments are device pointers. */
+int qol_engine_create(int64_t p, int64_t npan, const double* sx, const double* snx,
+                      const double* sw, const double* rts, const double* rps, int64_t ichart,
+                      const double* sxbd, const double* rv, int64_t device_ptrs, void** engine);
+int qol_engine_destroy(void* engine);
+int qol_engine_set_targets(void* engine, int64_t nt, const double* tx, int64_t device_ptr);
+/* near list by the ball rule (stellarator_grf_core_mod.f90:652-695) */
+int qol_engine_nearlist(void* engine, int64_t* nnz);
+/* caller's CSR by panel: offs(npan+1) 0-based, idx(nnz) 1-based target indices */
+int qol_engine_set_pairs(void* engine, const int64_t* offs, const int64_t* idx);
+int qol_engine_get_pairs(void* engine, int64_t* offs, int64_t* idx);
+/* levels + RRQ weights for every pair.  Targets within side_tol*patch size of a
+ * patch take the exterior (1) / interior (0) limit; others their own side. */
+int qol_engine_eval(void* engine, int64_t exterior, double side_tol, int64_t want_omega);
+int qol_engine_sync(void* engine);
+/* stage times (ms) of the last eval: levels, classify, direct, rrq setup,
+ * rrq solve, rrq targets */
+int qol_engine_stage_ms(void* engine, double* ms);
+int qol_engine_get_weights(void* engine, double* As, double* Ad, int64_t* label, double* Omega,
+                           double* Ias);
+/* near-correction apply + fixed-order scatter into u (in: far field) */
+int qol_engine_apply(void* engine, const double* ub, const double* ubn, double* u, double* val);
+int qol_engine_apply_resident(void* engine);
+/* far field by direct summation (stellarator_grf_core_mod.f90:560-611) */
+int qol_engine_far(void* engine, const double* ub, const double* ubn, double* u);
+int qol_engine_level_data(void* engine, double* lgeo, double* balls);
+int qol_engine_triplets(void* engine, const int64_t* trow, int64_t* si, int64_t* sj, double* vs,
+                        double* vd);
+int qol_engine_stream(void* engine, void** stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -442,8 +442,12 @@ static void ssq_w1(const Panel& P, const Edge& E, const double* x, double* w, do
   }
 }
 
-/* solid angle int_S z.n_uv/|z|^3 dS via the Dirac-string potential, with
-   adaptive Gauss-Legendre bisection of each edge toward its near point */
+/* solid angle int_S z.n_uv/|z|^3 dS via the Dirac-string potential
+   A(y) = (q x z)/(|z|(|z| + q.z)), z = y - x, whose string x - s q leaves the
+   patch on the target's side.  Each edge is integrated left to right with
+   Gauss-Legendre panels that shrink geometrically toward the edge's near point
+   (accepted when the complex root lies outside the rho_thr Bernstein ellipse of
+   the panel) and grow again past it. */
 static double solid_angle(const Panel& P, const double* x, const double* q, double rho_thr) {
   i64 n = P.nq;
   double total = 0.0;
@@ -451,24 +455,19 @@ static double solid_angle(const Panel& P, const double* x, const double* q, doub
     const Edge& E = P.e[e];
     cplx s0;
     bool ok = edge_root(E, n, x, s0);
-    /* stack of intervals */
-    std::vector<std::pair<double, double>> st;
-    st.push_back({-1.0, 1.0});
+    double lo = -1.0, len = 2.0;
     int guard = 0;
-    while (!st.empty()) {
-      auto iv = st.back();
-      st.pop_back();
-      double cc = 0.5 * (iv.first + iv.second), hh = 0.5 * (iv.second - iv.first);
-      bool accept = true;
-      if (ok && hh > 1e-15) {
-        cplx t0 = (s0 - cc) / hh;
-        if (bernstein_rho(t0) < rho_thr) accept = false;
+    while (lo < 1.0 && guard < 512) {
+      ++guard;
+      double hi = std::min(1.0, lo + len);
+      if (ok) {
+        while (hi - lo > 1e-14) {
+          double cc = 0.5 * (lo + hi), hh = 0.5 * (hi - lo);
+          if (bernstein_rho((s0 - cc) / hh) >= rho_thr) break;
+          hi = lo + 0.5 * (hi - lo);
+        }
       }
-      if (!accept && ++guard < 4000) {
-        st.push_back({cc, iv.second});
-        st.push_back({iv.first, cc});
-        continue;
-      }
+      double cc = 0.5 * (lo + hi), hh = 0.5 * (hi - lo);
       for (i64 i = 0; i < n; ++i) {
         double s = cc + hh * P.tgl[i];
         double X[3], dX[3];
@@ -481,6 +480,8 @@ static double solid_angle(const Panel& P, const double* x, const double* q, doub
         double qzx = q[0] * zx[0] + q[1] * zx[1] + q[2] * zx[2];
         total += hh * P.wgl[i] * qzx / (rz * (rz + qz));
       }
+      len = 2.0 * (hi - lo);
+      lo = hi;
     }
   }
   return total;

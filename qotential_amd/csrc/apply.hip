@@ -1,0 +1,148 @@
+// Near-correction apply, deterministic scatter, far field and COO triplets.
+//
+// Reference: naive block + apply   stellarator_grf_core_mod.f90:847-870
+//            fixed-order scatter   stellarator_grf_core_mod.f90:895-899
+//            far field (direct)    stellarator_grf_core_mod.f90:560-611
+//            COO triplets          lap3d_close_mod.f90:935-956
+#include "qol_kernels.h"
+
+namespace qol {
+
+// val(pair) = sum_i (S - S_naive)_i ubn_i - (K - K_naive)_i ub_i, i over the
+// panel's h nodes, left to right as in the reference loop.
+__global__ void __launch_bounds__(256) naive_apply_kernel(int nnz, int h, const int* __restrict__ tidx,
+                                                          const int* __restrict__ ppanel, const double* __restrict__ tx,
+                                                          const double* __restrict__ sx, const double* __restrict__ snx,
+                                                          const double* __restrict__ sw, const double* __restrict__ ub,
+                                                          const double* __restrict__ ubn, const double* __restrict__ wout,
+                                                          double* __restrict__ val) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  const double pi4inv = 1.0 / (16.0 * atan(1.0));
+  const int t = tidx[j], k = ppanel[j];
+  const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
+  const double* w = wout + (size_t)j * 2 * h;
+  double acc = 0.0;
+  for (int i = 0; i < h; ++i) {
+    const size_t g = (size_t)k * h + i;
+    const double dx = x0 - sx[3 * g], dy = x1 - sx[3 * g + 1], dz = x2 - sx[3 * g + 2];
+    const double rr = dx * dx + dy * dy + dz * dz;
+    double an = 0.0, dn = 0.0;
+    if (rr > 1.0e-28) {
+      const double r = sqrt(rr);
+      an = pi4inv * sw[g] / r;
+      dn = pi4inv * sw[g] * (dx * snx[3 * g] + dy * snx[3 * g + 1] + dz * snx[3 * g + 2]) / (r * rr);
+    }
+    acc = acc + (w[i] - an) * ubn[g] - (w[h + i] - dn) * ub[g];
+  }
+  val[j] = acc;
+}
+
+// COO triplets of getnearquad_lap_rrq: out = (tcxi(k)-1)*h + (j-1)*m + i (1-based)
+__global__ void __launch_bounds__(256) triplet_kernel(int nnz, int h, const int* __restrict__ tidx,
+                                                      const int* __restrict__ ppanel, const int* __restrict__ trow,
+                                                      const int* __restrict__ offs, const double* __restrict__ tx,
+                                                      const double* __restrict__ sx, const double* __restrict__ snx,
+                                                      const double* __restrict__ sw, const double* __restrict__ wout,
+                                                      int64_t* __restrict__ si, int64_t* __restrict__ sj,
+                                                      double* __restrict__ vs, double* __restrict__ vd) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  const double pi4inv = 1.0 / (16.0 * atan(1.0));
+  const int t = tidx[j], k = ppanel[j];
+  const int m = offs[k + 1] - offs[k], i = j - offs[k];
+  const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
+  const double* w = wout + (size_t)j * 2 * h;
+  const int row = trow[t];  // 1-based source index of the target, 0 if none
+  for (int c = 0; c < h; ++c) {
+    const size_t g = (size_t)k * h + c;
+    const double dx = x0 - sx[3 * g], dy = x1 - sx[3 * g + 1], dz = x2 - sx[3 * g + 2];
+    const double rr = dx * dx + dy * dy + dz * dz;
+    double an = 0.0, dn = 0.0;
+    if (rr > 1.0e-28) {
+      const double r = sqrt(rr);
+      an = pi4inv * sw[g] / r;
+      dn = pi4inv * sw[g] * (dx * snx[3 * g] + dy * snx[3 * g + 1] + dz * snx[3 * g + 2]) / (r * rr);
+    }
+    if (row == (int)(g + 1)) { an = 0.0; dn = 0.0; }
+    const size_t o = (size_t)offs[k] * h + (size_t)c * m + i;
+    si[o] = row;
+    sj[o] = (int64_t)g + 1;
+    vs[o] = w[c] - an;
+    vd[o] = w[h + c] - dn;
+  }
+}
+
+// target-major pair lists (stable in pair order after the per-target sort)
+__global__ void target_count_kernel(int nnz, const int* __restrict__ tidx, int* __restrict__ tcount) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nnz) atomicAdd(&tcount[tidx[j]], 1);
+}
+
+__global__ void target_fill_kernel(int nnz, const int* __restrict__ tidx, const int* __restrict__ toffs,
+                                   int* __restrict__ tcur, int* __restrict__ tpairs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  const int t = tidx[j];
+  const int pos = atomicAdd(&tcur[t], 1);
+  tpairs[toffs[t] + pos] = j;
+}
+
+// u(t) += val(pairs of t) in pair (= panel) order, starting from the far field
+__global__ void target_sum_kernel(int nt, const int* __restrict__ toffs, int* __restrict__ tpairs,
+                                  const double* __restrict__ val, double* __restrict__ u) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int a = toffs[t], b = toffs[t + 1];
+  for (int i = a + 1; i < b; ++i) {  // insertion sort: lists are short
+    const int v = tpairs[i];
+    int j = i - 1;
+    while (j >= a && tpairs[j] > v) { tpairs[j + 1] = tpairs[j]; --j; }
+    tpairs[j + 1] = v;
+  }
+  double acc = u[t];
+  for (int i = a; i < b; ++i) acc = acc + val[tpairs[i]];
+  u[t] = acc;
+}
+
+// far field by direct summation (zero at coincident points), sources staged in LDS
+constexpr int kFT = 256, kFS = 256;
+__global__ void __launch_bounds__(256) far_direct_kernel(int nt, const double* __restrict__ tx, int ns,
+                                                         const double* __restrict__ sx, const double* __restrict__ snx,
+                                                         const double* __restrict__ sw, const double* __restrict__ ub,
+                                                         const double* __restrict__ ubn, double* __restrict__ u) {
+  __shared__ double s[8][kFS];
+  const double pi4inv = 1.0 / (16.0 * atan(1.0));
+  const int t = blockIdx.x * kFT + threadIdx.x;
+  double x0 = 0, x1 = 0, x2 = 0;
+  if (t < nt) { x0 = tx[3 * (size_t)t]; x1 = tx[3 * (size_t)t + 1]; x2 = tx[3 * (size_t)t + 2]; }
+  double acc = 0.0;
+  for (int j0 = 0; j0 < ns; j0 += kFS) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kFS; i += blockDim.x) {
+      const int j = j0 + i;
+      if (j < ns) {
+        s[0][i] = sx[3 * (size_t)j]; s[1][i] = sx[3 * (size_t)j + 1]; s[2][i] = sx[3 * (size_t)j + 2];
+        s[3][i] = snx[3 * (size_t)j]; s[4][i] = snx[3 * (size_t)j + 1]; s[5][i] = snx[3 * (size_t)j + 2];
+        s[6][i] = pi4inv * sw[j] * ubn[j];
+        s[7][i] = pi4inv * sw[j] * ub[j];
+      } else {
+        for (int c = 0; c < 8; ++c) s[c][i] = 0.0;
+        s[0][i] = 1e300;
+      }
+    }
+    __syncthreads();
+    const int jn = min(kFS, ns - j0);
+    for (int i = 0; i < jn; ++i) {
+      const double dx = x0 - s[0][i], dy = x1 - s[1][i], dz = x2 - s[2][i];
+      const double rr = dx * dx + dy * dy + dz * dz;
+      if (rr > 1.0e-28) {
+        const double ri = 1.0 / sqrt(rr);
+        acc += s[6][i] * ri - s[7][i] * (dx * s[3][i] + dy * s[4][i] + dz * s[5][i]) * ri * ri * ri;
+      }
+    }
+  }
+  if (t < nt) u[t] = acc;
+}
+
+}  // namespace qol

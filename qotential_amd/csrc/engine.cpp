@@ -1,0 +1,463 @@
+// Host engine: device buffers, per-order tables, kernel launches and the C-ABI
+// of the batched close-evaluation path.
+//
+// One engine holds one triangulated surface of patch order p (geometry resident
+// in HBM), a target set and its near list, and the per-pair close-evaluation
+// weights.  Every stage runs on the engine's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qol_engine.h"
+#include "qol_host.h"
+#include "qol_kernels.h"
+#include "qotential_hip.h"
+
+namespace qol {
+
+void set_status(int64_t s);
+void set_error(const std::string& s);
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess)                                                                      \
+      throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
+                               std::to_string(__LINE__) + ")");                                \
+  } while (0)
+
+DevBuf::~DevBuf() {
+  if (ptr) (void)hipFree(ptr);
+}
+void DevBuf::alloc(size_t b) {
+  if (b <= bytes && ptr) return;
+  if (ptr) { (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
+  if (b == 0) return;
+  HIPCHK(hipMalloc(&ptr, b));
+  bytes = b;
+}
+
+namespace {
+
+template <class T>
+void upload(DevBuf& d, const T* h, size_t n, hipStream_t s) {
+  d.alloc(n * sizeof(T));
+  if (n) HIPCHK(hipMemcpyAsync(d.ptr, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+int blocks(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+void set_lds(const void* f, size_t bytes) {
+  if (bytes > 64 * 1024) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  if (bytes > 160 * 1024) throw std::runtime_error("kernel needs more LDS than a CU has");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+DeviceTables::DeviceTables(i64 p, hipStream_t s) {
+  const OrderTables& T = order_tables(p);
+  h = (int)T.h;
+  ntot = (int)T.ntot;
+  nsubtot = (int)T.nsubtot;
+  std::vector<double> interp((size_t)T.h * T.ntot), wt(T.ntot);
+  std::vector<int> suboff, subtyp;
+  LA.h = h; LA.ntot = ntot; LA.nsubtot = nsubtot;
+  int off = 0, sb = 0;
+  for (int l = 0; l < 4; ++l) {
+    const LevelTable& L = T.lev[l];
+    LA.lev_n[l] = (int)L.n;
+    LA.lev_off[l] = off;
+    LA.lev_nsub[l] = (int)L.nsub;
+    LA.sub_base[l] = sb;
+    std::memcpy(interp.data() + (size_t)T.h * off, L.interpmat.data(), sizeof(double) * L.n * T.h);
+    std::memcpy(wt.data() + off, L.wt.data(), sizeof(double) * L.n);
+    for (i64 q = 0; q <= L.nsub; ++q) suboff.push_back((int)L.suboff[q]);
+    for (i64 q = 0; q < L.nsub; ++q) subtyp.push_back((int)L.subtyp[q]);
+    off += (int)L.n;
+    sb += (int)L.nsub;
+  }
+  upload(d_interp, interp.data(), interp.size(), s);
+  upload(d_wt, wt.data(), wt.size(), s);
+  upload(d_suboff, suboff.data(), suboff.size(), s);
+  upload(d_subtyp, subtyp.data(), subtyp.size(), s);
+  LA.interp = (const double*)d_interp.ptr;
+  LA.wt = (const double*)d_wt.ptr;
+  LA.suboff = (const int*)d_suboff.ptr;
+  LA.subtyp = (const int*)d_subtyp.ptr;
+  LA.dist2 = 1.4 * 1.4;
+  upload(d_tgl, T.tgl.data(), T.tgl.size(), s);
+  upload(d_wgl, T.wgl.data(), T.wgl.size(), s);
+  upload(d_Leg, T.Legmat.data(), T.Legmat.size(), s);
+  upload(d_umatr, T.umatr.data(), T.umatr.size(), s);
+  upload(d_uvn, T.uvn.data(), T.uvn.size(), s);
+  RT.p = (int)T.p; RT.h = h; RT.nq = (int)T.nq; RT.ncoef = (int)T.ncoef;
+  RT.tgl = (const double*)d_tgl.ptr;
+  RT.wgl = (const double*)d_wgl.ptr;
+  RT.Legmat = (const double*)d_Leg.ptr;
+  RT.umatr = (const double*)d_umatr.ptr;
+  RT.uvn = (const double*)d_uvn.ptr;
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const double* sw, const double* rts,
+               const double* rps, i64 ichart_, const double* sxbd, const double* rv, bool device_ptrs)
+    : p((int)p_), npan((int)npan_), ichart((int)ichart_) {
+  if (p < 2 || p > QOL_MAXP) throw std::runtime_error("patch order out of range for the device kernels");
+  if (npan <= 0) throw std::runtime_error("npan must be positive");
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  tab = std::make_unique<DeviceTables>(p, stream);
+  h = tab->h;
+  nq = p + 2;
+  ncoef = (p + 1) * (p + 2) / 2;
+  nsrc = (size_t)npan * h;
+  const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  auto put = [&](DevBuf& d, const double* a, size_t n) {
+    d.alloc(n * sizeof(double));
+    HIPCHK(hipMemcpyAsync(d.ptr, a, n * sizeof(double), kind, stream));
+  };
+  put(d_sx, sx, 3 * nsrc);
+  put(d_snx, snx, 3 * nsrc);
+  put(d_sw, sw, nsrc);
+  put(d_rts, rts, 3 * nsrc);
+  put(d_rps, rps, 3 * nsrc);
+  if (ichart == 1) {
+    if (!sxbd || !rv) throw std::runtime_error("ichart=1 needs the chart boundary and vertices");
+    put(d_sxbd, sxbd, (size_t)npan * 9 * nq);
+    put(d_rv, rv, (size_t)npan * 9);
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+}
+
+Engine::~Engine() {
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::set_targets(i64 nt_, const double* tx, bool device_ptr) {
+  nt = (int)nt_;
+  d_tx.alloc(sizeof(double) * 3 * (size_t)(nt > 0 ? nt : 1));
+  if (nt > 0)
+    HIPCHK(hipMemcpyAsync(d_tx.ptr, tx, sizeof(double) * 3 * nt, device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          stream));
+  nnz = -1;
+  evaluated = false;
+}
+
+void Engine::alloc_pairs() {
+  d_label.alloc((size_t)(nnz > 0 ? nnz : 1));
+  d_ppanel.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  d_wout.alloc(sizeof(double) * 2 * (size_t)h * (nnz > 0 ? nnz : 1));
+  d_val.alloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+  d_nrrq.alloc(sizeof(int) * npan);
+  d_lcount.alloc(sizeof(int) * 5 * npan);
+}
+
+i64 Engine::nearlist() {
+  if (nt < 0) throw std::runtime_error("targets not set");
+  d_nball.alloc(sizeof(double) * 4 * npan);
+  d_cnt.alloc(sizeof(int) * npan);
+  d_offs.alloc(sizeof(int) * (npan + 1));
+  HIPCHK(hipMemsetAsync(d_cnt.ptr, 0, sizeof(int) * npan, stream));
+  hipLaunchKernelGGL(nearlist_prep_kernel, dim3(blocks(npan, 256)), dim3(256), 0, stream, npan, h,
+                     (const double*)d_sx.ptr, (const double*)d_sw.ptr, (double*)d_nball.ptr);
+  const int pb = 16;
+  const int gy = nt > 0 ? std::min(64, blocks(nt, 256)) : 1;
+  if (nt > 0)
+    hipLaunchKernelGGL(nearlist_count_kernel, dim3(blocks(npan, pb), gy), dim3(256), 0, stream, npan, nt,
+                       (const double*)d_tx.ptr, (const double*)d_nball.ptr, (int*)d_cnt.ptr);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, (const int*)d_cnt.ptr, npan, (int*)d_offs.ptr);
+  int tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, (int*)d_offs.ptr + npan, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  nnz = tot;
+  d_tidx.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  if (nt > 0 && nnz > 0)
+    hipLaunchKernelGGL(nearlist_fill_kernel, dim3(blocks(npan, pb)), dim3(256), 0, stream, npan, nt,
+                       (const double*)d_tx.ptr, (const double*)d_nball.ptr, (const int*)d_offs.ptr, (int*)d_tidx.ptr);
+  HIPCHK(hipGetLastError());
+  alloc_pairs();
+  evaluated = false;
+  return nnz;
+}
+
+void Engine::set_pairs(const i64* offs, const i64* idx1) {
+  std::vector<int> o(npan + 1);
+  for (int k = 0; k <= npan; ++k) o[k] = (int)offs[k];
+  if (o[0] != 0) throw std::runtime_error("pair offsets must start at 0");
+  for (int k = 0; k < npan; ++k)
+    if (o[k + 1] < o[k]) throw std::runtime_error("pair offsets must be non-decreasing");
+  nnz = o[npan];
+  std::vector<int> t((size_t)(nnz > 0 ? nnz : 1));
+  for (i64 j = 0; j < nnz; ++j) {
+    if (idx1[j] < 1 || idx1[j] > nt) throw std::runtime_error("pair target index out of range");
+    t[j] = (int)(idx1[j] - 1);
+  }
+  upload(d_offs, o.data(), o.size(), stream);
+  upload(d_tidx, t.data(), t.size(), stream);
+  HIPCHK(hipStreamSynchronize(stream));
+  alloc_pairs();
+  evaluated = false;
+}
+
+void Engine::get_pairs(i64* offs, i64* idx1) {
+  std::vector<int> o(npan + 1), t((size_t)(nnz > 0 ? nnz : 1));
+  HIPCHK(hipMemcpyAsync(o.data(), d_offs.ptr, sizeof(int) * (npan + 1), hipMemcpyDeviceToHost, stream));
+  if (nnz > 0) HIPCHK(hipMemcpyAsync(t.data(), d_tidx.ptr, sizeof(int) * nnz, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  for (int k = 0; k <= npan; ++k) offs[k] = o[k];
+  for (i64 j = 0; j < nnz; ++j) idx1[j] = (i64)t[j] + 1;
+}
+
+void Engine::build_levels() {
+  if (levels_built) return;
+  d_lgeo.alloc(sizeof(double) * 7 * (size_t)tab->ntot * npan);
+  d_balls.alloc(sizeof(double) * 4 * (size_t)tab->nsubtot * npan);
+  const int nmax = std::max(tab->LA.lev_n[2], tab->LA.lev_n[3]);
+  const size_t lds = sizeof(double) * (9 * (size_t)h + 4 * (size_t)nmax);
+  set_lds((const void*)level_geom_kernel, lds);
+  hipLaunchKernelGGL(level_geom_kernel, dim3(npan), dim3(256), lds, stream, tab->LA, (const double*)d_sx.ptr,
+                     (const double*)d_rts.ptr, (const double*)d_rps.ptr, (double*)d_lgeo.ptr, (double*)d_balls.ptr);
+  HIPCHK(hipGetLastError());
+  levels_built = true;
+}
+
+void Engine::alloc_rrq() {
+  if (rrq_alloc) return;
+  const size_t np_ = npan, nb = 3 * (size_t)nq;
+  d_frame.alloc(sizeof(double) * 16 * np_);
+  d_edge.alloc(sizeof(double) * 9 * nq * np_);
+  d_bX.alloc(sizeof(double) * 6 * nb * np_);
+  d_cx.alloc(sizeof(double) * 3 * (size_t)h * np_);
+  d_basis.alloc(sizeof(double) * 12 * nb * ncoef * np_);
+  d_MT.alloc(sizeof(double) * 16 * (size_t)h * h * np_);
+  d_piv.alloc(sizeof(int) * 5 * (size_t)h * np_);
+  d_Msl.alloc(sizeof(double) * (size_t)h * h * np_);
+  d_Fn.alloc(sizeof(double) * (size_t)h * h * np_);
+  d_G.alloc(sizeof(double) * 10 * (size_t)h * h * np_);
+  d_scr.alloc(sizeof(double) * 9 * (size_t)h * h * np_);
+  RP.frame = (double*)d_frame.ptr; RP.edge = (double*)d_edge.ptr; RP.bX = (double*)d_bX.ptr;
+  RP.cx = (double*)d_cx.ptr; RP.basis = (double*)d_basis.ptr; RP.MT = (double*)d_MT.ptr;
+  RP.piv = (int*)d_piv.ptr; RP.Msl = (double*)d_Msl.ptr; RP.Fn = (double*)d_Fn.ptr;
+  RP.G = (double*)d_G.ptr; RP.scr = (double*)d_scr.ptr;
+  rrq_alloc = true;
+}
+
+void Engine::eval_levels_only() {
+  if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
+  build_levels();
+  if (nnz > 0) {
+    const size_t lds_b = sizeof(double) * 4 * (size_t)tab->nsubtot;
+    hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, (const int*)d_offs.ptr,
+                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_balls.ptr,
+                       (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr, (int*)d_lcount.ptr);
+    const size_t lds = direct_level_smem(h);
+    set_lds((const void*)direct_level_kernel, lds);
+    for (int l = 0; l < 4; ++l)
+      hipLaunchKernelGGL(direct_level_kernel, dim3(npan), dim3(256), lds, stream, tab->LA, l, (const int*)d_offs.ptr,
+                         (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const int8_t*)d_label.ptr,
+                         (const double*)d_lgeo.ptr, (double*)d_wout.ptr);
+  }
+  HIPCHK(hipGetLastError());
+  evaluated = true;
+  have_omega = false;
+}
+
+void Engine::eval(int exterior, double side_tol, bool want_omega) {
+  if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
+  if (p > 12) throw std::runtime_error("the RRQ kernels support patch orders up to 12");
+  HIPCHK(hipEventRecord(ev[0], stream));
+  build_levels();
+  HIPCHK(hipEventRecord(ev[1], stream));
+  if (nnz > 0) {
+    const size_t lds_b = sizeof(double) * 4 * (size_t)tab->nsubtot;
+    hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, (const int*)d_offs.ptr,
+                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_balls.ptr,
+                       (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr, (int*)d_lcount.ptr);
+  } else {
+    HIPCHK(hipMemsetAsync(d_nrrq.ptr, 0, sizeof(int) * npan, stream));
+    HIPCHK(hipMemsetAsync(d_lcount.ptr, 0, sizeof(int) * 5 * npan, stream));
+  }
+  HIPCHK(hipEventRecord(ev[2], stream));
+  if (nnz > 0) {
+    const size_t lds = direct_level_smem(h);
+    set_lds((const void*)direct_level_kernel, lds);
+    for (int l = 0; l < 4; ++l)
+      hipLaunchKernelGGL(direct_level_kernel, dim3(npan), dim3(256), lds, stream, tab->LA, l, (const int*)d_offs.ptr,
+                         (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const int8_t*)d_label.ptr,
+                         (const double*)d_lgeo.ptr, (double*)d_wout.ptr);
+  }
+  HIPCHK(hipEventRecord(ev[3], stream));
+  if (nnz > 0) {
+    alloc_rrq();
+    const size_t lds1 = rrq_setup_smem(h, nq, ncoef);
+    set_lds((const void*)rrq_panel_setup_kernel, lds1);
+    hipLaunchKernelGGL(rrq_panel_setup_kernel, dim3(npan), dim3(256), lds1, stream, tab->RT, RP, npan,
+                       (const int*)d_nrrq.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
+                       (const double*)d_sw.ptr, ichart, (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
+    HIPCHK(hipEventRecord(ev[4], stream));
+    const size_t lds2 = sizeof(double) * 4 * (size_t)h;
+    hipLaunchKernelGGL(rrq_panel_lu_kernel, dim3(npan), dim3(256), lds2, stream, tab->RT, RP, npan,
+                       (const int*)d_nrrq.ptr);
+    hipLaunchKernelGGL(rrq_panel_weights_kernel, dim3(npan), dim3(256), 0, stream, tab->RT, RP, npan,
+                       (const int*)d_nrrq.ptr);
+    HIPCHK(hipEventRecord(ev[5], stream));
+    if (want_omega) {
+      d_omega.alloc(sizeof(double) * 4 * (size_t)h * nnz);
+      d_ias.alloc(sizeof(double) * (size_t)nnz);
+      HIPCHK(hipMemsetAsync(d_omega.ptr, 0, sizeof(double) * 4 * (size_t)h * nnz, stream));
+      HIPCHK(hipMemsetAsync(d_ias.ptr, 0, sizeof(double) * (size_t)nnz, stream));
+    }
+    const size_t lds3 = rrq_target_smem(h, nq, ncoef);
+    set_lds((const void*)rrq_target_kernel, lds3);
+    hipLaunchKernelGGL(rrq_target_kernel, dim3(npan), dim3(256), lds3, stream, tab->RT, RP, (const int*)d_offs.ptr,
+                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const int8_t*)d_label.ptr,
+                       (const int*)d_nrrq.ptr, exterior, side_tol, (double*)d_wout.ptr,
+                       want_omega ? (double*)d_omega.ptr : nullptr, want_omega ? (double*)d_ias.ptr : nullptr);
+  } else {
+    HIPCHK(hipEventRecord(ev[4], stream));
+    HIPCHK(hipEventRecord(ev[5], stream));
+  }
+  HIPCHK(hipEventRecord(ev[6], stream));
+  HIPCHK(hipGetLastError());
+  evaluated = true;
+  have_omega = want_omega;
+}
+
+void Engine::stage_ms(double* ms) {
+  HIPCHK(hipEventSynchronize(ev[6]));
+  for (int i = 0; i < 6; ++i) {
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+    ms[i] = t;
+  }
+}
+
+void Engine::get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias) {
+  if (!evaluated) throw std::runtime_error("weights not evaluated");
+  HIPCHK(hipStreamSynchronize(stream));
+  std::vector<double> w(2 * (size_t)h * (nnz > 0 ? nnz : 1));
+  if (nnz > 0) HIPCHK(hipMemcpy(w.data(), d_wout.ptr, sizeof(double) * 2 * h * (size_t)nnz, hipMemcpyDeviceToHost));
+  // column-major (nnz, h)
+  for (i64 j = 0; j < nnz; ++j)
+    for (int c = 0; c < h; ++c) {
+      if (As) As[j + (size_t)nnz * c] = w[(size_t)j * 2 * h + c];
+      if (Ad) Ad[j + (size_t)nnz * c] = w[(size_t)j * 2 * h + h + c];
+    }
+  if (label && nnz > 0) {
+    std::vector<int8_t> l(nnz);
+    HIPCHK(hipMemcpy(l.data(), d_label.ptr, nnz, hipMemcpyDeviceToHost));
+    for (i64 j = 0; j < nnz; ++j) label[j] = l[j];
+  }
+  if (Omega && nnz > 0) {
+    if (!have_omega) throw std::runtime_error("Omega was not requested at eval");
+    HIPCHK(hipMemcpy(Omega, d_omega.ptr, sizeof(double) * 4 * (size_t)h * nnz, hipMemcpyDeviceToHost));
+  }
+  if (Ias && nnz > 0) {
+    if (!have_omega) throw std::runtime_error("Ias was not requested at eval");
+    HIPCHK(hipMemcpy(Ias, d_ias.ptr, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost));
+  }
+}
+
+void Engine::apply(const double* ub, const double* ubn, double* u, double* val, bool device_ptrs) {
+  if (!evaluated) throw std::runtime_error("weights not evaluated");
+  const hipMemcpyKind kin = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind kout = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  d_ub.alloc(sizeof(double) * nsrc);
+  d_ubn.alloc(sizeof(double) * nsrc);
+  d_u.alloc(sizeof(double) * (size_t)(nt > 0 ? nt : 1));
+  HIPCHK(hipMemcpyAsync(d_ub.ptr, ub, sizeof(double) * nsrc, kin, stream));
+  HIPCHK(hipMemcpyAsync(d_ubn.ptr, ubn, sizeof(double) * nsrc, kin, stream));
+  if (nt > 0) HIPCHK(hipMemcpyAsync(d_u.ptr, u, sizeof(double) * nt, kin, stream));
+  apply_device();
+  if (nt > 0) HIPCHK(hipMemcpyAsync(u, d_u.ptr, sizeof(double) * nt, kout, stream));
+  if (val && nnz > 0) HIPCHK(hipMemcpyAsync(val, d_val.ptr, sizeof(double) * nnz, kout, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+}
+
+void Engine::build_target_lists() {
+  if (tlists_nnz == nnz) return;
+  d_tcount.alloc(sizeof(int) * (size_t)(nt > 0 ? nt : 1));
+  d_toffs.alloc(sizeof(int) * (size_t)(nt + 1));
+  d_tcur.alloc(sizeof(int) * (size_t)(nt > 0 ? nt : 1));
+  d_tpairs.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  HIPCHK(hipMemsetAsync(d_tcount.ptr, 0, sizeof(int) * nt, stream));
+  HIPCHK(hipMemsetAsync(d_tcur.ptr, 0, sizeof(int) * nt, stream));
+  hipLaunchKernelGGL(target_count_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz,
+                     (const int*)d_tidx.ptr, (int*)d_tcount.ptr);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, (const int*)d_tcount.ptr, nt, (int*)d_toffs.ptr);
+  hipLaunchKernelGGL(target_fill_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz,
+                     (const int*)d_tidx.ptr, (const int*)d_toffs.ptr, (int*)d_tcur.ptr, (int*)d_tpairs.ptr);
+  HIPCHK(hipGetLastError());
+  tlists_nnz = nnz;
+}
+
+void Engine::apply_device() {
+  if (nnz <= 0) return;
+  hipLaunchKernelGGL(naive_apply_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz, h,
+                     (const int*)d_tidx.ptr, (const int*)d_ppanel.ptr, (const double*)d_tx.ptr,
+                     (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,
+                     (const double*)d_ub.ptr, (const double*)d_ubn.ptr, (const double*)d_wout.ptr, (double*)d_val.ptr);
+  build_target_lists();
+  hipLaunchKernelGGL(target_sum_kernel, dim3(blocks(nt, 256)), dim3(256), 0, stream, nt, (const int*)d_toffs.ptr,
+                     (int*)d_tpairs.ptr, (const double*)d_val.ptr, (double*)d_u.ptr);
+  HIPCHK(hipGetLastError());
+}
+
+void Engine::far(const double* ub, const double* ubn, double* u) {
+  d_ub.alloc(sizeof(double) * nsrc);
+  d_ubn.alloc(sizeof(double) * nsrc);
+  d_u.alloc(sizeof(double) * (size_t)(nt > 0 ? nt : 1));
+  HIPCHK(hipMemcpyAsync(d_ub.ptr, ub, sizeof(double) * nsrc, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(d_ubn.ptr, ubn, sizeof(double) * nsrc, hipMemcpyHostToDevice, stream));
+  if (nt > 0)
+    hipLaunchKernelGGL(far_direct_kernel, dim3(blocks(nt, 256)), dim3(256), 0, stream, nt, (const double*)d_tx.ptr,
+                       (int)nsrc, (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,
+                       (const double*)d_ub.ptr, (const double*)d_ubn.ptr, (double*)d_u.ptr);
+  HIPCHK(hipGetLastError());
+  if (nt > 0) HIPCHK(hipMemcpyAsync(u, d_u.ptr, sizeof(double) * nt, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+}
+
+void Engine::level_data(double* lgeo, double* balls) {
+  build_levels();
+  HIPCHK(hipStreamSynchronize(stream));
+  if (lgeo) HIPCHK(hipMemcpy(lgeo, d_lgeo.ptr, sizeof(double) * 7 * (size_t)tab->ntot * npan, hipMemcpyDeviceToHost));
+  if (balls) HIPCHK(hipMemcpy(balls, d_balls.ptr, sizeof(double) * 4 * (size_t)tab->nsubtot * npan, hipMemcpyDeviceToHost));
+}
+
+void Engine::triplets(const i64* trow, i64* si, i64* sj, double* vs, double* vd) {
+  if (!evaluated) throw std::runtime_error("weights not evaluated");
+  std::vector<int> tr((size_t)(nt > 0 ? nt : 1), 0);
+  for (int t = 0; t < nt; ++t) tr[t] = (int)trow[t];
+  DevBuf d_tr, d_si, d_sj, d_vs, d_vd;
+  upload(d_tr, tr.data(), tr.size(), stream);
+  const size_t n = (size_t)nnz * h;
+  d_si.alloc(sizeof(int64_t) * (n ? n : 1)); d_sj.alloc(sizeof(int64_t) * (n ? n : 1));
+  d_vs.alloc(sizeof(double) * (n ? n : 1)); d_vd.alloc(sizeof(double) * (n ? n : 1));
+  if (nnz > 0)
+    hipLaunchKernelGGL(triplet_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz, h,
+                       (const int*)d_tidx.ptr, (const int*)d_ppanel.ptr, (const int*)d_tr.ptr, (const int*)d_offs.ptr,
+                       (const double*)d_tx.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
+                       (const double*)d_sw.ptr, (const double*)d_wout.ptr, (int64_t*)d_si.ptr, (int64_t*)d_sj.ptr,
+                       (double*)d_vs.ptr, (double*)d_vd.ptr);
+  HIPCHK(hipGetLastError());
+  if (n) {
+    HIPCHK(hipMemcpyAsync(si, d_si.ptr, sizeof(int64_t) * n, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(sj, d_sj.ptr, sizeof(int64_t) * n, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(vs, d_vs.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(vd, d_vd.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+}
+
+}  // namespace qol

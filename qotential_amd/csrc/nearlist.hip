@@ -1,0 +1,123 @@
+// Near list by the reference's ball rule, CSR by panel with targets ascending.
+//
+// Reference: stellarator_grf_core_mod.f90:652-695 (count, then fill) and
+// test/stellarator_bench.f90:88-94: ball centre = mean of the panel's nodes,
+// radius 1.75*sqrt(sum sw), target kept when |x - c|^2 < r^2.
+// Brute force over (panel, target) like the reference, but kPB panels per
+// workgroup share every target load, and the fill keeps ascending target order
+// with wavefront ballots so the CSR is bit-identical to the serial loop.
+#include "qol_kernels.h"
+
+namespace qol {
+
+constexpr int kPB = 16;  // panels per workgroup
+
+// ball[k] = (c0, c1, c2, r^2), summed in the reference's order
+__global__ void nearlist_prep_kernel(int npan, int h, const double* __restrict__ sx, const double* __restrict__ sw,
+                                     double* __restrict__ ball) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npan) return;
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0, s = 0.0;
+  for (int i = 0; i < h; ++i) {
+    const size_t g = (size_t)k * h + i;
+    q0 += sx[3 * g];
+    q1 += sx[3 * g + 1];
+    q2 += sx[3 * g + 2];
+  }
+  for (int i = 0; i < h; ++i) s += sw[(size_t)k * h + i];
+  const double qr = 1.75 * sqrt(s);
+  ball[4 * k] = q0 / (double)h;
+  ball[4 * k + 1] = q1 / (double)h;
+  ball[4 * k + 2] = q2 / (double)h;
+  ball[4 * k + 3] = qr * qr;
+}
+
+__device__ __forceinline__ bool in_ball(const double* b, double x0, double x1, double x2) {
+  const double d0 = x0 - b[0], d1 = x1 - b[1], d2 = x2 - b[2];
+  return d0 * d0 + d1 * d1 + d2 * d2 < b[3];
+}
+
+__global__ void __launch_bounds__(256) nearlist_count_kernel(int npan, int nt, const double* __restrict__ tx,
+                                                             const double* __restrict__ ball, int* __restrict__ count) {
+  __shared__ double sb[4 * kPB];
+  __shared__ int sc[kPB];
+  const int k0 = blockIdx.x * kPB;
+  if (threadIdx.x < 4 * kPB) sb[threadIdx.x] = (k0 + threadIdx.x / 4 < npan) ? ball[4 * k0 + threadIdx.x] : 0.0;
+  if (threadIdx.x < kPB) sc[threadIdx.x] = 0;
+  __syncthreads();
+  int my[kPB];
+  for (int q = 0; q < kPB; ++q) my[q] = 0;
+  for (int t = blockIdx.y * blockDim.x + threadIdx.x; t < nt; t += gridDim.y * blockDim.x) {
+    const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
+    for (int q = 0; q < kPB; ++q) my[q] += in_ball(sb + 4 * q, x0, x1, x2) ? 1 : 0;
+  }
+  for (int q = 0; q < kPB; ++q) {
+    int v = my[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&sc[q], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < kPB && k0 + threadIdx.x < npan && sc[threadIdx.x]) atomicAdd(&count[k0 + threadIdx.x], sc[threadIdx.x]);
+}
+
+// one workgroup per kPB panels, targets in ascending chunks of blockDim
+__global__ void __launch_bounds__(256) nearlist_fill_kernel(int npan, int nt, const double* __restrict__ tx,
+                                                            const double* __restrict__ ball,
+                                                            const int* __restrict__ offs, int* __restrict__ idx) {
+  __shared__ double sb[4 * kPB];
+  __shared__ int base[kPB];
+  __shared__ int wcnt[kPB][4];
+  const int k0 = blockIdx.x * kPB;
+  if (threadIdx.x < 4 * kPB) sb[threadIdx.x] = (k0 + threadIdx.x / 4 < npan) ? ball[4 * k0 + threadIdx.x] : 0.0;
+  if (threadIdx.x < kPB) base[threadIdx.x] = (k0 + threadIdx.x < npan) ? offs[k0 + threadIdx.x] : 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int t0 = 0; t0 < nt; t0 += blockDim.x) {
+    const int t = t0 + threadIdx.x;
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    if (t < nt) { x0 = tx[3 * (size_t)t]; x1 = tx[3 * (size_t)t + 1]; x2 = tx[3 * (size_t)t + 2]; }
+    uint64_t bal[kPB];
+    for (int q = 0; q < kPB; ++q) {
+      const bool in = (t < nt) && (k0 + q < npan) && in_ball(sb + 4 * q, x0, x1, x2);
+      bal[q] = __ballot(in);
+      if (lane == 0) wcnt[q][wv] = __popcll(bal[q]);
+    }
+    __syncthreads();
+    for (int q = 0; q < kPB; ++q) {
+      if ((bal[q] >> lane) & 1ull) {
+        int pos = base[q];
+        for (int w = 0; w < wv; ++w) pos += wcnt[q][w];
+        pos += __popcll(bal[q] & ((1ull << lane) - 1ull));
+        idx[pos] = t;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kPB) {
+      int s = 0;
+      for (int w = 0; w < 4; ++w) s += wcnt[threadIdx.x][w];
+      base[threadIdx.x] += s;
+    }
+    __syncthreads();
+  }
+}
+
+// exclusive scan of n ints into out[0..n] with a single workgroup of 1024 threads
+__global__ void __launch_bounds__(1024) scan_kernel(const int* __restrict__ in, int n, int* __restrict__ out) {
+  __shared__ int part[1024];
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int a = threadIdx.x * per, b = min(n, a + per);
+  int s = 0;
+  for (int i = a; i < b; ++i) s += in[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) { const int v = part[i]; part[i] = acc; acc += v; }
+    out[n] = acc;
+  }
+  __syncthreads();
+  int acc = part[threadIdx.x];
+  for (int i = a; i < b; ++i) { out[i] = acc; acc += in[i]; }
+}
+
+}  // namespace qol

@@ -1,0 +1,86 @@
+// Kernel interfaces shared by the HIP translation units and the host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qol_device.h"
+
+namespace qol {
+
+// per-order level tables on the device
+struct LevelArgs {
+  int h, ntot, nsubtot;
+  int lev_n[4], lev_off[4], lev_nsub[4], sub_base[4];
+  const double* interp;  // level l at interp + h*lev_off[l], column-major lev_n[l] x h
+  const double* wt;      // ntot
+  const int* suboff;     // per level (nsub+1) node offsets, concatenated
+  const int* subtyp;     // nsubtot
+  double dist2;
+};
+
+// per-order RRQ tables on the device
+struct RRQTables {
+  int p, h, nq, ncoef;
+  const double* tgl;     // nq
+  const double* wgl;     // nq
+  const double* Legmat;  // nq x nq (coef = Legmat * vals)
+  const double* umatr;   // h x h
+  const double* uvn;     // 2 x h VR nodes
+};
+
+// per-panel RRQ data on the device (one slot per panel)
+struct RRQPanels {
+  double* frame;   // [npan][16]: R(9) c(3) alpha sigma L pad
+  double* edge;    // [npan][3][3][nq] Legendre coefficients of the frame boundary
+  double* bX;      // [npan][3*nq][6] boundary X, dX (frame)
+  double* cx;      // [npan][3][h] Koornwinder coefficients of the geometry (original coords)
+  double* basis;   // [npan][3*nq][ncoef][12] grad H (3 complex) and V (3 complex)
+  double* MT;      // [npan][4h][4h] M^T, LU-factored in place
+  int* piv;        // [npan][5h] pivots of M^T (4h) and Msl^T (h)
+  double* Msl;     // [npan][h][h] Msl^T, LU-factored in place
+  double* Fn;      // [npan][h][h] F_k at the nodes, F(i,k)
+  double* G;       // [npan][2h][5h] weights operator
+  double* scr;     // [npan][9h^2] scratch for the weights operator
+};
+
+size_t direct_level_smem(int h);
+
+__global__ void level_geom_kernel(LevelArgs A, const double* sx, const double* rts, const double* rps,
+                                  double* lgeo, double* balls);
+__global__ void classify_kernel(LevelArgs A, const int* offs, const int* tidx, const double* tx,
+                                const double* balls, int8_t* label, int* ppanel, int* nrrq, int* lcount);
+__global__ void direct_level_kernel(LevelArgs A, int l, const int* offs, const int* tidx, const double* tx,
+                                    const int8_t* label, const double* lgeo, double* wout);
+
+__global__ void nearlist_prep_kernel(int npan, int h, const double* sx, const double* sw, double* ball);
+__global__ void nearlist_count_kernel(int npan, int nt, const double* tx, const double* ball, int* count);
+__global__ void nearlist_fill_kernel(int npan, int nt, const double* tx, const double* ball, const int* offs,
+                                     int* idx);
+__global__ void scan_kernel(const int* in, int n, int* out);
+
+__global__ void naive_apply_kernel(int nnz, int h, const int* tidx, const int* ppanel, const double* tx,
+                                   const double* sx, const double* snx, const double* sw, const double* ub,
+                                   const double* ubn, const double* wout, double* val);
+__global__ void target_count_kernel(int nnz, const int* tidx, int* tcount);
+__global__ void target_fill_kernel(int nnz, const int* tidx, const int* toffs, int* tcur, int* tpairs);
+__global__ void target_sum_kernel(int nt, const int* toffs, int* tpairs, const double* val, double* u);
+__global__ void far_direct_kernel(int nt, const double* tx, int ns, const double* sx, const double* snx,
+                                  const double* sw, const double* ub, const double* ubn, double* u);
+__global__ void triplet_kernel(int nnz, int h, const int* tidx, const int* ppanel, const int* trow,
+                               const int* offs, const double* tx, const double* sx, const double* snx,
+                               const double* sw, const double* wout, int64_t* si, int64_t* sj, double* vs,
+                               double* vd);
+
+// RRQ
+__global__ void rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq, const double* sx,
+                                       const double* snx, const double* sw, int ichart, const double* sxbd_chart,
+                                       const double* rv_chart);
+__global__ void rrq_panel_lu_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq);
+__global__ void rrq_panel_weights_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq);
+__global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* offs, const int* tidx, const double* tx,
+                                  const int8_t* label, const int* nrrq, int exterior, double side_tol,
+                                  double* wout, double* omega_out, double* ias_out);
+size_t rrq_setup_smem(int h, int nq, int ncoef);
+size_t rrq_target_smem(int h, int nq, int ncoef);
+
+}  // namespace qol

@@ -1,0 +1,858 @@
+// Recursive-reduction quadrature (RRQ) for the targets no refinement level can
+// serve: per-panel set-up, the quaternion system and its weights operator, and
+// the per-target boundary moments, recursion and weights.
+//
+// Reference: rrq_r64                       lap3d_close_mod.f90:622-881
+//            build_closepanel_precomp_r64  lap3d_close_mod.f90:472-620
+//            build_tc_chialpha_r64         lap3d_close_mod.f90:336-470
+//            (SSQ weights, solid angle, omega moments: absent LQ/QA modules,
+//             re-derived -- see the oracle header for the identities used)
+// FP64 throughout.
+#include "qol_kernels.h"
+
+namespace qol {
+
+// ---------------------------------------------------------------------------
+// geometry polynomial x(u,v) = sum_k K_k(u,v) cx[c][k] and its uv derivatives,
+// accumulated while the Koornwinder basis is generated (no h-sized arrays)
+__device__ void geom_eval(const double* cx, int h, int korder, double u, double v, double* x, double* xu, double* xv) {
+  const int N = korder + 1;
+  double Q[QOL_MAXP + 1], Qu[QOL_MAXP + 1], Qv[QOL_MAXP + 1];
+  const double a = 2.0 * u + v - 1.0, b = 1.0 - v, bb = b * b;
+  Q[0] = 1.0; Qu[0] = 0.0; Qv[0] = 0.0;
+  if (N > 1) { Q[1] = a; Qu[1] = 2.0; Qv[1] = 1.0; }
+  for (int m = 1; m + 1 < N; ++m) {
+    const double c1 = 2.0 * m + 1.0, c2 = (double)m, inv = 1.0 / (double)(m + 1);
+    Q[m + 1] = (c1 * a * Q[m] - c2 * bb * Q[m - 1]) * inv;
+    Qu[m + 1] = (c1 * (2.0 * Q[m] + a * Qu[m]) - c2 * bb * Qu[m - 1]) * inv;
+    Qv[m + 1] = (c1 * (Q[m] + a * Qv[m]) - c2 * (bb * Qv[m - 1] - 2.0 * b * Q[m - 1])) * inv;
+  }
+  for (int c = 0; c < 3; ++c) { x[c] = 0.0; if (xu) { xu[c] = 0.0; xv[c] = 0.0; } }
+  const double xx = 2.0 * v - 1.0;
+  int idx = 0;
+  for (int d = 0; d < N; ++d) {
+    for (int bd = 0; bd <= d; ++bd) {
+      const int am = d - bd;
+      const double al = 2.0 * am + 1.0;
+      double p = 1.0, dp = 0.0;
+      if (bd >= 1) {
+        double pm = 1.0, dpm = 0.0;
+        p = 0.5 * ((al + 2.0) * xx + al);
+        dp = al + 2.0;
+        for (int k = 1; k < bd; ++k) {
+          const double dk = (double)k;
+          const double a1 = 2.0 * (dk + 1.0) * (dk + al + 1.0) * (2.0 * dk + al);
+          const double a2 = (2.0 * dk + al + 1.0) * al * al;
+          const double a3 = (2.0 * dk + al) * (2.0 * dk + al + 1.0) * (2.0 * dk + al + 2.0);
+          const double a4 = 2.0 * (dk + al) * dk * (2.0 * dk + al + 2.0);
+          const double pn = ((a2 + a3 * xx) * p - a4 * pm) / a1;
+          const double dpn = ((a2 + a3 * xx) * dp + 2.0 * a3 * p - a4 * dpm) / a1;
+          pm = p; p = pn; dpm = dp; dp = dpn;
+        }
+      }
+      const double sc = sqrt((2.0 * am + 1.0) * 2.0 * (double)(d + 1));
+      const double f = sc * Q[am] * p;
+      for (int c = 0; c < 3; ++c) x[c] += f * cx[c * h + idx];
+      if (xu) {
+        const double fu = sc * Qu[am] * p, fv = sc * (Qv[am] * p + Q[am] * dp);
+        for (int c = 0; c < 3; ++c) { xu[c] += fu * cx[c * h + idx]; xv[c] += fv * cx[c * h + idx]; }
+      }
+      ++idx;
+    }
+  }
+}
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// frame[] = R (rows e1,e2,e3), c, alpha, sigma, L
+__device__ __forceinline__ void to_frame(const double* fr, const double* x, double* X) {
+  const double d0 = x[0] - fr[9], d1 = x[1] - fr[10], d2 = x[2] - fr[11];
+  for (int r = 0; r < 3; ++r) X[r] = fr[12] * (fr[3 * r] * d0 + fr[3 * r + 1] * d1 + fr[3 * r + 2] * d2);
+}
+__device__ __forceinline__ void rotate(const double* fr, const double* v, double* V) {
+  for (int r = 0; r < 3; ++r) V[r] = fr[3 * r] * v[0] + fr[3 * r + 1] * v[1] + fr[3 * r + 2] * v[2];
+}
+
+// harmonic values H'_n^m (m=0..n, n=0..p) at frame point X into H (ncoef complex)
+__device__ void harm_vals(const double* X, int p, cd* H) {
+  const cd W = cmk(X[1], X[2]);
+  const double Z = X[0];
+  H[0] = cmk(1.0, 0.0);
+  for (int n = 1; n <= p; ++n)
+    for (int m = 0; m <= n; ++m) {
+      const cd a = cscale(hget(H, n - 1, m), Z);
+      const cd b = cadd(cmul(W, hget(H, n - 1, m - 1)), cmul(cconj(W), hget(H, n - 1, m + 1)));
+      H[hidx(n, m)] = cadd(a, cscale(cmuli(b), 0.5));
+    }
+}
+
+// gradient of H'_n^m (frame x,y,z) from degree n-1 values
+__device__ __forceinline__ void harm_grad(const cd* H, int n, int m, cd* g) {
+  const double dn = (double)n;
+  const cd lo = hget(H, n - 1, m - 1), hi = hget(H, n - 1, m + 1);
+  g[0] = cscale(hget(H, n - 1, m), dn);
+  g[1] = cscale(cmuli(cadd(lo, hi)), 0.5 * dn);
+  g[2] = cscale(csub(hi, lo), 0.5 * dn);
+}
+
+// Im((-i)^m z)
+__device__ __forceinline__ double imrot(int m, cd z) {
+  switch (m & 3) {
+    case 0: return z.im;
+    case 1: return -z.re;
+    case 2: return -z.im;
+    default: return z.re;
+  }
+}
+
+// ---------------------------------------------------------------------------
+size_t rrq_setup_smem(int h, int nq, int ncoef) {
+  const int nb = 3 * nq;
+  size_t a = (size_t)(7 * h + 3 * h + 16 + 9 * nq + 6 * nb + h) * sizeof(double);
+  size_t b = (size_t)nb * ncoef * sizeof(cd);
+  size_t c = (size_t)64 * ncoef * sizeof(cd);
+  return a + (b > c ? b : c);
+}
+
+// One workgroup per panel that has RRQ pairs: frame, boundary, basis data at the
+// boundary nodes, and the quaternion matrix M^T and Msl^T at the panel nodes.
+__global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan,
+                                                              const int* __restrict__ nrrq,
+                                                              const double* __restrict__ sx,
+                                                              const double* __restrict__ snx,
+                                                              const double* __restrict__ sw, int ichart,
+                                                              const double* __restrict__ sxbd_chart,
+                                                              const double* __restrict__ rv_chart) {
+  const int k = blockIdx.x;
+  if (nrrq[k] == 0) return;
+  extern __shared__ double sm[];
+  const int h = T.h, nq = T.nq, p = T.p, nb = 3 * nq, ncoef = T.ncoef, H4 = 4 * h;
+  double* ssx = sm;             // [3][h]
+  double* ssn = ssx + 3 * h;    // [3][h]
+  double* ssw = ssn + 3 * h;    // [h]
+  double* scx = ssw + h;        // [3][h]
+  double* fr = scx + 3 * h;     // 16
+  double* ecf = fr + 16;        // [3 edges][3][nq]
+  double* bx = ecf + 9 * nq;    // [nb][6]
+  double* sg = bx + 6 * nb;     // [h] sigma contributions
+  cd* Hs = (cd*)(sg + h);       // [nb or 64][ncoef]
+  const size_t gk = (size_t)k * h;
+  for (int i = threadIdx.x; i < h; i += blockDim.x) {
+    for (int c = 0; c < 3; ++c) {
+      ssx[c * h + i] = sx[3 * (gk + i) + c];
+      ssn[c * h + i] = snx[3 * (gk + i) + c];
+    }
+    ssw[i] = sw[gk + i];
+  }
+  __syncthreads();
+  // Koornwinder coefficients cx = umatr * sx
+  for (int i = threadIdx.x; i < 3 * h; i += blockDim.x) {
+    const int c = i / h, a = i % h;
+    double s = 0.0;
+    for (int q = 0; q < h; ++q) s += T.umatr[a + (size_t)h * q] * ssx[c * h + q];
+    scx[c * h + a] = s;
+    P.cx[(size_t)k * 3 * h + c * h + a] = s;
+  }
+  __syncthreads();
+  // orientation: sigma = sign(sum (x_u x x_v) . snx) over the nodes
+  for (int i = threadIdx.x; i < h; i += blockDim.x) {
+    double x[3], xu[3], xv[3], n[3];
+    geom_eval(scx, h, p - 1, T.uvn[2 * i], T.uvn[2 * i + 1], x, xu, xv);
+    cross3(xu, xv, n);
+    sg[i] = n[0] * ssn[i] + n[1] * ssn[h + i] + n[2] * ssn[2 * h + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double rv[9];
+    if (ichart == 1) {
+      for (int i = 0; i < 9; ++i) rv[i] = rv_chart[9 * (size_t)k + i];
+    } else {
+      const double uvv[6] = {0, 0, 1, 0, 0, 1};
+      for (int j = 0; j < 3; ++j) geom_eval(scx, h, p - 1, uvv[2 * j], uvv[2 * j + 1], rv + 3 * j, nullptr, nullptr);
+    }
+    double swsum = 0.0, acc = 0.0, mn[3] = {0, 0, 0};
+    for (int i = 0; i < h; ++i) {
+      swsum += ssw[i];
+      acc += sg[i];
+      for (int c = 0; c < 3; ++c) mn[c] += ssn[c * h + i];
+    }
+    const double alpha = 1.25 / sqrt(2.0 * swsum);
+    double a[3], b[3], nrm[3], axb[3], t1[3], t2[3];
+    for (int c = 0; c < 3; ++c) { a[c] = rv[3 + c] - rv[c]; b[c] = rv[6 + c] - rv[c]; }
+    cross3(a, b, nrm);
+    const double nn = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    double e3[3] = {nrm[0] / nn, nrm[1] / nn, nrm[2] / nn};
+    if (mn[0] * e3[0] + mn[1] * e3[1] + mn[2] * e3[2] < 0.0) for (int c = 0; c < 3; ++c) e3[c] = -e3[c];
+    const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    double e1[3] = {a[0] / la, a[1] / la, a[2] / la}, e2[3];
+    cross3(e3, e1, e2);
+    cross3(a, b, axb);
+    const double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2], bbv = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    cross3(axb, a, t1);
+    cross3(b, axb, t2);
+    const double den = 2.0 * (axb[0] * axb[0] + axb[1] * axb[1] + axb[2] * axb[2]);
+    double c0[3];
+    for (int c = 0; c < 3; ++c) c0[c] = rv[c] + (bbv * t1[c] + aa * t2[c]) / den;
+    for (int c = 0; c < 3; ++c) { fr[c] = e1[c]; fr[3 + c] = e2[c]; fr[6 + c] = e3[c]; fr[9 + c] = c0[c]; }
+    fr[12] = alpha;
+    double zmin = 1e300;
+    for (int i = 0; i < h; ++i) {
+      double x[3] = {ssx[i], ssx[h + i], ssx[2 * h + i]}, X[3];
+      to_frame(fr, x, X);
+      zmin = fmin(zmin, X[2]);
+    }
+    const double s3 = fmax(0.0, -zmin);
+    for (int c = 0; c < 3; ++c) fr[9 + c] = c0[c] - (1.25 * s3 / alpha) * e3[c];
+    fr[13] = acc >= 0.0 ? 1.0 : -1.0;
+    double L = 0.0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = i + 1; j < 3; ++j) {
+        double d = 0.0;
+        for (int c = 0; c < 3; ++c) d += (rv[3 * i + c] - rv[3 * j + c]) * (rv[3 * i + c] - rv[3 * j + c]);
+        L = fmax(L, sqrt(d));
+      }
+    fr[14] = L;
+    fr[15] = 0.0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 16; i += blockDim.x) P.frame[(size_t)k * 16 + i] = fr[i];
+  // boundary positions: edges (0,1)->(0,0), (0,0)->(1,0), (1,0)->(0,1)
+  for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
+    const int e = ib / nq, i = ib % nq;
+    double x[3];
+    if (ichart == 1) {
+      for (int c = 0; c < 3; ++c) x[c] = sxbd_chart[(size_t)k * 9 * nq + 3 * ib + c];
+    } else {
+      const double EA[3][2] = {{0, 1}, {0, 0}, {1, 0}}, EB[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+      const double s01 = 0.5 * (T.tgl[i] + 1.0);
+      const double u = EA[e][0] + (EB[e][0] - EA[e][0]) * s01, v = EA[e][1] + (EB[e][1] - EA[e][1]) * s01;
+      geom_eval(scx, h, p - 1, u, v, x, nullptr, nullptr);
+    }
+    to_frame(fr, x, bx + 6 * ib);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 9 * nq; i += blockDim.x) {
+    const int e = i / (3 * nq), c = (i / nq) % 3, kk = i % nq;
+    double s = 0.0;
+    for (int q = 0; q < nq; ++q) s += T.Legmat[kk + (size_t)nq * q] * bx[6 * (e * nq + q) + c];
+    ecf[i] = s;
+    P.edge[(size_t)k * 9 * nq + i] = s;
+  }
+  __syncthreads();
+  for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
+    const int e = ib / nq, i = ib % nq;
+    for (int c = 0; c < 3; ++c) {
+      double v, d;
+      leg_eval_r(ecf + (e * 3 + c) * nq, nq, T.tgl[i], v, d);
+      bx[6 * ib + 3 + c] = d;
+    }
+    for (int c = 0; c < 6; ++c) P.bX[((size_t)k * nb + ib) * 6 + c] = bx[6 * ib + c];
+  }
+  __syncthreads();
+  // basis data at the boundary nodes: grad H'_n^m and V = grad(dX . grad H'_n^m)
+  for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) harm_vals(bx + 6 * ib, p, Hs + (size_t)ib * ncoef);
+  __syncthreads();
+  for (int it = threadIdx.x; it < nb * ncoef; it += blockDim.x) {
+    const int ib = it / ncoef, kk = it % ncoef;
+    int n = 0;
+    while (hidx(n + 1, 0) <= kk) ++n;
+    const int m = kk - hidx(n, 0);
+    const cd* H = Hs + (size_t)ib * ncoef;
+    const double* dX = bx + 6 * ib + 3;
+    cd g[3];
+    harm_grad(H, n, m, g);
+    double* o = P.basis + (((size_t)k * nb + ib) * ncoef + kk) * 12;
+    for (int c = 0; c < 3; ++c) { o[2 * c] = g[c].re; o[2 * c + 1] = g[c].im; }
+    // V = n [dX_Z grad H'_{n-1}^m + i/2 (dW grad H'_{n-1}^{m-1} + conj(dW) grad H'_{n-1}^{m+1})]
+    cd V[3] = {cmk(0, 0), cmk(0, 0), cmk(0, 0)};
+    if (n >= 2) {
+      cd gl[3], gm[3], gh[3];
+      const int n1 = n - 1;
+      // gradients of degree n-1 at m-1, m, m+1 (negative orders through conjugation)
+      auto gradm = [&](int mm, cd* gg) {
+        if (mm > n1 || mm < -n1) { gg[0] = gg[1] = gg[2] = cmk(0, 0); return; }
+        if (mm >= 0) { harm_grad(H, n1, mm, gg); return; }
+        harm_grad(H, n1, -mm, gg);
+        for (int c = 0; c < 3; ++c) { gg[c] = cconj(gg[c]); if ((-mm) & 1) gg[c] = cmk(-gg[c].re, -gg[c].im); }
+      };
+      gradm(m - 1, gl); gradm(m, gm); gradm(m + 1, gh);
+      const cd dW = cmk(dX[1], dX[2]);
+      for (int c = 0; c < 3; ++c) {
+        const cd t = cadd(cmul(dW, gl[c]), cmul(cconj(dW), gh[c]));
+        V[c] = cscale(cadd(cscale(gm[c], dX[0]), cscale(cmuli(t), 0.5)), (double)n);
+      }
+    }
+    for (int c = 0; c < 3; ++c) { o[6 + 2 * c] = V[c].re; o[6 + 2 * c + 1] = V[c].im; }
+  }
+  __syncthreads();
+  // quaternion system at the nodes.  F_k = Im((-i)^m H'_n^m), n=1..p, m=1..n.
+  double* MT = P.MT + (size_t)k * H4 * H4;
+  double* MS = P.Msl + (size_t)k * h * h;
+  double* FN = P.Fn + (size_t)k * h * h;
+  for (int i = threadIdx.x; i < H4 * H4; i += blockDim.x) MT[i] = 0.0;
+  __syncthreads();
+  for (int i0 = 0; i0 < h; i0 += 64) {
+    const int i = i0 + threadIdx.x;
+    if (threadIdx.x < 64 && i < h) {
+      cd* H = Hs + (size_t)threadIdx.x * ncoef;
+      double x[3] = {ssx[i], ssx[h + i], ssx[2 * h + i]}, nn0[3] = {ssn[i], ssn[h + i], ssn[2 * h + i]}, X[3], N[3];
+      to_frame(fr, x, X);
+      rotate(fr, nn0, N);
+      harm_vals(X, p, H);
+      int kk = 0;
+      for (int n = 1; n <= p; ++n)
+        for (int m = 1; m <= n; ++m, ++kk) {
+          cd g[3];
+          harm_grad(H, n, m, g);
+          const double F = imrot(m, H[hidx(n, m)]);
+          const double F1 = imrot(m, g[0]), F2 = imrot(m, g[1]), F3 = imrot(m, g[2]);
+          FN[i + (size_t)h * kk] = F;
+          MS[kk + (size_t)h * i] = N[0] * F1 + N[1] * F2 + N[2] * F3;
+          // M blocks (rows: component a at node i; cols: component b of coefficient kk):
+          // a=0: [0,-F1,-F2,-F3]; a=1: [F1,0,-F3,F2]; a=2: [F2,F3,0,-F1]; a=3: [F3,-F2,F1,0]
+          const double blk[4][4] = {{0, -F1, -F2, -F3}, {F1, 0, -F3, F2}, {F2, F3, 0, -F1}, {F3, -F2, F1, 0}};
+          for (int a = 0; a < 4; ++a)
+            for (int bb = 0; bb < 4; ++bb)
+              if (a != bb) MT[(size_t)(bb * h + kk) + (size_t)H4 * (a * h + i)] = blk[a][bb];
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting (whole-row swaps) of M^T (4h) and Msl^T (h), in place
+__device__ void lu_inplace(double* A, int n, int* piv, double* prow) {
+  __shared__ int sip;
+  __shared__ double smax[4];
+  __shared__ int sidx[4];
+  for (int kcol = 0; kcol < n; ++kcol) {
+    // pivot search by the first wavefronts
+    double best = -1.0;
+    int bi = kcol;
+    for (int i = kcol + threadIdx.x; i < n; i += blockDim.x) {
+      const double v = fabs(A[i + (size_t)n * kcol]);
+      if (v > best) { best = v; bi = i; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o);
+      const int oi = __shfl_xor(bi, o);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    if ((threadIdx.x & 63) == 0) { smax[threadIdx.x >> 6] = best; sidx[threadIdx.x >> 6] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double b0 = smax[0];
+      int i0 = sidx[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+        if (smax[w] > b0 || (smax[w] == b0 && sidx[w] < i0)) { b0 = smax[w]; i0 = sidx[w]; }
+      sip = i0;
+      piv[kcol] = i0;
+    }
+    __syncthreads();
+    const int ip = sip;
+    if (ip != kcol)
+      for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double t = A[kcol + (size_t)n * j];
+        A[kcol + (size_t)n * j] = A[ip + (size_t)n * j];
+        A[ip + (size_t)n * j] = t;
+      }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) prow[j] = A[kcol + (size_t)n * j];
+    __syncthreads();
+    const double pv = prow[kcol];
+    for (int i = kcol + 1 + threadIdx.x; i < n; i += blockDim.x) A[i + (size_t)n * kcol] = A[i + (size_t)n * kcol] / pv;
+    __syncthreads();
+    const int m = n - kcol - 1;
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+      const int i = kcol + 1 + e % m, j = kcol + 1 + e / m;
+      A[i + (size_t)n * j] = A[i + (size_t)n * j] - A[i + (size_t)n * kcol] * prow[j];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) rrq_panel_lu_kernel(RRQTables T, RRQPanels P, int npan,
+                                                           const int* __restrict__ nrrq) {
+  const int k = blockIdx.x;
+  if (nrrq[k] == 0) return;
+  extern __shared__ double prow[];
+  const int h = T.h;
+  lu_inplace(P.MT + (size_t)k * 16 * h * h, 4 * h, P.piv + (size_t)k * 5 * h, prow);
+  lu_inplace(P.Msl + (size_t)k * h * h, h, P.piv + (size_t)k * 5 * h + 4 * h, prow);
+}
+
+// Weights operator G (2h x 5h) acting on v = [rhs(4h); W(h)]:
+//   Ad = G1 rhs / 4pi,    G1 = first h rows of (M^T)^{-1}
+//   As = (Msl^T)^{-1} (W/4pi + Fn^T Ad) / alpha
+// Rows j<h: As, rows h..2h-1: Ad.  Scratch per panel (global, L2 resident):
+// Y (h x 4h, rows of G1), T1 = Fn^T G1 (h x 4h), B = (Msl^T)^{-1} (h x h).
+__global__ void __launch_bounds__(256) rrq_panel_weights_kernel(RRQTables T, RRQPanels P, int npan,
+                                                                const int* __restrict__ nrrq) {
+  const int k = blockIdx.x;
+  if (nrrq[k] == 0) return;
+  const int h = T.h, H4 = 4 * h, H5 = 5 * h;
+  const double* A = P.MT + (size_t)k * H4 * H4;
+  const double* S = P.Msl + (size_t)k * h * h;
+  const int* piv = P.piv + (size_t)k * 5 * h;
+  const double* FN = P.Fn + (size_t)k * h * h;
+  double* G = P.G + (size_t)k * 2 * h * H5;
+  double* y = P.scr + (size_t)k * 9 * h * h;  // [h][4h]
+  double* T1 = y + (size_t)4 * h * h;          // [h][4h]
+  double* Bm = T1 + (size_t)4 * h * h;         // [h][h], Bm[r*h + c] = B(c, r)
+  const double alpha = P.frame[(size_t)k * 16 + 12];
+  const double fac = 1.0 / (16.0 * atan(1.0));
+  // row r of (M^T)^{-1}: with P A = L U, A^T y = e_r is U^T z = e_r, L^T w = z, y = P^T w
+  for (int r = threadIdx.x; r < h; r += blockDim.x) {
+    double* v = y + (size_t)r * H4;
+    for (int i = 0; i < H4; ++i) v[i] = (i == r) ? 1.0 : 0.0;
+    for (int i = 0; i < H4; ++i) {
+      double s = v[i];
+      for (int q = 0; q < i; ++q) s -= A[q + (size_t)H4 * i] * v[q];
+      v[i] = s / A[i + (size_t)H4 * i];
+    }
+    for (int i = H4 - 1; i >= 0; --i) {
+      double s = v[i];
+      for (int q = i + 1; q < H4; ++q) s -= A[q + (size_t)H4 * i] * v[q];
+      v[i] = s;
+    }
+    for (int i = H4 - 1; i >= 0; --i) {
+      const int ip = piv[i];
+      if (ip != i) { const double t = v[i]; v[i] = v[ip]; v[ip] = t; }
+    }
+  }
+  // columns of B = (Msl^T)^{-1}
+  for (int r = threadIdx.x; r < h; r += blockDim.x) {
+    double* v = Bm + (size_t)r * h;
+    for (int i = 0; i < h; ++i) v[i] = (i == r) ? 1.0 : 0.0;
+    for (int i = 0; i < h; ++i) {
+      const int ip = piv[4 * h + i];
+      if (ip != i) { const double t = v[i]; v[i] = v[ip]; v[ip] = t; }
+    }
+    for (int i = 0; i < h; ++i) {
+      double s = v[i];
+      for (int q = 0; q < i; ++q) s -= S[i + (size_t)h * q] * v[q];
+      v[i] = s;
+    }
+    for (int i = h - 1; i >= 0; --i) {
+      double s = v[i];
+      for (int q = i + 1; q < h; ++q) s -= S[i + (size_t)h * q] * v[q];
+      v[i] = s / S[i + (size_t)h * i];
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  // T1 = Fn^T G1
+  for (int e = threadIdx.x; e < h * H4; e += blockDim.x) {
+    const int r = e / H4, c = e % H4;
+    double t = 0.0;
+    for (int i = 0; i < h; ++i) t += FN[i + (size_t)h * r] * y[(size_t)i * H4 + c];
+    T1[(size_t)r * H4 + c] = t;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
+    const int r = e / H5, c = e % H5;
+    G[(size_t)(h + r) * H5 + c] = c < H4 ? y[(size_t)r * H4 + c] * fac : 0.0;
+  }
+  for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
+    const int j = e / H5, c = e % H5;
+    double s = 0.0;
+    if (c < H4) {
+      for (int r = 0; r < h; ++r) s += Bm[(size_t)r * h + j] * T1[(size_t)r * H4 + c];
+    } else {
+      s = Bm[(size_t)(c - H4) * h + j];
+    }
+    G[(size_t)j * H5 + c] = s * fac / alpha;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-target pieces
+
+// complex root of |X_e(s) - x|^2 near [-1,1] by Newton from the chord guess
+__device__ bool edge_root(const double* cf, int nq, const double* x, cd& s0) {
+  double A[3], B[3], D[3], C[3];
+  for (int c = 0; c < 3; ++c) {
+    double v, d;
+    leg_eval_r(cf + c * nq, nq, -1.0, v, d); A[c] = v;
+    leg_eval_r(cf + c * nq, nq, 1.0, v, d); B[c] = v;
+    D[c] = 0.5 * (B[c] - A[c]);
+    C[c] = 0.5 * (A[c] + B[c]) - x[c];
+  }
+  const double dd = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+  const double cdv = C[0] * D[0] + C[1] * D[1] + C[2] * D[2];
+  const double cc = C[0] * C[0] + C[1] * C[1] + C[2] * C[2];
+  cd s = cmk(-cdv / dd, sqrt(fmax(0.0, cc * dd - cdv * cdv)) / dd);
+  for (int it = 0; it < 30; ++it) {
+    cd f = cmk(0, 0), fp = cmk(0, 0);
+    for (int c = 0; c < 3; ++c) {
+      cd v, d;
+      leg_eval_c(cf + c * nq, nq, s, v, d);
+      v.re -= x[c];
+      f = cadd(f, cmul(v, v));
+      fp = cadd(fp, cscale(cmul(v, d), 2.0));
+    }
+    const cd ds = cdiv(f, fp);
+    s = csub(s, ds);
+    if (cabs_(ds) < 1e-15 * (1.0 + cabs_(s))) { s0 = s; return isfinite(s.re) && isfinite(s.im); }
+  }
+  s0 = s;
+  return false;
+}
+
+// singularity-swap weights for the 1/|X(s)-x| kernel on one edge; w applies to f_i/R_i
+__device__ void ssq_w1(const double* cf, int nq, const double* tgl, const double* wgl, const double* x,
+                       double rho_thr, double* w, cd* root, bool* rok) {
+  cd s0;
+  const bool ok = edge_root(cf, nq, x, s0);
+  *root = s0;
+  *rok = ok;
+  if (!ok || bernstein_rho(s0) >= rho_thr) {
+    for (int i = 0; i < nq; ++i) w[i] = wgl[i];
+    return;
+  }
+  const double a = s0.re, b = fabs(s0.im), sg = 1.0 + fabs(a);
+  const double x1 = 1.0 - a, x2 = -1.0 - a;
+  const double r1 = sqrt(x1 * x1 + b * b), r2 = sqrt(x2 * x2 + b * b);
+  double I[QOL_MAXNQ];
+  if (x2 >= 0.0) I[0] = log((x1 + r1) / (x2 + r2));
+  else if (x1 <= 0.0) I[0] = log((r2 - x2) / (r1 - x1));
+  else I[0] = log((x1 + r1) * (r2 - x2) / (b * b));
+  if (nq > 1) I[1] = r1 - r2;
+  double p1 = x1, p2 = x2;
+  for (int kk = 2; kk < nq; ++kk) {
+    const double dk = (double)kk;
+    I[kk] = (p1 * r1 - p2 * r2 - (dk - 1.0) * b * b * I[kk - 2]) / dk;
+    p1 *= x1;
+    p2 *= x2;
+  }
+  double sc = 1.0;
+  for (int kk = 0; kk < nq; ++kk) { I[kk] *= sc; sc /= sg; }
+  // dual Vandermonde system sum_i lam_i t_i^k = I_k (Bjorck-Pereyra), t_i = (s_i - a)/sg
+  double t[QOL_MAXNQ];
+  for (int i = 0; i < nq; ++i) t[i] = (tgl[i] - a) / sg;
+  const int N = nq - 1;
+  for (int kk = 0; kk < N; ++kk)
+    for (int i = N; i > kk; --i) I[i] -= t[kk] * I[i - 1];
+  for (int kk = N - 1; kk >= 0; --kk) {
+    for (int i = kk + 1; i <= N; ++i) I[i] /= (t[i] - t[i - kk - 1]);
+    for (int i = kk; i < N; ++i) I[i] -= I[i + 1];
+  }
+  for (int i = 0; i < nq; ++i) {
+    const double ds = tgl[i] - a;
+    w[i] = I[i] * sqrt(ds * ds + b * b);
+  }
+}
+
+// Dirac-string line integral of one edge, adaptive march toward the near point
+__device__ double solid_angle_edge(const double* cf, int nq, const double* tgl, const double* wgl, const double* x,
+                                   const double* q, cd s0, bool ok, double rho_thr) {
+  double total = 0.0, lo = -1.0, len = 2.0;
+  int guard = 0;
+  while (lo < 1.0 && guard < 512) {
+    ++guard;
+    double hi = fmin(1.0, lo + len);
+    if (ok) {
+      while (hi - lo > 1e-14) {
+        const double c = 0.5 * (lo + hi), hh = 0.5 * (hi - lo);
+        const cd t0 = cdiv(csub(s0, cmk(c, 0.0)), cmk(hh, 0.0));
+        if (bernstein_rho(t0) >= rho_thr) break;
+        hi = lo + 0.5 * (hi - lo);
+      }
+    }
+    const double c = 0.5 * (lo + hi), hh = 0.5 * (hi - lo);
+    for (int i = 0; i < nq; ++i) {
+      const double s = c + hh * tgl[i];
+      double X[3], dX[3];
+      for (int cc = 0; cc < 3; ++cc) leg_eval_r(cf + cc * nq, nq, s, X[cc], dX[cc]);
+      const double z[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
+      double zx[3];
+      cross3(z, dX, zx);
+      const double rz = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+      const double qz = q[0] * z[0] + q[1] * z[1] + q[2] * z[2];
+      const double qzx = q[0] * zx[0] + q[1] * zx[1] + q[2] * zx[2];
+      total += hh * wgl[i] * qzx / (rz * (rz + qz));
+    }
+    len = 2.0 * (hi - lo);
+    lo = hi;
+  }
+  return total;
+}
+
+// Gauss-Newton projection of x (original coordinates) onto the patch; returns
+// the signed distance along the uv normal and that normal
+__device__ double project(const double* cx, int h, int korder, const double* ssx, const double* uvn,
+                          const double* x, double* nstar) {
+  double best = 1e300, u = 1.0 / 3.0, v = 1.0 / 3.0;
+  for (int i = 0; i < h; ++i) {
+    const double d0 = ssx[i] - x[0], d1 = ssx[h + i] - x[1], d2 = ssx[2 * h + i] - x[2];
+    const double d = d0 * d0 + d1 * d1 + d2 * d2;
+    if (d < best) { best = d; u = uvn[2 * i]; v = uvn[2 * i + 1]; }
+  }
+  double y[3], yu[3], yv[3];
+  for (int it = 0; it < 40; ++it) {
+    geom_eval(cx, h, korder, u, v, y, yu, yv);
+    const double r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+    const double a11 = yu[0] * yu[0] + yu[1] * yu[1] + yu[2] * yu[2];
+    const double a12 = yu[0] * yv[0] + yu[1] * yv[1] + yu[2] * yv[2];
+    const double a22 = yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2];
+    const double b1 = -(yu[0] * r[0] + yu[1] * r[1] + yu[2] * r[2]);
+    const double b2 = -(yv[0] * r[0] + yv[1] * r[1] + yv[2] * r[2]);
+    const double det = a11 * a22 - a12 * a12;
+    double un = u + (a22 * b1 - a12 * b2) / det, vn = v + (a11 * b2 - a12 * b1) / det;
+    if (un < 0.0) un = 0.0;
+    if (vn < 0.0) vn = 0.0;
+    if (un + vn > 1.0) {
+      const double ex = 0.5 * (un + vn - 1.0);
+      un -= ex; vn -= ex;
+      if (un < 0.0) { vn += un; un = 0.0; }
+      if (vn < 0.0) { un += vn; vn = 0.0; }
+    }
+    const double step = fabs(un - u) + fabs(vn - v);
+    u = un; v = vn;
+    if (step < 1e-15) break;
+  }
+  geom_eval(cx, h, korder, u, v, y, yu, yv);
+  double nn[3];
+  cross3(yu, yv, nn);
+  const double l = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+  for (int c = 0; c < 3; ++c) nstar[c] = nn[c] / l;
+  return (x[0] - y[0]) * nstar[0] + (x[1] - y[1]) * nstar[1] + (x[2] - y[2]) * nstar[2];
+}
+
+__host__ __device__ int rrq_tile(int h, int nq, int ncoef) {
+  const size_t per = (size_t)(6 + 3 * nq + 3 * nq * 6 + 5 * h + 8) * 8 + (size_t)ncoef * 5 * 16;
+  const size_t fixed = (size_t)(7 * h + 16 + 9 * nq + 6 * 3 * nq + 2 * nq + 2 * h) * 8;
+  int t = (int)((150 * 1024 - fixed) / per);
+  return t > 16 ? 16 : (t < 1 ? 1 : t);
+}
+
+size_t rrq_target_smem(int h, int nq, int ncoef) {
+  const int T = rrq_tile(h, nq, ncoef);
+  const size_t per = (size_t)(6 + 3 * nq + 3 * nq * 6 + 5 * h + 8) * 8 + (size_t)ncoef * 5 * 16;
+  const size_t fixed = (size_t)(7 * h + 16 + 9 * nq + 6 * 3 * nq + 2 * nq + 2 * h) * 8;
+  return fixed + per * T;
+}
+
+// One workgroup per panel with RRQ pairs; the panel's RRQ pairs in tiles.
+__global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels P, const int* __restrict__ offs,
+                                                         const int* __restrict__ tidx, const double* __restrict__ tx,
+                                                         const int8_t* __restrict__ label, const int* __restrict__ nrrq,
+                                                         int exterior, double side_tol, double* __restrict__ wout,
+                                                         double* __restrict__ omega_out, double* __restrict__ ias_out) {
+  const int k = blockIdx.x;
+  if (nrrq[k] == 0) return;
+  const int h = T.h, nq = T.nq, p = T.p, nb = 3 * nq, ncoef = T.ncoef, H4 = 4 * h, H5 = 5 * h;
+  const int TT = rrq_tile(h, nq, ncoef);
+  extern __shared__ double sm[];
+  double* ssx = sm;                 // [3][h]
+  double* scx = ssx + 3 * h;        // [3][h]
+  double* fr = scx + 3 * h;         // 16
+  double* ecf = fr + 16;            // [3][3][nq]
+  double* bx = ecf + 9 * nq;        // [nb][6]
+  double* stg = bx + 6 * nb;        // tgl[nq], wgl[nq]
+  double* swg = stg + nq;
+  double* suv = swg + nq;           // [2h]
+  double* tX = suv + 2 * h;         // [TT][6]   frame X, original x
+  double* w1 = tX + 6 * TT;         // [TT][nb]
+  double* ab = w1 + (size_t)nb * TT;  // [TT][nb][6]
+  double* vv = ab + (size_t)6 * nb * TT;  // [TT][5h]
+  double* tq = vv + (size_t)H5 * TT;      // [TT][8]: q(3), omega parts(3), spare
+  cd* mom = (cd*)(tq + 8 * TT);           // [TT][ncoef][5]
+  __shared__ int plist[16];
+  __shared__ int ntile;
+  __shared__ cd sroot[16][3];
+  __shared__ int srok[16][3];
+  for (int i = threadIdx.x; i < 3 * h; i += blockDim.x) scx[i] = P.cx[(size_t)k * 3 * h + i];
+  for (int i = threadIdx.x; i < 16; i += blockDim.x) fr[i] = P.frame[(size_t)k * 16 + i];
+  for (int i = threadIdx.x; i < 9 * nq; i += blockDim.x) ecf[i] = P.edge[(size_t)k * 9 * nq + i];
+  for (int i = threadIdx.x; i < 6 * nb; i += blockDim.x) bx[i] = P.bX[(size_t)k * nb * 6 + i];
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) { stg[i] = T.tgl[i]; swg[i] = T.wgl[i]; }
+  for (int i = threadIdx.x; i < 2 * h; i += blockDim.x) suv[i] = T.uvn[i];
+  __syncthreads();
+  // node positions from the geometry polynomial at the VR nodes
+  for (int i = threadIdx.x; i < h; i += blockDim.x) {
+    double x[3];
+    geom_eval(scx, h, p - 1, suv[2 * i], suv[2 * i + 1], x, nullptr, nullptr);
+    for (int c = 0; c < 3; ++c) ssx[c * h + i] = x[c];
+  }
+  const double rho_thr = pow(8.0, 8.0 / (double)nq);
+  const double sigma = fr[13], alpha = fr[12], L = fr[14];
+  const double* G = P.G + (size_t)k * 2 * h * H5;
+  const double* basis = P.basis + (size_t)k * nb * ncoef * 12;
+  const int j0 = offs[k], j1 = offs[k + 1];
+  int scan = j0;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int c = 0;
+      while (scan < j1 && c < TT) {
+        if (label[scan] == 5) plist[c++] = scan;
+        ++scan;
+      }
+      ntile = c;
+    }
+    __syncthreads();
+    const int nt = ntile;
+    if (nt == 0) break;
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+      const size_t g = (size_t)tidx[plist[t]];
+      const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
+      to_frame(fr, x, tX + 6 * t);
+      tX[6 * t + 3] = x[0]; tX[6 * t + 4] = x[1]; tX[6 * t + 5] = x[2];
+    }
+    __syncthreads();
+    // SSQ weights, one thread per (target, edge)
+    for (int it = threadIdx.x; it < 3 * nt; it += blockDim.x) {
+      const int t = it / 3, e = it % 3;
+      cd r;
+      bool ok;
+      ssq_w1(ecf + 9 * nq * 0 + e * 3 * nq, nq, stg, swg, tX + 6 * t, rho_thr, w1 + (size_t)t * nb + e * nq, &r, &ok);
+      sroot[t][e] = r;
+      srok[t][e] = ok ? 1 : 0;
+    }
+    // side of each target and the Dirac-string direction (frame)
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+      double ns[3];
+      const double d = project(scx, h, p - 1, ssx, suv, tX + 6 * t + 3, ns);
+      double side;
+      if (fabs(d) > side_tol * L) side = d > 0.0 ? 1.0 : -1.0;
+      else side = (exterior ? 1.0 : -1.0) * sigma;
+      const double qo[3] = {-side * ns[0], -side * ns[1], -side * ns[2]};
+      rotate(fr, qo, tq + 8 * t);
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < 3 * nt; it += blockDim.x) {
+      const int t = it / 3, e = it % 3;
+      tq[8 * t + 3 + e] = solid_angle_edge(ecf + e * 3 * nq, nq, stg, swg, tX + 6 * t, tq + 8 * t, sroot[t][e],
+                                           srok[t][e] != 0, rho_thr);
+    }
+    // a = (w1/R)(z x dX), b = (w1/R) z
+    for (int it = threadIdx.x; it < nt * nb; it += blockDim.x) {
+      const int t = it / nb, ib = it % nb;
+      const double* Y = bx + 6 * ib;
+      const double z[3] = {Y[0] - tX[6 * t], Y[1] - tX[6 * t + 1], Y[2] - tX[6 * t + 2]};
+      const double wr = w1[(size_t)t * nb + ib] / sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+      double zx[3];
+      cross3(z, Y + 3, zx);
+      double* o = ab + ((size_t)t * nb + ib) * 6;
+      for (int c = 0; c < 3; ++c) { o[c] = wr * zx[c]; o[3 + c] = wr * z[c]; }
+    }
+    __syncthreads();
+    // boundary moments chi (1 complex) and alpha (4 complex) per coefficient
+    for (int it = threadIdx.x; it < nt * ncoef; it += blockDim.x) {
+      const int t = it / ncoef, kk = it % ncoef;
+      double c0r = 0, c0i = 0, a0r = 0, a0i = 0, a1r = 0, a1i = 0, a2r = 0, a2i = 0, a3r = 0, a3i = 0;
+      for (int ib = 0; ib < nb; ++ib) {
+        const double* o = ab + ((size_t)t * nb + ib) * 6;
+        const double* B = basis + ((size_t)ib * ncoef + kk) * 12;
+        // chi += a . gradH
+        c0r += o[0] * B[0] + o[1] * B[2] + o[2] * B[4];
+        c0i += o[0] * B[1] + o[1] * B[3] + o[2] * B[5];
+        // z (x) V = (-z.V, z x V), z = o[3..5]
+        const double z0 = o[3], z1 = o[4], z2 = o[5];
+        a0r -= z0 * B[6] + z1 * B[8] + z2 * B[10];
+        a0i -= z0 * B[7] + z1 * B[9] + z2 * B[11];
+        a1r += z1 * B[10] - z2 * B[8];
+        a1i += z1 * B[11] - z2 * B[9];
+        a2r += z2 * B[6] - z0 * B[10];
+        a2i += z2 * B[7] - z0 * B[11];
+        a3r += z0 * B[8] - z1 * B[6];
+        a3i += z0 * B[9] - z1 * B[7];
+      }
+      cd* M = mom + ((size_t)t * ncoef + kk) * 5;
+      M[0] = cmk(c0r, c0i);
+      M[1] = cmk(a0r, a0i);
+      M[2] = cmk(a1r, a1i);
+      M[3] = cmk(a2r, a2i);
+      M[4] = cmk(a3r, a3i);
+    }
+    __syncthreads();
+    // recursion in degree (in place: chi -> W~, alpha -> Q~), then the moment vector
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+      cd* M = mom + (size_t)t * ncoef * 5;
+      const double* X = tX + 6 * t;
+      const double om = tq[8 * t + 3] + tq[8 * t + 4] + tq[8 * t + 5];
+      double lam[3] = {0, 0, 0};
+      for (int ib = 0; ib < nb; ++ib) {
+        const double* Y = bx + 6 * ib;
+        const double z[3] = {Y[0] - X[0], Y[1] - X[1], Y[2] - X[2]};
+        const double wr = w1[(size_t)t * nb + ib] / sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+        for (int c = 0; c < 3; ++c) lam[c] += wr * Y[3 + c];
+      }
+      const double xZ = X[0];
+      const cd w = cmk(X[1], X[2]), wc = cmk(X[1], -X[2]);
+      M[0] = cmk(om, 0.0);  // W~_00
+      for (int c = 1; c < 5; ++c) M[c] = cmk(0.0, 0.0);
+      // degree-1 Q: Lambda (x) grad H'_1^m, Lambda = -omega + lam
+      for (int m = 0; m <= 1; ++m) {
+        cd g[3];
+        if (m == 0) { g[0] = cmk(1, 0); g[1] = cmk(0, 0); g[2] = cmk(0, 0); }
+        else { g[0] = cmk(0, 0); g[1] = cmk(0, 0.5); g[2] = cmk(-0.5, 0); }
+        cd* Q = M + (size_t)hidx(1, m) * 5 + 1;
+        Q[0] = cscale(cadd(cadd(cscale(g[0], lam[0]), cscale(g[1], lam[1])), cscale(g[2], lam[2])), -1.0);
+        Q[1] = cadd(cscale(g[0], -om), csub(cscale(g[2], lam[1]), cscale(g[1], lam[2])));
+        Q[2] = cadd(cscale(g[1], -om), csub(cscale(g[0], lam[2]), cscale(g[2], lam[0])));
+        Q[3] = cadd(cscale(g[2], -om), csub(cscale(g[1], lam[0]), cscale(g[0], lam[1])));
+      }
+      for (int n = 1; n <= p; ++n) {
+        const double dn = (double)n;
+        for (int m = 0; m <= n; ++m) {
+          // entries of degree n-1 at m-1, m, m+1 (negative m by conjugation)
+          auto get = [&](int mm, int c) -> cd {
+            const int n1 = n - 1;
+            if (mm > n1 || mm < -n1) return cmk(0, 0);
+            if (mm >= 0) return M[(size_t)hidx(n1, mm) * 5 + c];
+            cd v = cconj(M[(size_t)hidx(n1, -mm) * 5 + c]);
+            return ((-mm) & 1) ? cmk(-v.re, -v.im) : v;
+          };
+          cd* E = M + (size_t)hidx(n, m) * 5;
+          {
+            const cd lo = get(m - 1, 0), mid = get(m, 0), hi = get(m + 1, 0);
+            const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
+            E[0] = cadd(cadd(cscale(E[0], 1.0 / dn), cscale(mid, xZ)), t2);
+          }
+          if (n >= 2)
+            for (int c = 1; c < 5; ++c) {
+              const cd lo = get(m - 1, c), mid = get(m, c), hi = get(m + 1, c);
+              const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
+              E[c] = cscale(cadd(E[c], cscale(cadd(cscale(mid, xZ), t2), dn)), 1.0 / (dn - 1.0));
+            }
+        }
+      }
+      double* v = vv + (size_t)t * H5;
+      int kk = 0;
+      for (int n = 1; n <= p; ++n)
+        for (int m = 1; m <= n; ++m, ++kk) {
+          const cd* E = M + (size_t)hidx(n, m) * 5;
+          v[kk] = sigma * imrot(m, E[1]);
+          v[h + kk] = -sigma * imrot(m, E[2]);
+          v[2 * h + kk] = -sigma * imrot(m, E[3]);
+          v[3 * h + kk] = -sigma * imrot(m, E[4]);
+          v[4 * h + kk] = sigma * imrot(m, E[0]);
+        }
+      if (omega_out) {
+        const double f = 1.0 / (16.0 * atan(1.0)) / alpha;
+        for (int i = 0; i < H4; ++i) omega_out[(size_t)plist[t] * H4 + i] = v[i] * f;
+      }
+      if (ias_out) ias_out[plist[t]] = sigma * om;
+    }
+    __syncthreads();
+    // weights: out(j) = sum_r G(j, r) v(r)
+    for (int it = threadIdx.x; it < nt * 2 * h; it += blockDim.x) {
+      const int t = it / (2 * h), j = it % (2 * h);
+      const double* v = vv + (size_t)t * H5;
+      const double* g = G + (size_t)j * H5;
+      double s = 0.0;
+      for (int r = 0; r < H5; ++r) s += g[r] * v[r];
+      wout[(size_t)plist[t] * 2 * h + j] = s;
+    }
+  }
+}
+
+}  // namespace qol

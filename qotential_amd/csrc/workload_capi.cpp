@@ -30,13 +30,16 @@ Surface make_surface(int64_t nfp, int64_t nmode, const double* mnrz) {
   return s;
 }
 
-thread_local std::string g_err;
-
 }  // namespace
+
+namespace qol {
+void set_error(const std::string& s);
+const char* error_text();
+}
 
 extern "C" {
 
-const char* qol_last_error(void) { return g_err.c_str(); }
+const char* qol_last_error(void) { return qol::error_text(); }
 
 int64_t qol_geo_count(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
                       int64_t iterate_aspect) {
@@ -44,7 +47,7 @@ int64_t qol_geo_count(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nm
     Surface s = make_surface(nfp, nmode, mnrz);
     return 2 * (int64_t)build_charts(s, mp, np, p, iterate_aspect != 0).size();
   } catch (const std::exception& e) {
-    g_err = e.what();
+    qol::set_error(e.what());
     return -1;
   }
 }
@@ -69,7 +72,7 @@ int qol_geo_build(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
       }
     return 0;
   } catch (const std::exception& e) {
-    g_err = e.what();
+    qol::set_error(e.what());
     return 1;
   }
 }
@@ -110,7 +113,7 @@ int qol_geo_chart_boundary(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64
     }
     return 0;
   } catch (const std::exception& e) {
-    g_err = e.what();
+    qol::set_error(e.what());
     return 1;
   }
 }
@@ -180,7 +183,7 @@ int qol_make_targets(int64_t p, int64_t ntri, const double* sx, const double* sn
     }
     return 0;
   } catch (const std::exception& e) {
-    g_err = e.what();
+    qol::set_error(e.what());
     return 1;
   }
 }
