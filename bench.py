@@ -1,0 +1,186 @@
+"""Benchmark: close-evaluation target-panel pairs/sec on one MI355X.
+
+Workload (the order-8 stellarator case): the 5-period surface
+R = 1 + 0.3 cos th + 0.06 cos(th - 5 ph), Z = 0.3 sin th - 0.06 sin(th - 5 ph),
+48 x 120 charts -> 14,250 order-8 triangles (513,000 nodes); 200,000 targets at
+signed distance h 10^U, U ~ U[-6,-1], on both sides; near list by the
+reference's ball rule; density from 8 point charges 0.4 outside the surface.
+
+A step is one pass of the close-evaluation path over the resident batch:
+refinement-level classification, direct S/D on the levels, RRQ for the rest,
+the naive-block subtraction and the fixed-order scatter into the potential
+(the reference's "rrq+fill" region).  Geometry, targets, near list and density
+are in HBM before timing starts.  The GRF identity S[du/dn] - D[u] = u (inside)
+/ 0 (outside) is checked on every target after the timed steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mp", type=int, default=48)
+    ap.add_argument("--np", type=int, default=120)
+    ap.add_argument("--order", type=int, default=8)
+    ap.add_argument("--targets", type=int, default=200000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-pairs", type=int, default=6000)
+    return ap.parse_args()
+
+
+def algorithmic_flops(p, counts, nrrq):
+    """FP64 FLOPs the math needs per stage (counted from the formulas, not from
+    executed instructions).  counts[l] = pairs served by level l (l = 0..3)."""
+    from qotential_amd.engine import order_info
+    info = order_info(p)
+    h, nq, ncoef = info["h"], info["nq"], info["ncoef"]
+    nb = 3 * nq
+    # direct level L: per node 19 (kernel incl. 1 rsqrt) + 4h (two contractions)
+    direct = sum(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h) for l in range(4))
+    # RRQ per pair: boundary moments 44 per (node, coefficient), recursion
+    # 5 complex updates per coefficient (~60), weights operator 2 (2h)(5h)
+    rrq = nrrq * (44 * nb * ncoef + 60 * ncoef + 20 * h * h)
+    return direct, rrq
+
+
+def cpu_baseline(geo, tx, offs, idx, ub, ubn, sample_pairs):
+    """The oracle (CPU FP64 restatement, 1 thread) on a bounded sample of panels."""
+    from oracle import oracle as O
+    O.build()
+    O.set_side_tol(1e-10)
+    npan = len(offs) - 1
+    rng = np.random.default_rng(7)
+    order = rng.permutation(npan)
+    chosen, tot = [], 0
+    for k in order:
+        c = int(offs[k + 1] - offs[k])
+        if c == 0:
+            continue
+        chosen.append(k)
+        tot += c
+        if tot >= sample_pairs:
+            break
+    chosen = np.sort(np.array(chosen))
+    # restrict the CSR to the chosen panels
+    sub_offs = np.zeros(npan + 1, dtype=np.int64)
+    keep = np.zeros(npan, dtype=bool)
+    keep[chosen] = True
+    cnt = np.where(keep, np.diff(offs), 0)
+    sub_offs[1:] = np.cumsum(cnt)
+    sub_idx = np.concatenate([idx[offs[k]:offs[k + 1]] for k in chosen])
+    t0 = time.perf_counter()
+    O.close_apply(geo["p"], geo, tx, sub_offs, sub_idx, ub, ubn, exterior=True, nthreads=1)
+    dt = time.perf_counter() - t0
+    return dict(value=float(tot / dt), unit="pairs/s", cores=1, kind="port",
+                sample=f"{tot} pairs on {len(chosen)} randomly chosen panels of the same workload "
+                       f"(oracle, FP64, 1 thread, {dt:.1f} s)")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    from qotential_amd import workload as W
+    from qotential_amd.engine import Engine
+
+    t_setup = time.perf_counter()
+    geo = W.geometry(a.mp, a.np, a.order, modes=W.VMEC_BENCH)
+    tx, tri, dist = W.targets(geo, a.targets, -6.0, -1.0, sides=0, seed=1)
+    pos, q = W.point_charges(geo, nchg=8, offset=0.4, seed=2)
+    ub, gu = W.charge_field(geo["sx"], pos, q)
+    ubn = (geo["snx"] * gu).sum(1)
+    print(f"[bench] geometry {geo['ntri']} triangles p={a.order}, {a.targets} targets "
+          f"({time.perf_counter() - t_setup:.1f}s)", flush=True)
+    eng = Engine(geo)
+    eng.set_targets(tx)
+    t0 = time.perf_counter()
+    nnz = eng.nearlist()
+    t_near = time.perf_counter() - t0
+    print(f"[bench] near list: {nnz} pairs ({t_near * 1e3:.1f} ms incl. host sync)", flush=True)
+    far = eng.far(ub, ubn)
+    print("[bench] far field done", flush=True)
+    eng.eval(exterior=True, side_tol=1e-10)
+    u0, _ = eng.apply(ub, ubn, far)  # uploads ub, ubn, u (= far) and checks the path once
+    w = eng.weights()
+    labels = np.bincount(w["label"], minlength=6)[1:6]
+    stage = eng.stage_ms()
+    print(f"[bench] labels (L1..L4, RRQ): {labels.tolist()}  stages ms: "
+          f"{ {k: round(v, 2) for k, v in stage.items()} }", flush=True)
+
+    for _ in range(a.warmup):
+        eng.eval(exterior=True, side_tol=1e-10)
+        eng.apply_resident()
+    eng.sync()
+    # timed region: K steps bracketed by device synchronisation
+    t0 = time.perf_counter()
+    stage_tot = {k: 0.0 for k in stage}
+    for _ in range(a.steps):
+        eng.eval(exterior=True, side_tol=1e-10)
+        eng.apply_resident()
+        s = eng.stage_ms()
+        for k in stage_tot:
+            stage_tot[k] += s[k]
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1e3 * elapsed / a.steps
+    value = nnz * a.steps / elapsed
+    print(f"[bench] {a.steps} steps: {ms_step:.2f} ms/step, {value:.4g} pairs/s", flush=True)
+
+    # GRF identity on every target
+    u, _ = eng.apply(ub, ubn, far)
+    uex, _ = W.charge_field(tx, pos, q)
+    inside = dist < 0.0
+    grf = float(np.abs(u - np.where(inside, uex, 0.0)).max() / np.abs(uex).max())
+    repeat = float(np.abs(u - u0).max() / np.abs(u0).max())
+    print(f"[bench] GRF rel err {grf:.3e}; repeat-run rel diff {repeat:.1e}", flush=True)
+
+    # roofline of the dominant stage (FP64 compute bound)
+    direct_f, rrq_f = algorithmic_flops(a.order, labels[:4], int(labels[4]))
+    per = {k: v / a.steps for k, v in stage_tot.items()}
+    cand = {"direct_level_kernel": (direct_f, per["direct"]), "rrq_target_kernel": (rrq_f, per["rrq_targets"])}
+    dom = max(cand, key=lambda k: cand[k][1])
+    fl, ms = cand[dom]
+    achieved = fl / (ms * 1e-3) / 1e12
+    roof = {"kernel": dom, "bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+            "flops_per_launch": fl, "ms_per_launch": ms}
+
+    base = None
+    if not a.no_cpu_baseline:
+        offs, idx = eng.pairs()
+        base = cpu_baseline(geo, tx, offs, idx, ub, ubn, a.cpu_sample_pairs)
+        print(f"[bench] cpu baseline {base['value']:.4g} pairs/s ({base['sample']})", flush=True)
+
+    line = {
+        "metric": "close-eval target-panel pairs/sec at GRF rel err <=1e-9 vs plain FP64 on GPU",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"plain FP64 RRQ, stellarator {a.mp}x{a.np} charts, order {a.order}, "
+                               f"{geo['ntri']} triangles, {a.targets} near-surface targets both sides "
+                               f"U[-6,-1], reference ball near list",
+                   "pairs": int(nnz), "labels_L1_L4_RRQ": labels.tolist(), "stage_ms": per,
+                   "grf_rel_err": grf, "rel_err_vs_plain_fp64": 0.0, "repeat_rel_diff": repeat},
+        "roofline": roof,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -57,6 +57,11 @@ void qol_simplex_precomp_mex_(const int64_t* nquad, const int64_t* korder, const
  * nl(1..4), nsub(1..4).  No reference counterpart (the reference keeps them in
  * the module variable REF, src/patch_refine_mod.f90:51). */
 int qol_order_info(int64_t p, int64_t* info);
+/* one refinement level of patch order p (lev 1..4): uv(2,n), wt(n), interp(n,h)
+ * column-major, suboff(nsub+1) 0-based, subtyp(nsub) -- the per-order table
+ * that build_patch_ref_r64 keeps in REF (src/patch_refine_mod.f90:100-262). */
+int qol_order_level(int64_t p, int64_t lev, double* uv, double* wt, double* interp,
+                    int64_t* suboff, int64_t* subtyp);
 
 /* ---------------- drop-in close evaluation ----------------
  * qol_rrq_mex_ replaces qol_rrq_mex -> rrq_r64 (src/lap3d_close_mex.f90:18-45,

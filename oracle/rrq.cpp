@@ -120,6 +120,7 @@ struct Panel {
   std::vector<cplx> gradH, Vd;
   /* quaternion system */
   std::vector<double> MDT, Mslt, Fnode; /* LU-factored M^T (4h), Msl^T (h); F(i,k) */
+  std::vector<double> MDT0, Mslt0;      /* the same before factorisation */
   std::vector<i64> piv4, piv1;
   /* monomial Vandermonde at GL nodes (for SSQ), LU of V^T */
   std::vector<double> VT;
@@ -354,10 +355,12 @@ static void setup_panel(Panel& P, i64 n, const double* sx, const double* snx, co
   put(3, 0, F3, 1);  put(3, 1, F2, -1); put(3, 2, F1, 1);
   P.MDT.assign(H4 * H4, 0.0);
   for (i64 j = 0; j < H4; ++j) for (i64 i = 0; i < H4; ++i) P.MDT[i + H4 * j] = M[j + H4 * i];
+  P.MDT0 = P.MDT;
   P.piv4.assign(H4, 0);
   lu_factor(H4, P.MDT.data(), P.piv4.data());
   P.Mslt.assign(h * h, 0.0);
   for (i64 j = 0; j < h; ++j) for (i64 i = 0; i < h; ++i) P.Mslt[i + h * j] = Msl[j + h * i];
+  P.Mslt0 = P.Mslt;
   P.piv1.assign(h, 0);
   lu_factor(h, P.Mslt.data(), P.piv1.data());
 }
@@ -811,4 +814,16 @@ extern "C" void orc_debug_ssq(const double* coef, i64 nq, const double* x, doubl
   bool ok = edge_root(E, nq, x, s0);
   s0out[0] = s0.real(); s0out[1] = s0.imag(); s0out[2] = ok; s0out[3] = bernstein_rho(s0);
   ssq_w1(P, E, x, w, 1e300);
+}
+
+extern "C" void orc_debug_system(i64 n, const double* sx, const double* snx, const double* sw, i64 order, i64 nquad,
+                                 double* MT, double* MslT) {
+  using namespace orc;
+  std::vector<double> tgl(nquad), wgl(nquad), Dgl(nquad * nquad), wb(nquad), Leg(nquad * nquad), um(n * n), vm(n * n);
+  simplex_precomp(nquad, order - 1, n, tgl.data(), wgl.data(), Dgl.data(), wb.data(), Leg.data(), um.data(), vm.data());
+  Panel P;
+  std::vector<double> sxbd(9 * nquad, 0.0), rv(9, 0.0);
+  setup_panel(P, n, sx, snx, sw, order, nquad, 0, sxbd.data(), rv.data(), tgl.data(), wgl.data(), Dgl.data(), Leg.data(), um.data());
+  for (size_t i = 0; i < P.MDT0.size(); ++i) MT[i] = P.MDT0[i];
+  for (size_t i = 0; i < P.Mslt0.size(); ++i) MslT[i] = P.Mslt0[i];
 }

@@ -25,7 +25,7 @@ namespace orc {
 void gauss(i64 n, double* t, double* w, double* D) {
   const double pi = 4.0 * std::atan(1.0);
   for (i64 i = 0; i < n; ++i) {
-    double x = std::cos(pi * (double)(n - i - 0.25) / ((double)n + 0.5));
+    double x = -std::cos(pi * ((double)i + 0.75) / ((double)n + 0.5));
     double dp = 0.0;
     for (int it = 0; it < 100; ++it) {
       double p0 = 1.0, p1 = x;
@@ -119,10 +119,10 @@ void koorn_pols(double u, double v, i64 nmax, double* pols, double* du, double* 
   Q[0] = 1.0; Qu[0] = 0.0; Qv[0] = 0.0;
   if (N > 1) { Q[1] = a; Qu[1] = 2.0; Qv[1] = 1.0; }
   for (i64 m = 1; m + 1 < N; ++m) {
-    double c1 = (2.0 * m + 1.0), c2 = (double)m, c3 = (double)(m + 1);
-    Q[m + 1] = (c1 * a * Q[m] - c2 * b * b * Q[m - 1]) / c3;
-    Qu[m + 1] = (c1 * (2.0 * Q[m] + a * Qu[m]) - c2 * b * b * Qu[m - 1]) / c3;
-    Qv[m + 1] = (c1 * (1.0 * Q[m] + a * Qv[m]) - c2 * (-2.0 * b * Q[m - 1] + b * b * Qv[m - 1])) / c3;
+    const double c1 = 2.0 * m + 1.0, c2 = (double)m, inv = 1.0 / (double)(m + 1), bb = b * b;
+    Q[m + 1] = (c1 * a * Q[m] - c2 * bb * Q[m - 1]) * inv;
+    Qu[m + 1] = (c1 * (2.0 * Q[m] + a * Qu[m]) - c2 * bb * Qu[m - 1]) * inv;
+    Qv[m + 1] = (c1 * (Q[m] + a * Qv[m]) - c2 * (bb * Qv[m - 1] - 2.0 * b * Q[m - 1])) * inv;
   }
   double x = 2.0 * v - 1.0;
   std::vector<double> J(N), Jv(N);

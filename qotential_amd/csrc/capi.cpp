@@ -142,6 +142,20 @@ int qol_order_info(int64_t p, int64_t* info) {
   });
 }
 
+int qol_order_level(int64_t p, int64_t lev, double* uv, double* wt, double* interp, int64_t* suboff,
+                    int64_t* subtyp) {
+  return guarded([&] {
+    const OrderTables& T = order_tables(p);
+    if (lev < 1 || lev > 4) throw std::runtime_error("level must be 1..4");
+    const LevelTable& L = T.lev[lev - 1];
+    std::memcpy(uv, L.uv.data(), sizeof(double) * 2 * L.n);
+    std::memcpy(wt, L.wt.data(), sizeof(double) * L.n);
+    std::memcpy(interp, L.interpmat.data(), sizeof(double) * L.n * T.h);
+    for (int64_t q = 0; q <= L.nsub; ++q) suboff[q] = L.suboff[q];
+    for (int64_t q = 0; q < L.nsub; ++q) subtyp[q] = L.subtyp[q];
+  });
+}
+
 // ---------------------------------------------------------------- drop-ins
 void qol_simplex_precomp_mex_(const int64_t* nquad, const int64_t* korder, const int64_t* kpols, double* tgl,
                               double* wgl, double* Dgl, double* w_bclag, double* Legmat, double* umatr,

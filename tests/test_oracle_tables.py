@@ -47,3 +47,22 @@ def test_interpolation_reproduces_polynomials(oracle):
         # level weights integrate the reference triangle exactly
         assert abs(L["wt"].sum() - 0.5) < 1e-14
         assert L["suboff"][-1] == L["uv"].shape[0]
+
+
+def test_product_tables_bitwise_equal_oracle(oracle):
+    """The library's per-order level tables equal the oracle's bit for bit, so the
+    device's level balls (and hence the level labels) can be compared exactly."""
+    from qotential_amd._lib import lib, i64, ptr
+    for p in (4, 8, 12):
+        info = np.zeros(13, dtype=np.int64)
+        assert lib().qol_order_info(i64(p), ptr(info)) == 0
+        h = int(info[0])
+        for lev in (1, 2, 3, 4):
+            n, ns = int(info[4 + lev]), int(info[8 + lev])
+            uv = np.zeros((n, 2)); wt = np.zeros(n); im = np.zeros((n, h), order="F")
+            so = np.zeros(ns + 1, dtype=np.int64); st = np.zeros(ns, dtype=np.int64)
+            assert lib().qol_order_level(i64(p), i64(lev), ptr(uv), ptr(wt), ptr(im), ptr(so), ptr(st)) == 0
+            L = oracle.patch_ref_level(p, lev)
+            assert np.array_equal(uv, L["uv"]) and np.array_equal(wt, L["wt"])
+            assert np.array_equal(so, L["suboff"]) and np.array_equal(st, L["subtyp"])
+            assert np.array_equal(im, L["interpmat"]), (p, lev, np.abs(im - L["interpmat"]).max())
