@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--order", type=int, default=8)
     ap.add_argument("--targets", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-pairs", type=int, default=6000)
+    ap.add_argument("--cpu-sample-pairs", type=int, default=600000)
     return ap.parse_args()
 
 
