@@ -219,7 +219,6 @@ void Engine::get_pairs(i64* offs, i64* idx1) {
 }
 
 void Engine::build_levels() {
-  if (levels_built) return;
   d_lgeo.alloc(sizeof(double) * 7 * (size_t)tab->ntot * npan);
   d_balls.alloc(sizeof(double) * 4 * (size_t)tab->nsubtot * npan);
   const int nmax = std::max(tab->LA.lev_n[2], tab->LA.lev_n[3]);
@@ -239,15 +238,14 @@ void Engine::alloc_rrq() {
   d_bX.alloc(sizeof(double) * 6 * nb * np_);
   d_cx.alloc(sizeof(double) * 3 * (size_t)h * np_);
   d_basis.alloc(sizeof(double) * 12 * nb * ncoef * np_);
-  d_MT.alloc(sizeof(double) * 16 * (size_t)h * h * np_);
-  d_piv.alloc(sizeof(int) * 5 * (size_t)h * np_);
+  d_Fq.alloc(sizeof(double) * 3 * (size_t)h * h * np_);
   d_Msl.alloc(sizeof(double) * (size_t)h * h * np_);
   d_Fn.alloc(sizeof(double) * (size_t)h * h * np_);
   d_G.alloc(sizeof(double) * 10 * (size_t)h * h * np_);
   d_scr.alloc(sizeof(double) * 9 * (size_t)h * h * np_);
   RP.frame = (double*)d_frame.ptr; RP.edge = (double*)d_edge.ptr; RP.bX = (double*)d_bX.ptr;
-  RP.cx = (double*)d_cx.ptr; RP.basis = (double*)d_basis.ptr; RP.MT = (double*)d_MT.ptr;
-  RP.piv = (int*)d_piv.ptr; RP.Msl = (double*)d_Msl.ptr; RP.Fn = (double*)d_Fn.ptr;
+  RP.cx = (double*)d_cx.ptr; RP.basis = (double*)d_basis.ptr; RP.Fq = (double*)d_Fq.ptr;
+  RP.Msl = (double*)d_Msl.ptr; RP.Fn = (double*)d_Fn.ptr;
   RP.G = (double*)d_G.ptr; RP.scr = (double*)d_scr.ptr;
   rrq_alloc = true;
 }
@@ -305,11 +303,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                        (const int*)d_nrrq.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
                        (const double*)d_sw.ptr, ichart, (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
     HIPCHK(hipEventRecord(ev[4], stream));
-    const size_t lds2 = sizeof(double) * 4 * (size_t)h;
-    hipLaunchKernelGGL(rrq_panel_lu_kernel, dim3(npan), dim3(256), lds2, stream, tab->RT, RP, npan,
-                       (const int*)d_nrrq.ptr);
-    hipLaunchKernelGGL(rrq_panel_weights_kernel, dim3(npan), dim3(256), 0, stream, tab->RT, RP, npan,
-                       (const int*)d_nrrq.ptr);
+    launch_rrq_panel_solve(tab->RT, RP, npan, (const int*)d_nrrq.ptr, stream);
     HIPCHK(hipEventRecord(ev[5], stream));
     if (want_omega) {
       d_omega.alloc(sizeof(double) * 4 * (size_t)h * nnz);

@@ -35,9 +35,8 @@ struct RRQPanels {
   double* bX;      // [npan][3*nq][6] boundary X, dX (frame)
   double* cx;      // [npan][3][h] Koornwinder coefficients of the geometry (original coords)
   double* basis;   // [npan][3*nq][ncoef][12] grad H (3 complex) and V (3 complex)
-  double* MT;      // [npan][4h][4h] M^T, LU-factored in place
-  int* piv;        // [npan][5h] pivots of M^T (4h) and Msl^T (h)
-  double* Msl;     // [npan][h][h] Msl^T, LU-factored in place
+  double* Fq;      // [npan][h][h][3] pure quaternions grad F_k(y_i)
+  double* Msl;     // [npan][h][h] Msl^T (column-major)
   double* Fn;      // [npan][h][h] F_k at the nodes, F(i,k)
   double* G;       // [npan][2h][5h] weights operator
   double* scr;     // [npan][9h^2] scratch for the weights operator
@@ -75,8 +74,7 @@ __global__ void triplet_kernel(int nnz, int h, const int* tidx, const int* ppane
 __global__ void rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq, const double* sx,
                                        const double* snx, const double* sw, int ichart, const double* sxbd_chart,
                                        const double* rv_chart);
-__global__ void rrq_panel_lu_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq);
-__global__ void rrq_panel_weights_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq);
+void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s);
 __global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* offs, const int* tidx, const double* tx,
                                   const int8_t* label, const int* nrrq, int exterior, double side_tol,
                                   double* wout, double* omega_out, double* ias_out);

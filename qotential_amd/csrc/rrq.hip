@@ -8,6 +8,8 @@
 //            (SSQ weights, solid angle, omega moments: absent LQ/QA modules,
 //             re-derived -- see the oracle header for the identities used)
 // FP64 throughout.
+#include <stdexcept>
+
 #include "qol_kernels.h"
 
 namespace qol {
@@ -290,11 +292,12 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
   }
   __syncthreads();
   // quaternion system at the nodes.  F_k = Im((-i)^m H'_n^m), n=1..p, m=1..n.
-  double* MT = P.MT + (size_t)k * H4 * H4;
+  // Fq(i,k) = grad F_k(y_i) as a pure quaternion; the real 4h x 4h matrix of the
+  // reference (Mmatrix with its first two diagonal blocks zeroed, rrq_r64:851-858)
+  // is left multiplication by this h x h quaternion matrix.
+  double* FQ = P.Fq + (size_t)k * 3 * h * h;
   double* MS = P.Msl + (size_t)k * h * h;
   double* FN = P.Fn + (size_t)k * h * h;
-  for (int i = threadIdx.x; i < H4 * H4; i += blockDim.x) MT[i] = 0.0;
-  __syncthreads();
   for (int i0 = 0; i0 < h; i0 += 64) {
     const int i = i0 + threadIdx.x;
     if (threadIdx.x < 64 && i < h) {
@@ -312,12 +315,8 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
           const double F1 = imrot(m, g[0]), F2 = imrot(m, g[1]), F3 = imrot(m, g[2]);
           FN[i + (size_t)h * kk] = F;
           MS[kk + (size_t)h * i] = N[0] * F1 + N[1] * F2 + N[2] * F3;
-          // M blocks (rows: component a at node i; cols: component b of coefficient kk):
-          // a=0: [0,-F1,-F2,-F3]; a=1: [F1,0,-F3,F2]; a=2: [F2,F3,0,-F1]; a=3: [F3,-F2,F1,0]
-          const double blk[4][4] = {{0, -F1, -F2, -F3}, {F1, 0, -F3, F2}, {F2, F3, 0, -F1}, {F3, -F2, F1, 0}};
-          for (int a = 0; a < 4; ++a)
-            for (int bb = 0; bb < 4; ++bb)
-              if (a != bb) MT[(size_t)(bb * h + kk) + (size_t)H4 * (a * h + i)] = blk[a][bb];
+          double* q = FQ + 3 * ((size_t)i * h + kk);
+          q[0] = F1; q[1] = F2; q[2] = F3;
         }
     }
     __syncthreads();
@@ -325,150 +324,236 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
 }
 
 // ---------------------------------------------------------------------------
-// LU with partial pivoting (whole-row swaps) of M^T (4h) and Msl^T (h), in place
-__device__ void lu_inplace(double* A, int n, int* piv, double* prow) {
+// Register-resident Gauss-Jordan elimination with partial pivoting.  The n x m
+// matrix (elements real, Q=1, or quaternions, Q=4, combined by left
+// multiplication) is spread over a 16 x 16 thread grid, thread (ty,tx) holding
+// rows ty+16a and columns tx+16b.  Rows are exchanged whole; the pivot is the
+// largest |element| in the column (lowest row index on ties).
+template <int Q>
+__device__ __forceinline__ void emul(const double* a, const double* b, double* c) {
+  if constexpr (Q == 1) {
+    c[0] = a[0] * b[0];
+  } else {
+    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    c[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    c[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    c[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+  }
+}
+template <int Q>
+__device__ __forceinline__ double enorm(const double* a) {
+  if constexpr (Q == 1) return fabs(a[0]);
+  else return a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+}
+template <int Q>
+__device__ __forceinline__ void einv(const double* a, double* r) {
+  if constexpr (Q == 1) {
+    r[0] = 1.0 / a[0];
+  } else {
+    const double n2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+    r[0] = a[0] / n2; r[1] = -a[1] / n2; r[2] = -a[2] / n2; r[3] = -a[3] / n2;
+  }
+}
+
+template <int Q, int RPT, int CPT>
+__device__ void gauss_jordan(double (&A)[RPT][CPT][Q], int n, int m, double* srow, double* prow, double* nrow,
+                             double* colk, double* rv, int* ri) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   __shared__ int sip;
-  __shared__ double smax[4];
-  __shared__ int sidx[4];
-  for (int kcol = 0; kcol < n; ++kcol) {
-    // pivot search by the first wavefronts
-    double best = -1.0;
-    int bi = kcol;
-    for (int i = kcol + threadIdx.x; i < n; i += blockDim.x) {
-      const double v = fabs(A[i + (size_t)n * kcol]);
-      if (v > best) { best = v; bi = i; }
+  for (int k = 0; k < n; ++k) {
+    const int kt = k & 15, kb = k >> 4;
+    // pivot search in column k
+    if (tx == kt) {
+      double best = -1.0;
+      int bi = n;
+#pragma unroll
+      for (int a = 0; a < RPT; ++a) {
+        const int r = ty + 16 * a;
+#pragma unroll
+        for (int b = 0; b < CPT; ++b)
+          if (b == kb && r >= k && r < n) {
+            const double v = enorm<Q>(A[a][b]);
+            if (v > best) { best = v; bi = r; }
+          }
+      }
+      rv[ty] = best;
+      ri[ty] = bi;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ob = __shfl_xor(best, o);
-      const int oi = __shfl_xor(bi, o);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-    }
-    if ((threadIdx.x & 63) == 0) { smax[threadIdx.x >> 6] = best; sidx[threadIdx.x >> 6] = bi; }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double b0 = smax[0];
-      int i0 = sidx[0];
-      for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-        if (smax[w] > b0 || (smax[w] == b0 && sidx[w] < i0)) { b0 = smax[w]; i0 = sidx[w]; }
-      sip = i0;
-      piv[kcol] = i0;
+      double best = -1.0;
+      int bi = k;
+      for (int t = 0; t < 16; ++t)
+        if (rv[t] > best || (rv[t] == best && ri[t] < bi)) { best = rv[t]; bi = ri[t]; }
+      sip = bi;
     }
     __syncthreads();
     const int ip = sip;
-    if (ip != kcol)
-      for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const double t = A[kcol + (size_t)n * j];
-        A[kcol + (size_t)n * j] = A[ip + (size_t)n * j];
-        A[ip + (size_t)n * j] = t;
+    // rows k and ip to LDS
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+      const int r = ty + 16 * a;
+      if (r == k || r == ip)
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+          const int c = tx + 16 * b;
+          if (c < m)
+            for (int q = 0; q < Q; ++q) (r == ip ? prow : srow)[Q * c + q] = A[a][b][q];
+        }
+    }
+    __syncthreads();
+    // swap in registers, normalised pivot row, multipliers of column k
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+      const int r = ty + 16 * a;
+      if (ip != k && (r == k || r == ip))
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+          const int c = tx + 16 * b;
+          if (c < m)
+            for (int q = 0; q < Q; ++q) A[a][b][q] = (r == k ? prow : srow)[Q * c + q];
+        }
+    }
+    double inv[Q];
+    einv<Q>(prow + Q * k, inv);
+    for (int c = threadIdx.x; c < m; c += blockDim.x) emul<Q>(inv, prow + Q * c, nrow + Q * c);
+    if (tx == kt)
+#pragma unroll
+      for (int a = 0; a < RPT; ++a) {
+        const int r = ty + 16 * a;
+#pragma unroll
+        for (int b = 0; b < CPT; ++b)
+          if (b == kb && r < n)
+            for (int q = 0; q < Q; ++q) colk[Q * r + q] = A[a][b][q];
       }
     __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) prow[j] = A[kcol + (size_t)n * j];
-    __syncthreads();
-    const double pv = prow[kcol];
-    for (int i = kcol + 1 + threadIdx.x; i < n; i += blockDim.x) A[i + (size_t)n * kcol] = A[i + (size_t)n * kcol] / pv;
-    __syncthreads();
-    const int m = n - kcol - 1;
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-      const int i = kcol + 1 + e % m, j = kcol + 1 + e / m;
-      A[i + (size_t)n * j] = A[i + (size_t)n * j] - A[i + (size_t)n * kcol] * prow[j];
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+      const int r = ty + 16 * a;
+      if (r >= n) continue;
+      const double* mr = colk + Q * r;
+#pragma unroll
+      for (int b = 0; b < CPT; ++b) {
+        const int c = tx + 16 * b;
+        if (c >= m) continue;
+        if (r == k) {
+          for (int q = 0; q < Q; ++q) A[a][b][q] = nrow[Q * c + q];
+        } else {
+          double t[Q];
+          emul<Q>(mr, nrow + Q * c, t);
+          for (int q = 0; q < Q; ++q) A[a][b][q] = A[a][b][q] - t[q];
+        }
+      }
     }
     __syncthreads();
   }
 }
 
-__global__ void __launch_bounds__(256) rrq_panel_lu_kernel(RRQTables T, RRQPanels P, int npan,
-                                                           const int* __restrict__ nrrq) {
-  const int k = blockIdx.x;
-  if (nrrq[k] == 0) return;
-  extern __shared__ double prow[];
-  const int h = T.h;
-  lu_inplace(P.MT + (size_t)k * 16 * h * h, 4 * h, P.piv + (size_t)k * 5 * h, prow);
-  lu_inplace(P.Msl + (size_t)k * h * h, h, P.piv + (size_t)k * 5 * h + 4 * h, prow);
-}
-
-// Weights operator G (2h x 5h) acting on v = [rhs(4h); W(h)]:
-//   Ad = G1 rhs / 4pi,    G1 = first h rows of (M^T)^{-1}
-//   As = (Msl^T)^{-1} (W/4pi + Fn^T Ad) / alpha
-// Rows j<h: As, rows h..2h-1: Ad.  Scratch per panel (global, L2 resident):
-// Y (h x 4h, rows of G1), T1 = Fn^T G1 (h x 4h), B = (Msl^T)^{-1} (h x h).
-__global__ void __launch_bounds__(256) rrq_panel_weights_kernel(RRQTables T, RRQPanels P, int npan,
-                                                                const int* __restrict__ nrrq) {
+// Weights operator G (2h x 5h) acting on v = [rhs(4h); W(h)] from the quaternion
+// matrix Fq and Msl^T:
+//   X = Fq^{-1} (quaternion Gauss-Jordan on [Fq | I]); G1(i, (b,k)) = (X_ki)_b,
+//   so that Ad = G1 rhs / 4pi solves M^T w = rhs (first h components);
+//   B = (Msl^T)^{-1} (real Gauss-Jordan on [Msl^T | I]);
+//   As = B (W/4pi + Fn^T Ad) / alpha.
+// Rows j<h: As, rows h..2h-1: Ad.  Scratch per panel: G1 (h x 4h), T1 (h x 4h), B (h x h).
+template <int RPT, int CPT>
+__global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPanels P, int npan,
+                                                              const int* __restrict__ nrrq) {
   const int k = blockIdx.x;
   if (nrrq[k] == 0) return;
   const int h = T.h, H4 = 4 * h, H5 = 5 * h;
-  const double* A = P.MT + (size_t)k * H4 * H4;
-  const double* S = P.Msl + (size_t)k * h * h;
-  const int* piv = P.piv + (size_t)k * 5 * h;
+  __shared__ double srow[4 * 16 * CPT], prow[4 * 16 * CPT], nrow[4 * 16 * CPT], colk[4 * 16 * RPT];
+  __shared__ double rv[16];
+  __shared__ int ri[16];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const double* FQ = P.Fq + (size_t)k * 3 * h * h;
+  const double* MS = P.Msl + (size_t)k * h * h;
   const double* FN = P.Fn + (size_t)k * h * h;
   double* G = P.G + (size_t)k * 2 * h * H5;
-  double* y = P.scr + (size_t)k * 9 * h * h;  // [h][4h]
-  double* T1 = y + (size_t)4 * h * h;          // [h][4h]
-  double* Bm = T1 + (size_t)4 * h * h;         // [h][h], Bm[r*h + c] = B(c, r)
+  double* G1 = P.scr + (size_t)k * 9 * h * h;  // [h][4h]
+  double* T1 = G1 + (size_t)4 * h * h;          // [h][4h]
+  double* Bm = T1 + (size_t)4 * h * h;          // [h][h] B(r,c) at r*h+c
   const double alpha = P.frame[(size_t)k * 16 + 12];
   const double fac = 1.0 / (16.0 * atan(1.0));
-  // row r of (M^T)^{-1}: with P A = L U, A^T y = e_r is U^T z = e_r, L^T w = z, y = P^T w
-  for (int r = threadIdx.x; r < h; r += blockDim.x) {
-    double* v = y + (size_t)r * H4;
-    for (int i = 0; i < H4; ++i) v[i] = (i == r) ? 1.0 : 0.0;
-    for (int i = 0; i < H4; ++i) {
-      double s = v[i];
-      for (int q = 0; q < i; ++q) s -= A[q + (size_t)H4 * i] * v[q];
-      v[i] = s / A[i + (size_t)H4 * i];
-    }
-    for (int i = H4 - 1; i >= 0; --i) {
-      double s = v[i];
-      for (int q = i + 1; q < H4; ++q) s -= A[q + (size_t)H4 * i] * v[q];
-      v[i] = s;
-    }
-    for (int i = H4 - 1; i >= 0; --i) {
-      const int ip = piv[i];
-      if (ip != i) { const double t = v[i]; v[i] = v[ip]; v[ip] = t; }
-    }
+  {
+    double A[RPT][CPT][4];
+    for (int a = 0; a < RPT; ++a)
+#pragma unroll
+      for (int b = 0; b < CPT; ++b) {
+        const int r = ty + 16 * a, c = tx + 16 * b;
+        double* e = A[a][b];
+        e[0] = e[1] = e[2] = e[3] = 0.0;
+        if (r < h && c < h) {
+          const double* q = FQ + 3 * ((size_t)r * h + c);
+          e[1] = q[0]; e[2] = q[1]; e[3] = q[2];
+        } else if (r < h && c - h == r) {
+          e[0] = 1.0;
+        }
+      }
+    gauss_jordan<4, RPT, CPT>(A, h, 2 * h, srow, prow, nrow, colk, rv, ri);
+    for (int a = 0; a < RPT; ++a)
+#pragma unroll
+      for (int b = 0; b < CPT; ++b) {
+        const int kk = ty + 16 * a, c = tx + 16 * b;
+        if (kk < h && c >= h && c < 2 * h) {
+          const int i = c - h;
+          for (int q = 0; q < 4; ++q) G1[(size_t)i * H4 + q * h + kk] = A[a][b][q];
+        }
+      }
   }
-  // columns of B = (Msl^T)^{-1}
-  for (int r = threadIdx.x; r < h; r += blockDim.x) {
-    double* v = Bm + (size_t)r * h;
-    for (int i = 0; i < h; ++i) v[i] = (i == r) ? 1.0 : 0.0;
-    for (int i = 0; i < h; ++i) {
-      const int ip = piv[4 * h + i];
-      if (ip != i) { const double t = v[i]; v[i] = v[ip]; v[ip] = t; }
-    }
-    for (int i = 0; i < h; ++i) {
-      double s = v[i];
-      for (int q = 0; q < i; ++q) s -= S[i + (size_t)h * q] * v[q];
-      v[i] = s;
-    }
-    for (int i = h - 1; i >= 0; --i) {
-      double s = v[i];
-      for (int q = i + 1; q < h; ++q) s -= S[i + (size_t)h * q] * v[q];
-      v[i] = s / S[i + (size_t)h * i];
-    }
+  {
+    double A[RPT][CPT][1];
+    for (int a = 0; a < RPT; ++a)
+#pragma unroll
+      for (int b = 0; b < CPT; ++b) {
+        const int r = ty + 16 * a, c = tx + 16 * b;
+        A[a][b][0] = (r < h && c < h) ? MS[r + (size_t)h * c] : ((r < h && c - h == r) ? 1.0 : 0.0);
+      }
+    gauss_jordan<1, RPT, CPT>(A, h, 2 * h, srow, prow, nrow, colk, rv, ri);
+    for (int a = 0; a < RPT; ++a)
+#pragma unroll
+      for (int b = 0; b < CPT; ++b) {
+        const int r = ty + 16 * a, c = tx + 16 * b;
+        if (r < h && c >= h && c < 2 * h) Bm[(size_t)r * h + (c - h)] = A[a][b][0];
+      }
   }
   __threadfence_block();
   __syncthreads();
-  // T1 = Fn^T G1
   for (int e = threadIdx.x; e < h * H4; e += blockDim.x) {
     const int r = e / H4, c = e % H4;
     double t = 0.0;
-    for (int i = 0; i < h; ++i) t += FN[i + (size_t)h * r] * y[(size_t)i * H4 + c];
+    for (int i = 0; i < h; ++i) t += FN[i + (size_t)h * r] * G1[(size_t)i * H4 + c];
     T1[(size_t)r * H4 + c] = t;
   }
   __threadfence_block();
   __syncthreads();
   for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
     const int r = e / H5, c = e % H5;
-    G[(size_t)(h + r) * H5 + c] = c < H4 ? y[(size_t)r * H4 + c] * fac : 0.0;
+    G[(size_t)(h + r) * H5 + c] = c < H4 ? G1[(size_t)r * H4 + c] * fac : 0.0;
   }
   for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
     const int j = e / H5, c = e % H5;
     double s = 0.0;
     if (c < H4) {
-      for (int r = 0; r < h; ++r) s += Bm[(size_t)r * h + j] * T1[(size_t)r * H4 + c];
+      for (int r = 0; r < h; ++r) s += Bm[(size_t)j * h + r] * T1[(size_t)r * H4 + c];
     } else {
-      s = Bm[(size_t)(c - H4) * h + j];
+      s = Bm[(size_t)j * h + (c - H4)];
     }
     G[(size_t)j * H5 + c] = s * fac / alpha;
   }
+}
+
+void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s) {
+  const int h = T.h;
+  const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
+  if (rpt <= 3 && cpt <= 5)
+    hipLaunchKernelGGL((rrq_panel_solve_kernel<3, 5>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
+  else if (rpt <= 4 && cpt <= 7)
+    hipLaunchKernelGGL((rrq_panel_solve_kernel<4, 7>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
+  else if (rpt <= 5 && cpt <= 10)
+    hipLaunchKernelGGL((rrq_panel_solve_kernel<5, 10>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
+  else
+    throw std::runtime_error("RRQ solve supports patch orders up to 12");
 }
 
 // ---------------------------------------------------------------------------
