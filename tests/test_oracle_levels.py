@@ -1,0 +1,164 @@
+"""Pins of the oracle's tables, refinement levels, direct kernel and GRF
+harness (CPU only), by library routines, closed forms and brute force."""
+import numpy as np
+import pytest
+from numpy.polynomial import legendre as npleg
+
+from qotential_amd import workload as W
+
+
+@pytest.mark.parametrize("nq", [6, 10, 14])
+def test_simplex_precomp_1d_tables(oracle, nq):
+    pre = oracle.simplex_precomp(nq, 4)
+    t, w = pre["tgl"], pre["wgl"]
+    # nodes/weights against numpy's Gauss-Legendre
+    tr, wr = npleg.leggauss(nq)
+    assert np.abs(t - tr).max() < 1e-14 and np.abs(w - wr).max() < 1e-14
+    # Legmat maps values to Legendre coefficients (numpy legfit of the same data)
+    f = np.cos(2 * t) + t ** 3
+    assert np.abs(pre["Legmat"] @ f - npleg.legfit(t, f, nq - 1)).max() < 1e-12
+    # barycentric weights: interpolation is exact for degree < nq
+    wb = pre["w_bclag"]
+    g = lambda s: (s - 0.3) ** (nq - 1) + 2 * s
+    for s in (-0.77, 0.123, 0.9):
+        c = wb / (s - t)
+        assert abs((c @ g(t)) / c.sum() - g(s)) < 1e-12
+
+
+@pytest.mark.parametrize("p", [4, 8, 12])
+def test_koornwinder_orthonormal_and_inverse(oracle, p):
+    korder = p - 1
+    h = p * (p + 1) // 2
+    pre = oracle.simplex_precomp(p + 2, korder)
+    assert np.abs(pre["vmatr"] @ pre["umatr"] - np.eye(h)).max() < 1e-10
+    # orthonormal on the reference triangle: collapsed Gauss product rule
+    # u = a, v = (1 - a) b, dA = (1 - a) da db, exact for these degrees
+    g, gw = npleg.leggauss(p + 2)
+    g, gw = 0.5 * (g + 1), 0.5 * gw
+    uv = np.array([(a, (1 - a) * b) for a in g for b in g])
+    w = np.array([wa * wb * (1 - a) for a, wa in zip(g, gw) for wb in gw])
+    K = np.array([oracle.koorn_pols(u, v, korder)[0] for u, v in uv])
+    G = (K * w[:, None]).T @ K
+    assert np.abs(G - np.eye(h)).max() < 1e-12
+
+
+def test_direct_kernel_closed_form_and_coincident_zero(oracle):
+    geo = W.geometry(16, 40, 6, modes=W.VMEC_BENCH)
+    tx = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 1.0], geo["sx"][4321]])
+    As, Ad = oracle.lap3dsdlpmat(tx, geo["sx"], geo["snx"], geo["sw"])
+    # Gauss's law: -sum Ad = 1 inside, 0 outside
+    assert abs(-Ad[0].sum() - 1.0) < 1e-7 and abs(Ad[1].sum()) < 1e-7
+    # single entry against the kernel definition at one source
+    j = 777
+    d = tx[0] - geo["sx"][j]
+    r = np.linalg.norm(d)
+    assert abs(As[0, j] - geo["sw"][j] / (4 * np.pi * r)) < 1e-15 * abs(As[0, j]) * 10
+    assert abs(Ad[0, j] - geo["sw"][j] * d @ geo["snx"][j] / (4 * np.pi * r ** 3)) < 1e-14 * abs(Ad[0, j]) * 10
+    # coincident source: both kernels are zero
+    assert As[2, 4321] == 0.0 and Ad[2, 4321] == 0.0
+    assert np.isfinite(As[2]).all() and np.isfinite(Ad[2]).all()
+
+
+@pytest.mark.parametrize("p", [4, 8])
+def test_levels_vs_brute_force(oracle, p):
+    """Targets served by a refinement level get the direct rule on an exact
+    polynomial interpolant: per-weight agreement with brute force."""
+    h = p * (p + 1) // 2
+    geo = W.geometry(16, 40, p, modes=W.VMEC_BENCH)
+    k = 321
+    sl = slice(k * h, (k + 1) * h)
+    sx, snx, sw, rts, rps = (geo[key][sl] for key in ("sx", "snx", "sw", "rts", "rps"))
+    Lp = max(np.linalg.norm(sx[i] - sx[j]) for i in range(h) for j in range(h))
+    rng = np.random.default_rng(5)
+    xs = np.array([sx[rng.integers(h)] + s * 10 ** rng.uniform(-1.2, 0.3) * Lp * snx[0]
+                   for s in (1, -1) * 12])
+    As, Ad, idxs, lab = oracle.levels_eval(p, xs, sx, rts, rps)
+    served = lab <= 4
+    assert served.sum() >= 8 and (lab == 5).sum() >= 1
+    m = len(xs)
+    Ab = np.zeros((m, h), order="F"); Db = np.zeros((m, h), order="F"); om = np.zeros(m)
+    oracle.lib().orc_brute(oracle.i64(m), oracle._p(oracle._f(xs)), oracle.i64(p), oracle._p(oracle._f(sx)),
+                           oracle._p(oracle._f(snx)), oracle.i64(48), oracle.i64(20), oracle._p(Ab), oracle._p(Db),
+                           oracle._p(om))
+    # the level rule's accuracy is the order's (distff = 1.4 balls): measured
+    # per weight 2e-4 (p=4) / 7e-6 (p=8), on a smooth density 3e-5 / 1e-9
+    wtol, ftol = {4: (1e-3, 1e-4), 8: (3e-5, 5e-9)}[p]
+    f = np.exp(sx[:, 0] / 2) * np.cos(sx[:, 1] / 2)
+    for i in np.nonzero(served)[0]:
+        es = np.abs(As[i] - Ab[i]).max() / np.abs(Ab[i]).max()
+        ed = np.abs(Ad[i] - Db[i]).max() / np.abs(Db[i]).max()
+        fs = abs((As[i] - Ab[i]) @ f) / np.abs(Ab[i]).sum()
+        fd = abs((Ad[i] - Db[i]) @ f) / np.abs(Db[i]).sum()
+        assert max(es, ed) < wtol and max(fs, fd) < ftol, (i, lab[i], es, ed, fs, fd)
+    # the RRQ leftovers are listed in idxs, in target order (1-based)
+    assert list(idxs) == list(np.nonzero(lab == 5)[0] + 1)
+
+
+def test_level_geometry_area(oracle):
+    """Every level integrates the same polynomial patch: equal areas."""
+    p = 8
+    h = p * (p + 1) // 2
+    geo = W.geometry(16, 40, p, modes=W.VMEC_BENCH)
+    sl = slice(50 * h, 51 * h)
+    lv = oracle.patch_levels(p, geo["sx"][sl], geo["rts"][sl], geo["rps"][sl])
+    offs = np.concatenate([[0], np.cumsum(lv["nl"])])
+    areas = [lv["sw"][offs[l]:offs[l + 1]].sum() for l in range(4)]
+    # the area element |x_u x x_v| is not a polynomial: agreement to the rule accuracy
+    assert np.ptp(areas) < 1e-10 * areas[0]
+    assert abs(areas[0] - geo["sw"][sl].sum()) < 1e-9 * areas[0]
+
+
+def test_grf_fixture_order4_oracle(oracle, geo4, fixture4):
+    """The oracle's full GRF on the order-4 fixture surface (on-surface
+    targets, exterior limit, harmonic density): the reference reports
+    1.343e-3 for this case."""
+    ub, gu = W.harmonic_exp(geo4["sx"])
+    ubn = (geo4["snx"] * gu).sum(1)
+    # the fixture's own density at its sampled nodes
+    sel = fixture4["sel"]
+    assert np.abs(ub[sel] - fixture4["ub"]).max() < 1e-12
+    offs, idx = oracle.nearlist(10, geo4["sx"], geo4["sw"], geo4["sx"])
+    u = oracle.far_direct(geo4["sx"], geo4["sx"], geo4["snx"], geo4["sw"], ub, ubn, nthreads=8)
+    oracle.set_side_tol(1e300)
+    oracle.close_apply(4, geo4, geo4["sx"], offs, idx, ub, ubn, exterior=True, u=u, nthreads=8)
+    err = np.abs(u).max() / np.abs(ubn).max()
+    assert 5e-4 < err < 2e-3, err
+    # the residual field has the reference's shape: large where the reference's is
+    uf = fixture4["u"]
+    assert np.corrcoef(np.abs(u), np.abs(uf))[0, 1] > 0.5
+
+
+def test_getnearquad_triplets_match_close_apply(oracle):
+    """COO layout: triplets summed per target row equal the per-pair
+    near-correction values of the apply path."""
+    p = 4
+    h = 10
+    geo = W.geometry(12, 36, p)
+    offs, idx = oracle.nearlist(h, geo["sx"], geo["sw"], geo["sx"])
+    npan = geo["ntri"]
+    sel = set(range(0, npan, 29))
+    tcx_l, row_l, tcxi = [], [], [1]
+    sub_offs = [0]
+    sub_idx = []
+    for k in range(npan):
+        t = idx[offs[k]:offs[k + 1]] - 1 if k in sel else np.zeros(0, dtype=np.int64)
+        tcx_l.append(geo["sx"][t]); row_l.append(t + 1); tcxi.append(tcxi[-1] + len(t))
+        sub_idx.append(t + 1); sub_offs.append(sub_offs[-1] + len(t))
+    tcx = np.concatenate(tcx_l); tcxrow = np.concatenate(row_l); tcxi = np.array(tcxi)
+    oracle.set_side_tol(1e-10)
+    try:
+        si, sj, vs, vd = oracle.getnearquad(p, geo, tcx, tcxrow, tcxi, iside=30)
+        ub, gu = W.harmonic_exp(geo["sx"])
+        ubn = (geo["snx"] * gu).sum(1)
+        nt = geo["sx"].shape[0]
+        u_trip = np.zeros(nt)
+        np.add.at(u_trip, si - 1, vs * ubn[sj - 1] - vd * ub[sj - 1])
+        u_apply = np.zeros(nt)
+        oracle.close_apply(p, geo, geo["sx"], np.array(sub_offs), np.concatenate(sub_idx), ub, ubn, exterior=True,
+                           u=u_apply, nthreads=4)
+    finally:
+        oracle.set_side_tol(1e300)
+    assert np.abs(u_trip - u_apply).max() < 1e-12 * np.abs(u_apply).max()
+    # every triplet's column lies in the panel its row block belongs to
+    k_of = np.repeat(np.arange(npan), np.diff(tcxi) * h)
+    assert ((sj - 1) // h == k_of).all()
