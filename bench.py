@@ -152,7 +152,8 @@ def main():
     # roofline of the dominant stage (FP64 compute bound)
     direct_f, rrq_f = algorithmic_flops(a.order, labels[:4], int(labels[4]))
     per = {k: v / a.steps for k, v in stage_tot.items()}
-    cand = {"direct_level_kernel": (direct_f, per["direct"]), "rrq_target_kernel": (rrq_f, per["rrq_targets"])}
+    cand = {"direct_mfma_kernel": (direct_f, per["direct"]),
+            "rrq_project+edge+target": (rrq_f, per["rrq_project"] + per["rrq_edge"] + per["rrq_targets"])}
     dom = max(cand, key=lambda k: cand[k][1])
     fl, ms = cand[dom]
     achieved = fl / (ms * 1e-3) / 1e12

@@ -84,6 +84,17 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
     off += (int)L.n;
     sb += (int)L.nsub;
   }
+  // row-major copy padded to whole 16-column MFMA tiles
+  const int pad = 16 * ((h + 15) / 16);
+  std::vector<double> imp((size_t)pad * T.ntot, 0.0);
+  for (int l = 0; l < 4; ++l) {
+    const int n = LA.lev_n[l], o = LA.lev_off[l];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < h; ++j) imp[(size_t)(o + i) * pad + j] = interp[(size_t)h * o + i + (size_t)n * j];
+  }
+  upload(d_imp, imp.data(), imp.size(), s);
+  LA.imp = (const double*)d_imp.ptr;
+  LA.pad = pad;
   upload(d_interp, interp.data(), interp.size(), s);
   upload(d_wt, wt.data(), wt.size(), s);
   upload(d_suboff, suboff.data(), suboff.size(), s);
@@ -160,6 +171,26 @@ void Engine::alloc_pairs() {
   d_val.alloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
   d_nrrq.alloc(sizeof(int) * npan);
   d_lcount.alloc(sizeof(int) * 5 * npan);
+  d_lstart.alloc(sizeof(int) * 5 * npan);
+  d_ltot.alloc(sizeof(int) * 8);
+  d_lpairs.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+}
+
+// classification, per-label pair lists and the direct levels
+void Engine::classify_and_direct(bool timed) {
+  const size_t lds_b = sizeof(double) * 4 * (size_t)tab->nsubtot;
+  hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, (const int*)d_offs.ptr,
+                     (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_balls.ptr,
+                     (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr, (int*)d_lcount.ptr);
+  hipLaunchKernelGGL(level_scan_kernel, dim3(5), dim3(1024), 0, stream, (const int*)d_lcount.ptr, npan,
+                     (int*)d_lstart.ptr, (int*)d_ltot.ptr);
+  hipLaunchKernelGGL(compact_kernel, dim3(npan), dim3(256), 0, stream, (const int*)d_offs.ptr,
+                     (const int8_t*)d_label.ptr, (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan,
+                     (int*)d_lpairs.ptr);
+  if (timed) HIPCHK(hipEventRecord(ev[2], stream));
+  launch_direct_mfma(tab->LA, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                     (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr,
+                     nnz, stream);
 }
 
 i64 Engine::nearlist() {
@@ -253,18 +284,7 @@ void Engine::alloc_rrq() {
 void Engine::eval_levels_only() {
   if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
   build_levels();
-  if (nnz > 0) {
-    const size_t lds_b = sizeof(double) * 4 * (size_t)tab->nsubtot;
-    hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, (const int*)d_offs.ptr,
-                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_balls.ptr,
-                       (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr, (int*)d_lcount.ptr);
-    const size_t lds = direct_level_smem(h);
-    set_lds((const void*)direct_level_kernel, lds);
-    for (int l = 0; l < 4; ++l)
-      hipLaunchKernelGGL(direct_level_kernel, dim3(npan), dim3(256), lds, stream, tab->LA, l, (const int*)d_offs.ptr,
-                         (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const int8_t*)d_label.ptr,
-                         (const double*)d_lgeo.ptr, (double*)d_wout.ptr);
-  }
+  if (nnz > 0) classify_and_direct(false);
   HIPCHK(hipGetLastError());
   evaluated = true;
   have_omega = false;
@@ -277,22 +297,11 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   build_levels();
   HIPCHK(hipEventRecord(ev[1], stream));
   if (nnz > 0) {
-    const size_t lds_b = sizeof(double) * 4 * (size_t)tab->nsubtot;
-    hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, (const int*)d_offs.ptr,
-                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_balls.ptr,
-                       (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr, (int*)d_lcount.ptr);
+    classify_and_direct(true);
   } else {
     HIPCHK(hipMemsetAsync(d_nrrq.ptr, 0, sizeof(int) * npan, stream));
     HIPCHK(hipMemsetAsync(d_lcount.ptr, 0, sizeof(int) * 5 * npan, stream));
-  }
-  HIPCHK(hipEventRecord(ev[2], stream));
-  if (nnz > 0) {
-    const size_t lds = direct_level_smem(h);
-    set_lds((const void*)direct_level_kernel, lds);
-    for (int l = 0; l < 4; ++l)
-      hipLaunchKernelGGL(direct_level_kernel, dim3(npan), dim3(256), lds, stream, tab->LA, l, (const int*)d_offs.ptr,
-                         (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const int8_t*)d_label.ptr,
-                         (const double*)d_lgeo.ptr, (double*)d_wout.ptr);
+    HIPCHK(hipEventRecord(ev[2], stream));
   }
   HIPCHK(hipEventRecord(ev[3], stream));
   if (nnz > 0) {
@@ -311,25 +320,41 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
       HIPCHK(hipMemsetAsync(d_omega.ptr, 0, sizeof(double) * 4 * (size_t)h * nnz, stream));
       HIPCHK(hipMemsetAsync(d_ias.ptr, 0, sizeof(double) * (size_t)nnz, stream));
     }
+    const int ng = blocks((size_t)nnz, 256);
+    d_pq.alloc(sizeof(double) * 4 * (size_t)nnz);
+    d_w1.alloc(sizeof(double) * 3 * (size_t)nq * nnz);
+    d_oe.alloc(sizeof(double) * 3 * (size_t)nnz);
+    hipLaunchKernelGGL(rrq_project_kernel, dim3(ng), dim3(256), 0, stream, tab->RT, RP, (const int*)d_lpairs.ptr,
+                       (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr, (const int*)d_tidx.ptr,
+                       (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol, (double*)d_pq.ptr);
+    HIPCHK(hipEventRecord(ev[6], stream));
+    hipLaunchKernelGGL(rrq_edge_kernel, dim3(blocks(3 * (size_t)nnz, 256)), dim3(256), 0, stream, tab->RT, RP,
+                       (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
+                       (double*)d_oe.ptr);
+    HIPCHK(hipEventRecord(ev[7], stream));
     const size_t lds3 = rrq_target_smem(h, nq, ncoef);
     set_lds((const void*)rrq_target_kernel, lds3);
-    hipLaunchKernelGGL(rrq_target_kernel, dim3(npan), dim3(256), lds3, stream, tab->RT, RP, (const int*)d_offs.ptr,
-                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const int8_t*)d_label.ptr,
-                       (const int*)d_nrrq.ptr, exterior, side_tol, (double*)d_wout.ptr,
-                       want_omega ? (double*)d_omega.ptr : nullptr, want_omega ? (double*)d_ias.ptr : nullptr);
+    hipLaunchKernelGGL(rrq_target_kernel, dim3(npan), dim3(256), lds3, stream, tab->RT, RP, (const int*)d_lpairs.ptr,
+                       (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan, (const int*)d_tidx.ptr,
+                       (const double*)d_tx.ptr, (const int*)d_nrrq.ptr, (const double*)d_w1.ptr,
+                       (const double*)d_oe.ptr, (double*)d_wout.ptr, want_omega ? (double*)d_omega.ptr : nullptr,
+                       want_omega ? (double*)d_ias.ptr : nullptr);
   } else {
     HIPCHK(hipEventRecord(ev[4], stream));
     HIPCHK(hipEventRecord(ev[5], stream));
+    HIPCHK(hipEventRecord(ev[6], stream));
+    HIPCHK(hipEventRecord(ev[7], stream));
   }
-  HIPCHK(hipEventRecord(ev[6], stream));
+  HIPCHK(hipEventRecord(ev[8], stream));
   HIPCHK(hipGetLastError());
   evaluated = true;
   have_omega = want_omega;
 }
 
 void Engine::stage_ms(double* ms) {
-  HIPCHK(hipEventSynchronize(ev[6]));
-  for (int i = 0; i < 6; ++i) {
+  HIPCHK(hipEventSynchronize(ev[8]));
+  for (int i = 0; i < 8; ++i) {
     float t = 0.0f;
     HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
     ms[i] = t;

@@ -1,10 +1,12 @@
 // Refinement levels on the GPU: per-panel level geometry and sub-patch balls,
 // pair classification, and direct S/D on the upsampled levels contracted with the
-// interpolation matrix.
+// interpolation matrix on the FP64 matrix cores.
 //
 // Reference: build_patch_levels_r64   patch_refine_mod.f90:423-502
 //            lap3dsdlpmat_levels_r64  patch_refine_mod.f90:516-615
 //            lap3dsdlpmat_r64 (direct kernel; zero at coincident points)
+#include <stdexcept>
+
 #include "qol_kernels.h"
 
 namespace qol {
@@ -124,104 +126,188 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, const int* _
   if (threadIdx.x < 5) lcount[5 * k + threadIdx.x] = cnt[threadIdx.x];
 }
 
-// Direct S/D on level l for the pairs of one panel that classified to l, then
-// the contraction with the level's interpolation matrix:
-//   As(pair, j) = sum_i S(x, y_i) interp(i, j),  Ad likewise with the DLP kernel.
-// One workgroup per panel; targets in tiles of kT, level nodes in chunks of kNC
-// staged in LDS with the matching interpolation rows.
-constexpr int kT = 32, kNC = 64, kJG = 8;
 
-__global__ void __launch_bounds__(256) direct_level_kernel(LevelArgs A, int l, const int* __restrict__ offs,
-                                                           const int* __restrict__ tidx,
-                                                           const double* __restrict__ tx,
-                                                           const int8_t* __restrict__ label,
-                                                           const double* __restrict__ lgeo,
-                                                           double* __restrict__ wout) {
-  extern __shared__ double sm[];
-  const int h = A.h, k = blockIdx.x;
-  const int n = A.lev_n[l], off = A.lev_off[l];
-  double* sg = sm;                        // [7][kNC] level geometry chunk
-  double* sim = sg + 7 * kNC;             // [kNC][h] interpolation rows
-  double* sS = sim + (size_t)kNC * h;     // [kT][kNC]
-  double* sD = sS + kT * kNC;             // [kT][kNC]
-  double* stx = sD + kT * kNC;            // [kT][3]
-  __shared__ int plist[kT];
-  __shared__ int ntile;
-  const double pi4inv = 1.0 / (16.0 * atan(1.0));
-  const double* G = lgeo + (size_t)k * 7 * A.ntot;
-  const double* IM = A.interp + (size_t)h * off;
+// Per-label pair lists, stable (panel order, then pair order inside a panel):
+// lstart[l][k] = first slot of panel k's label-(l+1) pairs inside list l,
+// ltot[l] = length of list l.  One block per label.
+__global__ void __launch_bounds__(1024) level_scan_kernel(const int* __restrict__ lcount, int npan,
+                                                          int* __restrict__ lstart, int* __restrict__ ltot) {
+  __shared__ int part[1024];
+  const int l = blockIdx.x;
+  const int per = (npan + blockDim.x - 1) / blockDim.x;
+  const int a = threadIdx.x * per, b = min(npan, a + per);
+  int s = 0;
+  for (int k = a; k < b; ++k) s += lcount[5 * k + l];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) { const int v = part[i]; part[i] = acc; acc += v; }
+    ltot[l] = acc;
+  }
+  __syncthreads();
+  int acc = part[threadIdx.x];
+  for (int k = a; k < b; ++k) { lstart[(size_t)l * npan + k] = acc; acc += lcount[5 * k + l]; }
+}
+
+// Scatter every pair index into its label's list (lists concatenated in label
+// order: list l starts at ltot[0] + ... + ltot[l-1]).  One workgroup per panel;
+// order inside a panel kept by wave ballots.
+__global__ void __launch_bounds__(256) compact_kernel(const int* __restrict__ offs, const int8_t* __restrict__ label,
+                                                      const int* __restrict__ lstart, const int* __restrict__ ltot,
+                                                      int npan, int* __restrict__ lpairs) {
+  __shared__ int wcnt[4][5];
+  __shared__ int run[5];
+  const int k = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (threadIdx.x < 5) {
+    int base = 0;
+    for (int q = 0; q < (int)threadIdx.x; ++q) base += ltot[q];
+    run[threadIdx.x] = base + lstart[(size_t)threadIdx.x * npan + k];
+  }
+  __syncthreads();
   const int j0 = offs[k], j1 = offs[k + 1];
-  const int me_t = threadIdx.x / kJG, me_g = threadIdx.x % kJG;
-  int scan = j0;
-  while (true) {
-    // gather the next tile of pairs with this level (in pair order)
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int c = 0;
-      while (scan < j1 && c < kT) {
-        if (label[scan] == l + 1) plist[c++] = scan;
-        ++scan;
-      }
-      ntile = c;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int c = j0; c < j1; c += blockDim.x) {
+    const int j = c + threadIdx.x;
+    const int lab = j < j1 ? (int)label[j] : 0;
+    int rank = 0;
+    for (int q = 1; q <= 5; ++q) {
+      const unsigned long long bal = __ballot(lab == q);
+      if (lab == q) rank = __popcll(bal & below);
+      if (lane == 0) wcnt[wv][q - 1] = __popcll(bal);
     }
     __syncthreads();
-    const int nt = ntile;
-    if (nt == 0) break;
-    for (int i = threadIdx.x; i < 3 * nt; i += blockDim.x) stx[i] = tx[3 * (size_t)tidx[plist[i / 3]] + i % 3];
-    double accS[QOL_MAXH / kJG + 1], accD[QOL_MAXH / kJG + 1];
-    for (int q = 0; q < QOL_MAXH / kJG + 1; ++q) { accS[q] = 0.0; accD[q] = 0.0; }
-    for (int c0 = 0; c0 < n; c0 += kNC) {
-      const int nc = min(kNC, n - c0);
-      __syncthreads();
-      for (int i = threadIdx.x; i < 7 * kNC; i += blockDim.x) {
-        const int c = i / kNC, ii = i % kNC;
-        sg[i] = ii < nc ? G[(size_t)c * A.ntot + off + c0 + ii] : 0.0;
-      }
-      for (int i = threadIdx.x; i < kNC * h; i += blockDim.x) {
-        const int ii = i % kNC, j = i / kNC;
-        sim[ii * h + j] = ii < nc ? IM[(c0 + ii) + (size_t)n * j] : 0.0;
-      }
-      __syncthreads();
-      for (int i = threadIdx.x; i < kT * kNC; i += blockDim.x) {
-        const int t = i / kNC, ii = i % kNC;
-        double s = 0.0, d = 0.0;
-        if (t < nt && ii < nc) {
-          const double dx = stx[3 * t] - sg[ii], dy = stx[3 * t + 1] - sg[kNC + ii], dz = stx[3 * t + 2] - sg[2 * kNC + ii];
-          const double rr = dx * dx + dy * dy + dz * dz;
-          if (rr > 1.0e-28) {
-            const double r = sqrt(rr);
-            const double w = sg[6 * kNC + ii];
-            s = pi4inv * w / r;
-            d = pi4inv * w * (dx * sg[3 * kNC + ii] + dy * sg[4 * kNC + ii] + dz * sg[5 * kNC + ii]) / (r * rr);
-          }
-        }
-        sS[i] = s;
-        sD[i] = d;
-      }
-      __syncthreads();
-      if (me_t < nt) {
-        for (int ii = 0; ii < nc; ++ii) {
-          const double s = sS[me_t * kNC + ii], d = sD[me_t * kNC + ii];
-          int q = 0;
-          for (int j = me_g; j < h; j += kJG, ++q) {
-            const double v = sim[ii * h + j];
-            accS[q] = accS[q] + s * v;
-            accD[q] = accD[q] + d * v;
-          }
-        }
+    if (lab > 0) {
+      int pos = run[lab - 1] + rank;
+      for (int w = 0; w < wv; ++w) pos += wcnt[w][lab - 1];
+      lpairs[pos] = j;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+      int add = 0;
+      for (int w = 0; w < 4; ++w) add += wcnt[w][threadIdx.x];
+      run[threadIdx.x] += add;
+    }
+    __syncthreads();
+  }
+}
+
+// Direct S/D on the refinement levels with FP64 matrix cores.  One wave per
+// tile of 16 pairs of one level (tiles of the heaviest level first); for each
+// step of 4 level nodes every lane forms one kernel entry,
+//   S(pair m, node i) = w_i / (4 pi |x - y_i|),  D = w_i (x - y_i).n_i / (4 pi |x - y_i|^3)
+// (zero at coincident points), which is the A operand of
+// v_mfma_f64_16x16x4 against 4 rows of the level's interpolation matrix
+// (row-major, padded to 16*NT columns).  Accumulators: 16 pairs x 16*NT
+// outputs for S and for D.
+typedef double qd4 __attribute__((ext_vector_type(4)));
+
+template <int NT>
+__global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int* __restrict__ lpairs,
+                                                         const int* __restrict__ ltot,
+                                                         const int* __restrict__ ppanel,
+                                                         const int* __restrict__ tidx,
+                                                         const double* __restrict__ tx,
+                                                         const double* __restrict__ lgeo,
+                                                         double* __restrict__ wout) {
+  int b = blockIdx.x, l = -1;
+  int start[4];
+  {
+    int s = 0;
+    for (int q = 0; q < 4; ++q) { start[q] = s; s += ltot[q]; }
+  }
+  for (int q = 3; q >= 0; --q) {
+    const int ntile = (ltot[q] + 15) >> 4;
+    if (b < ntile) { l = q; break; }
+    b -= ntile;
+  }
+  if (l < 0) return;
+  const int h = A.h, ntot = A.ntot, pad = A.pad;
+  const int first = start[l] + 16 * b;
+  const int cnt = min(16, ltot[l] - 16 * b);
+  const int lane = threadIdx.x, m = lane & 15, kq = lane >> 4;
+  const bool live = m < cnt;
+  const int pair = live ? lpairs[first + m] : 0;
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+  const double* G = lgeo;
+  if (live) {
+    const size_t t = (size_t)tidx[pair];
+    x0 = tx[3 * t]; x1 = tx[3 * t + 1]; x2 = tx[3 * t + 2];
+    G = lgeo + (size_t)ppanel[pair] * 7 * ntot + A.lev_off[l];
+  }
+  const int n = A.lev_n[l];
+  const double* B = A.imp + (size_t)pad * A.lev_off[l];
+  const double c4 = 1.0 / (16.0 * atan(1.0));
+  qd4 accS[NT], accD[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    accS[q] = qd4{0.0, 0.0, 0.0, 0.0};
+    accD[q] = qd4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    const int i = k0 + kq;
+    const bool in = i < n;
+    double s = 0.0, d = 0.0;
+    if (live && in) {
+      const double dx = x0 - G[i], dy = x1 - G[ntot + i], dz = x2 - G[2 * ntot + i];
+      const double rr = dx * dx + dy * dy + dz * dz;
+      if (rr > 1.0e-28) {
+        // 1/r from the hardware estimate and two Newton steps
+        double y = __builtin_amdgcn_rsq(rr);
+        double t = rr * y;
+        y = fma(0.5 * y, fma(-t, y, 1.0), y);
+        t = rr * y;
+        y = fma(0.5 * y, fma(-t, y, 1.0), y);
+        const double cw = c4 * G[6 * ntot + i];
+        const double dn = dx * G[3 * ntot + i] + dy * G[4 * ntot + i] + dz * G[5 * ntot + i];
+        s = cw * y;
+        d = cw * dn * (y * y * y);
       }
     }
-    if (me_t < nt) {
-      double* o = wout + (size_t)plist[me_t] * 2 * h;
-      int q = 0;
-      for (int j = me_g; j < h; j += kJG, ++q) {
-        o[j] = accS[q];
-        o[h + j] = accD[q];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+      const double bv = in ? B[(size_t)i * pad + 16 * q + m] : 0.0;
+      accS[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(s, bv, accS[q], 0, 0, 0);
+      accD[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(d, bv, accD[q], 0, 0, 0);
+    }
+  }
+  // v_mfma_f64_16x16x4 operands and result: lane l supplies A[l%16][l/16] and
+  // B[l/16][l%16] and holds D[4r + l/16][l%16] in slot r, so this lane has
+  // pairs 4r + kq of output column 16q + m
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = 4 * r + kq;
+    const int pr = __shfl(pair, row, 64);
+    if (row < cnt) {
+      double* o = wout + (size_t)pr * 2 * h;
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        const int col = 16 * q + m;
+        if (col < h) {
+          o[col] = accS[q][r];
+          o[h + col] = accD[q][r];
+        }
       }
     }
   }
 }
 
-size_t direct_level_smem(int h) { return sizeof(double) * (7 * kNC + (size_t)kNC * h + 2 * kT * kNC + 3 * kT); }
+void launch_direct_mfma(LevelArgs A, const int* lpairs, const int* ltot, const int* ppanel, const int* tidx,
+                        const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s) {
+  const int grid = (int)((nnz + 15) / 16) + 4;
+  switch ((A.h + 15) / 16) {
+    case 1: hipLaunchKernelGGL(direct_mfma_kernel<1>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 2: hipLaunchKernelGGL(direct_mfma_kernel<2>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 3: hipLaunchKernelGGL(direct_mfma_kernel<3>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 4: hipLaunchKernelGGL(direct_mfma_kernel<4>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 5: hipLaunchKernelGGL(direct_mfma_kernel<5>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 6: hipLaunchKernelGGL(direct_mfma_kernel<6>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 7: hipLaunchKernelGGL(direct_mfma_kernel<7>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 8: hipLaunchKernelGGL(direct_mfma_kernel<8>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 9: hipLaunchKernelGGL(direct_mfma_kernel<9>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    default: throw std::runtime_error("direct level kernel: unsupported patch order");
+  }
+}
 
 }  // namespace qol

@@ -23,7 +23,7 @@ struct DeviceTables {
   int h = 0, ntot = 0, nsubtot = 0;
   LevelArgs LA{};
   RRQTables RT{};
-  DevBuf d_interp, d_wt, d_suboff, d_subtyp, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn;
+  DevBuf d_interp, d_imp, d_wt, d_suboff, d_subtyp, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn;
   DeviceTables(i64 p, hipStream_t s);
 };
 
@@ -35,12 +35,12 @@ struct Engine {
   bool levels_built = false, evaluated = false, have_omega = false, rrq_alloc = false;
   i64 tlists_nnz = -1;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[7] = {};
+  hipEvent_t ev[9] = {};
   std::unique_ptr<DeviceTables> tab;
   DevBuf d_sx, d_snx, d_sw, d_rts, d_rps, d_sxbd, d_rv;
   DevBuf d_tx, d_nball, d_cnt, d_offs, d_tidx;
-  DevBuf d_label, d_ppanel, d_nrrq, d_lcount, d_wout, d_val, d_omega, d_ias;
-  DevBuf d_lgeo, d_balls;
+  DevBuf d_label, d_ppanel, d_nrrq, d_lcount, d_lstart, d_ltot, d_lpairs, d_wout, d_val, d_omega, d_ias;
+  DevBuf d_lgeo, d_balls, d_pq, d_w1, d_oe;
   DevBuf d_frame, d_edge, d_bX, d_cx, d_basis, d_Fq, d_Msl, d_Fn, d_G, d_scr;
   DevBuf d_ub, d_ubn, d_u, d_tcount, d_toffs, d_tcur, d_tpairs;
   RRQPanels RP{};
@@ -57,6 +57,7 @@ struct Engine {
   void alloc_rrq();
   void eval(int exterior, double side_tol, bool want_omega);
   void eval_levels_only();
+  void classify_and_direct(bool timed);
   void stage_ms(double* ms);
   void get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias);
   void apply(const double* ub, const double* ubn, double* u, double* val, bool device_ptrs);

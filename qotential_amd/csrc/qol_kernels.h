@@ -12,6 +12,8 @@ struct LevelArgs {
   int h, ntot, nsubtot;
   int lev_n[4], lev_off[4], lev_nsub[4], sub_base[4];
   const double* interp;  // level l at interp + h*lev_off[l], column-major lev_n[l] x h
+  const double* imp;     // level l at imp + pad*lev_off[l], row-major lev_n[l] x pad, zero-padded
+  int pad;               // 16 * ceil(h / 16)
   const double* wt;      // ntot
   const int* suboff;     // per level (nsub+1) node offsets, concatenated
   const int* subtyp;     // nsubtot
@@ -42,14 +44,16 @@ struct RRQPanels {
   double* scr;     // [npan][9h^2] scratch for the weights operator
 };
 
-size_t direct_level_smem(int h);
-
 __global__ void level_geom_kernel(LevelArgs A, const double* sx, const double* rts, const double* rps,
                                   double* lgeo, double* balls);
 __global__ void classify_kernel(LevelArgs A, const int* offs, const int* tidx, const double* tx,
                                 const double* balls, int8_t* label, int* ppanel, int* nrrq, int* lcount);
-__global__ void direct_level_kernel(LevelArgs A, int l, const int* offs, const int* tidx, const double* tx,
-                                    const int8_t* label, const double* lgeo, double* wout);
+
+__global__ void level_scan_kernel(const int* lcount, int npan, int* lstart, int* ltot);
+__global__ void compact_kernel(const int* offs, const int8_t* label, const int* lstart, const int* ltot, int npan,
+                               int* lpairs);
+void launch_direct_mfma(LevelArgs A, const int* lpairs, const int* ltot, const int* ppanel, const int* tidx,
+                        const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s);
 
 __global__ void nearlist_prep_kernel(int npan, int h, const double* sx, const double* sw, double* ball);
 __global__ void nearlist_count_kernel(int npan, int nt, const double* tx, const double* ball, int* count);
@@ -75,9 +79,14 @@ __global__ void rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan, const
                                        const double* snx, const double* sw, int ichart, const double* sxbd_chart,
                                        const double* rv_chart);
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s);
-__global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* offs, const int* tidx, const double* tx,
-                                  const int8_t* label, const int* nrrq, int exterior, double side_tol,
-                                  double* wout, double* omega_out, double* ias_out);
+__global__ void rrq_project_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
+                                   const int* tidx, const double* tx, const double* sx, int exterior,
+                                   double side_tol, double* pq);
+__global__ void rrq_edge_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
+                                const int* tidx, const double* tx, const double* pq, double* w1, double* oe);
+__global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot,
+                                  int npan, const int* tidx, const double* tx, const int* nrrq, const double* w1g,
+                                  const double* oeg, double* wout, double* omega_out, double* ias_out);
 size_t rrq_setup_smem(int h, int nq, int ncoef);
 size_t rrq_target_smem(int h, int nq, int ncoef);
 

@@ -31,27 +31,26 @@ __device__ void geom_eval(const double* cx, int h, int korder, double u, double 
   }
   for (int c = 0; c < 3; ++c) { x[c] = 0.0; if (xu) { xu[c] = 0.0; xv[c] = 0.0; } }
   const double xx = 2.0 * v - 1.0;
-  int idx = 0;
-  for (int d = 0; d < N; ++d) {
-    for (int bd = 0; bd <= d; ++bd) {
-      const int am = d - bd;
-      const double al = 2.0 * am + 1.0;
-      double p = 1.0, dp = 0.0;
-      if (bd >= 1) {
-        double pm = 1.0, dpm = 0.0;
+  // Jacobi P_bd^(2am+1,0)(xx) by its three-term recurrence in bd, one sweep per am;
+  // basis index of (degree d = am + bd, bd) is d(d+1)/2 + bd
+  for (int am = 0; am < N; ++am) {
+    const double al = 2.0 * am + 1.0;
+    double p = 1.0, dp = 0.0, pm = 1.0, dpm = 0.0;
+    for (int bd = 0; am + bd < N; ++bd) {
+      if (bd == 1) {
         p = 0.5 * ((al + 2.0) * xx + al);
         dp = al + 2.0;
-        for (int k = 1; k < bd; ++k) {
-          const double dk = (double)k;
-          const double a1 = 2.0 * (dk + 1.0) * (dk + al + 1.0) * (2.0 * dk + al);
-          const double a2 = (2.0 * dk + al + 1.0) * al * al;
-          const double a3 = (2.0 * dk + al) * (2.0 * dk + al + 1.0) * (2.0 * dk + al + 2.0);
-          const double a4 = 2.0 * (dk + al) * dk * (2.0 * dk + al + 2.0);
-          const double pn = ((a2 + a3 * xx) * p - a4 * pm) / a1;
-          const double dpn = ((a2 + a3 * xx) * dp + 2.0 * a3 * p - a4 * dpm) / a1;
-          pm = p; p = pn; dpm = dp; dp = dpn;
-        }
+      } else if (bd >= 2) {
+        const double dk = (double)(bd - 1);
+        const double a1 = 2.0 * (dk + 1.0) * (dk + al + 1.0) * (2.0 * dk + al);
+        const double a2 = (2.0 * dk + al + 1.0) * al * al;
+        const double a3 = (2.0 * dk + al) * (2.0 * dk + al + 1.0) * (2.0 * dk + al + 2.0);
+        const double a4 = 2.0 * (dk + al) * dk * (2.0 * dk + al + 2.0);
+        const double pn = ((a2 + a3 * xx) * p - a4 * pm) / a1;
+        const double dpn = ((a2 + a3 * xx) * dp + 2.0 * a3 * p - a4 * dpm) / a1;
+        pm = p; p = pn; dpm = dp; dp = dpn;
       }
+      const int d = am + bd, idx = d * (d + 1) / 2 + bd;
       const double sc = sqrt((2.0 * am + 1.0) * 2.0 * (double)(d + 1));
       const double f = sc * Q[am] * p;
       for (int c = 0; c < 3; ++c) x[c] += f * cx[c * h + idx];
@@ -59,7 +58,6 @@ __device__ void geom_eval(const double* cx, int h, int korder, double u, double 
         const double fu = sc * Qu[am] * p, fv = sc * (Qv[am] * p + Q[am] * dp);
         for (int c = 0; c < 3; ++c) { xu[c] += fu * cx[c * h + idx]; xv[c] += fv * cx[c * h + idx]; }
       }
-      ++idx;
     }
   }
 }
@@ -669,13 +667,14 @@ __device__ double solid_angle_edge(const double* cf, int nq, const double* tgl, 
   return total;
 }
 
-// Gauss-Newton projection of x (original coordinates) onto the patch; returns
-// the signed distance along the uv normal and that normal
+// Gauss-Newton projection of x (original coordinates) onto the patch, started
+// at the nearest node (ssx: the panel's nodes, x(3,h)); returns the signed
+// distance along the uv normal and that normal
 __device__ double project(const double* cx, int h, int korder, const double* ssx, const double* uvn,
                           const double* x, double* nstar) {
   double best = 1e300, u = 1.0 / 3.0, v = 1.0 / 3.0;
   for (int i = 0; i < h; ++i) {
-    const double d0 = ssx[i] - x[0], d1 = ssx[h + i] - x[1], d2 = ssx[2 * h + i] - x[2];
+    const double d0 = ssx[3 * i] - x[0], d1 = ssx[3 * i + 1] - x[1], d2 = ssx[3 * i + 2] - x[2];
     const double d = d0 * d0 + d1 * d1 + d2 * d2;
     if (d < best) { best = d; u = uvn[2 * i]; v = uvn[2 * i + 1]; }
   }
@@ -710,81 +709,127 @@ __device__ double project(const double* cx, int h, int korder, const double* ssx
   return (x[0] - y[0]) * nstar[0] + (x[1] - y[1]) * nstar[1] + (x[2] - y[2]) * nstar[2];
 }
 
+// ---------------------------------------------------------------------------
+// Per-target work in three launches over the RRQ pair list (label 5, grouped by
+// panel; r = position in that list):
+//   rrq_project_kernel  one thread per pair: projection onto the patch, side,
+//                       Dirac-string direction q (frame)            -> pq[r][4]
+//   rrq_edge_kernel     one thread per (pair, edge): complex root, SSQ weights
+//                       for the 1/r line kernel and the edge's solid-angle
+//                       line integral                              -> w1[r][3nq], oe[r][3]
+//   rrq_target_kernel   one workgroup per panel, pairs in tiles: boundary
+//                       moments, degree recursion, weights operator.
+
+__device__ __forceinline__ int rrq_list_base(const int* ltot) { return ltot[0] + ltot[1] + ltot[2] + ltot[3]; }
+
+__global__ void __launch_bounds__(256) rrq_project_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
+                                                          const int* __restrict__ ltot,
+                                                          const int* __restrict__ ppanel,
+                                                          const int* __restrict__ tidx,
+                                                          const double* __restrict__ tx,
+                                                          const double* __restrict__ sx, int exterior,
+                                                          double side_tol, double* __restrict__ pq) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ltot[4]) return;
+  const int pair = lpairs[rrq_list_base(ltot) + r];
+  const int k = ppanel[pair], h = T.h;
+  const double* fr = P.frame + (size_t)k * 16;
+  const size_t g = (size_t)tidx[pair];
+  const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
+  double ns[3];
+  const double d = project(P.cx + (size_t)k * 3 * h, h, T.p - 1, sx + (size_t)k * 3 * h, T.uvn, x, ns);
+  double side;
+  if (fabs(d) > side_tol * fr[14]) side = d > 0.0 ? 1.0 : -1.0;
+  else side = (exterior ? 1.0 : -1.0) * fr[13];
+  const double qo[3] = {-side * ns[0], -side * ns[1], -side * ns[2]};
+  double q[3];
+  rotate(fr, qo, q);
+  double* o = pq + 4 * (size_t)r;
+  o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = side;
+}
+
+__global__ void __launch_bounds__(256) rrq_edge_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
+                                                       const int* __restrict__ ltot, const int* __restrict__ ppanel,
+                                                       const int* __restrict__ tidx, const double* __restrict__ tx,
+                                                       const double* __restrict__ pq, double* __restrict__ w1,
+                                                       double* __restrict__ oe) {
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= 3 * ltot[4]) return;
+  const int r = it / 3, e = it % 3, nq = T.nq;
+  const int pair = lpairs[rrq_list_base(ltot) + r];
+  const int k = ppanel[pair];
+  const double* fr = P.frame + (size_t)k * 16;
+  const size_t g = (size_t)tidx[pair];
+  const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
+  double X[3];
+  to_frame(fr, x, X);
+  const double* cf = P.edge + (size_t)k * 9 * nq + e * 3 * nq;
+  const double rho_thr = pow(8.0, 8.0 / (double)nq);
+  cd root;
+  bool ok;
+  ssq_w1(cf, nq, T.tgl, T.wgl, X, rho_thr, w1 + (size_t)r * 3 * nq + e * nq, &root, &ok);
+  const double q[3] = {pq[4 * (size_t)r], pq[4 * (size_t)r + 1], pq[4 * (size_t)r + 2]};
+  oe[3 * (size_t)r + e] = solid_angle_edge(cf, nq, T.tgl, T.wgl, X, q, root, ok, rho_thr);
+}
+
+// LDS plan of rrq_target_kernel: fixed panel data plus per-target slots
+__host__ __device__ inline size_t rrq_target_fixed(int h, int nq) {
+  return (size_t)(16 + 6 * 3 * nq) * 8;
+}
+__host__ __device__ inline size_t rrq_target_per(int h, int nq, int ncoef) {
+  const int nb = 3 * nq, big = 6 * nb > 5 * h ? 6 * nb : 5 * h;
+  return (size_t)(6 + nb + big + 4) * 8 + (size_t)ncoef * 5 * 16;
+}
 __host__ __device__ int rrq_tile(int h, int nq, int ncoef) {
-  const size_t per = (size_t)(6 + 3 * nq + 3 * nq * 6 + 5 * h + 8) * 8 + (size_t)ncoef * 5 * 16;
-  const size_t fixed = (size_t)(7 * h + 16 + 9 * nq + 6 * 3 * nq + 2 * nq + 2 * h) * 8;
-  int t = (int)((150 * 1024 - fixed) / per);
+  // about 48 KB per workgroup keeps three workgroups on a CU
+  const size_t budget = 48 * 1024, fixed = rrq_target_fixed(h, nq), per = rrq_target_per(h, nq, ncoef);
+  int t = budget > fixed + per ? (int)((budget - fixed) / per) : 1;
   return t > 16 ? 16 : (t < 1 ? 1 : t);
 }
-
 size_t rrq_target_smem(int h, int nq, int ncoef) {
-  const int T = rrq_tile(h, nq, ncoef);
-  const size_t per = (size_t)(6 + 3 * nq + 3 * nq * 6 + 5 * h + 8) * 8 + (size_t)ncoef * 5 * 16;
-  const size_t fixed = (size_t)(7 * h + 16 + 9 * nq + 6 * 3 * nq + 2 * nq + 2 * h) * 8;
-  return fixed + per * T;
+  return rrq_target_fixed(h, nq) + rrq_target_per(h, nq, ncoef) * rrq_tile(h, nq, ncoef);
 }
 
-// One workgroup per panel with RRQ pairs; the panel's RRQ pairs in tiles.
-__global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels P, const int* __restrict__ offs,
-                                                         const int* __restrict__ tidx, const double* __restrict__ tx,
-                                                         const int8_t* __restrict__ label, const int* __restrict__ nrrq,
-                                                         int exterior, double side_tol, double* __restrict__ wout,
-                                                         double* __restrict__ omega_out, double* __restrict__ ias_out) {
+__global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
+                                                         const int* __restrict__ lstart, const int* __restrict__ ltot,
+                                                         int npan, const int* __restrict__ tidx,
+                                                         const double* __restrict__ tx, const int* __restrict__ nrrq,
+                                                         const double* __restrict__ w1g, const double* __restrict__ oeg,
+                                                         double* __restrict__ wout, double* __restrict__ omega_out,
+                                                         double* __restrict__ ias_out) {
   const int k = blockIdx.x;
-  if (nrrq[k] == 0) return;
+  const int nmine = nrrq[k];
+  if (nmine == 0) return;
   const int h = T.h, nq = T.nq, p = T.p, nb = 3 * nq, ncoef = T.ncoef, H4 = 4 * h, H5 = 5 * h;
   const int TT = rrq_tile(h, nq, ncoef);
+  const int big = 6 * nb > H5 ? 6 * nb : H5;
   extern __shared__ double sm[];
-  double* ssx = sm;                 // [3][h]
-  double* scx = ssx + 3 * h;        // [3][h]
-  double* fr = scx + 3 * h;         // 16
-  double* ecf = fr + 16;            // [3][3][nq]
-  double* bx = ecf + 9 * nq;        // [nb][6]
-  double* stg = bx + 6 * nb;        // tgl[nq], wgl[nq]
-  double* swg = stg + nq;
-  double* suv = swg + nq;           // [2h]
-  double* tX = suv + 2 * h;         // [TT][6]   frame X, original x
-  double* w1 = tX + 6 * TT;         // [TT][nb]
-  double* ab = w1 + (size_t)nb * TT;  // [TT][nb][6]
-  double* vv = ab + (size_t)6 * nb * TT;  // [TT][5h]
-  double* tq = vv + (size_t)H5 * TT;      // [TT][8]: q(3), omega parts(3), spare
-  cd* mom = (cd*)(tq + 8 * TT);           // [TT][ncoef][5]
+  double* fr = sm;                        // 16
+  double* bx = fr + 16;                   // [nb][6]
+  double* tX = bx + 6 * nb;               // [TT][6]   frame X, original x
+  double* w1 = tX + 6 * TT;               // [TT][nb]
+  double* ab = w1 + (size_t)nb * TT;      // [TT][big]: a,b per node, later the moment vector v
+  double* tq = ab + (size_t)big * TT;     // [TT][4]: solid angle, spare
+  cd* mom = (cd*)(tq + 4 * TT);           // [TT][ncoef][5]
   __shared__ int plist[16];
-  __shared__ int ntile;
-  __shared__ cd sroot[16][3];
-  __shared__ int srok[16][3];
-  for (int i = threadIdx.x; i < 3 * h; i += blockDim.x) scx[i] = P.cx[(size_t)k * 3 * h + i];
   for (int i = threadIdx.x; i < 16; i += blockDim.x) fr[i] = P.frame[(size_t)k * 16 + i];
-  for (int i = threadIdx.x; i < 9 * nq; i += blockDim.x) ecf[i] = P.edge[(size_t)k * 9 * nq + i];
   for (int i = threadIdx.x; i < 6 * nb; i += blockDim.x) bx[i] = P.bX[(size_t)k * nb * 6 + i];
-  for (int i = threadIdx.x; i < nq; i += blockDim.x) { stg[i] = T.tgl[i]; swg[i] = T.wgl[i]; }
-  for (int i = threadIdx.x; i < 2 * h; i += blockDim.x) suv[i] = T.uvn[i];
-  __syncthreads();
-  // node positions from the geometry polynomial at the VR nodes
-  for (int i = threadIdx.x; i < h; i += blockDim.x) {
-    double x[3];
-    geom_eval(scx, h, p - 1, suv[2 * i], suv[2 * i + 1], x, nullptr, nullptr);
-    for (int c = 0; c < 3; ++c) ssx[c * h + i] = x[c];
-  }
-  const double rho_thr = pow(8.0, 8.0 / (double)nq);
-  const double sigma = fr[13], alpha = fr[12], L = fr[14];
+  const double sigma = P.frame[(size_t)k * 16 + 13], alpha = P.frame[(size_t)k * 16 + 12];
   const double* G = P.G + (size_t)k * 2 * h * H5;
   const double* basis = P.basis + (size_t)k * nb * ncoef * 12;
-  const int j0 = offs[k], j1 = offs[k + 1];
-  int scan = j0;
-  while (true) {
+  const int r0 = lstart[4 * (size_t)npan + k];
+  const int* mine = lpairs + rrq_list_base(ltot) + r0;
+  for (int tile0 = 0; tile0 < nmine; tile0 += TT) {
+    const int nt = min(TT, nmine - tile0);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int c = 0;
-      while (scan < j1 && c < TT) {
-        if (label[scan] == 5) plist[c++] = scan;
-        ++scan;
-      }
-      ntile = c;
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) plist[t] = mine[tile0 + t];
+    for (int i = threadIdx.x; i < nt * nb; i += blockDim.x)
+      w1[i] = w1g[(size_t)(r0 + tile0) * nb + i];
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+      const double* o = oeg + 3 * (size_t)(r0 + tile0 + t);
+      tq[4 * t] = o[0] + o[1] + o[2];
     }
     __syncthreads();
-    const int nt = ntile;
-    if (nt == 0) break;
     for (int t = threadIdx.x; t < nt; t += blockDim.x) {
       const size_t g = (size_t)tidx[plist[t]];
       const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
@@ -792,31 +837,6 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       tX[6 * t + 3] = x[0]; tX[6 * t + 4] = x[1]; tX[6 * t + 5] = x[2];
     }
     __syncthreads();
-    // SSQ weights, one thread per (target, edge)
-    for (int it = threadIdx.x; it < 3 * nt; it += blockDim.x) {
-      const int t = it / 3, e = it % 3;
-      cd r;
-      bool ok;
-      ssq_w1(ecf + 9 * nq * 0 + e * 3 * nq, nq, stg, swg, tX + 6 * t, rho_thr, w1 + (size_t)t * nb + e * nq, &r, &ok);
-      sroot[t][e] = r;
-      srok[t][e] = ok ? 1 : 0;
-    }
-    // side of each target and the Dirac-string direction (frame)
-    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-      double ns[3];
-      const double d = project(scx, h, p - 1, ssx, suv, tX + 6 * t + 3, ns);
-      double side;
-      if (fabs(d) > side_tol * L) side = d > 0.0 ? 1.0 : -1.0;
-      else side = (exterior ? 1.0 : -1.0) * sigma;
-      const double qo[3] = {-side * ns[0], -side * ns[1], -side * ns[2]};
-      rotate(fr, qo, tq + 8 * t);
-    }
-    __syncthreads();
-    for (int it = threadIdx.x; it < 3 * nt; it += blockDim.x) {
-      const int t = it / 3, e = it % 3;
-      tq[8 * t + 3 + e] = solid_angle_edge(ecf + e * 3 * nq, nq, stg, swg, tX + 6 * t, tq + 8 * t, sroot[t][e],
-                                           srok[t][e] != 0, rho_thr);
-    }
     // a = (w1/R)(z x dX), b = (w1/R) z
     for (int it = threadIdx.x; it < nt * nb; it += blockDim.x) {
       const int t = it / nb, ib = it % nb;
@@ -825,7 +845,7 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       const double wr = w1[(size_t)t * nb + ib] / sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
       double zx[3];
       cross3(z, Y + 3, zx);
-      double* o = ab + ((size_t)t * nb + ib) * 6;
+      double* o = ab + (size_t)t * big + 6 * ib;
       for (int c = 0; c < 3; ++c) { o[c] = wr * zx[c]; o[3 + c] = wr * z[c]; }
     }
     __syncthreads();
@@ -834,7 +854,7 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       const int t = it / ncoef, kk = it % ncoef;
       double c0r = 0, c0i = 0, a0r = 0, a0i = 0, a1r = 0, a1i = 0, a2r = 0, a2i = 0, a3r = 0, a3i = 0;
       for (int ib = 0; ib < nb; ++ib) {
-        const double* o = ab + ((size_t)t * nb + ib) * 6;
+        const double* o = ab + (size_t)t * big + 6 * ib;
         const double* B = basis + ((size_t)ib * ncoef + kk) * 12;
         // chi += a . gradH
         c0r += o[0] * B[0] + o[1] * B[2] + o[2] * B[4];
@@ -858,11 +878,12 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       M[4] = cmk(a3r, a3i);
     }
     __syncthreads();
-    // recursion in degree (in place: chi -> W~, alpha -> Q~), then the moment vector
+    // seeds: W~_00 = solid angle, degree-1 Q = Lambda (x) grad H'_1^m with
+    // Lambda = -omega + sum (w1/R) dX
     for (int t = threadIdx.x; t < nt; t += blockDim.x) {
       cd* M = mom + (size_t)t * ncoef * 5;
       const double* X = tX + 6 * t;
-      const double om = tq[8 * t + 3] + tq[8 * t + 4] + tq[8 * t + 5];
+      const double om = tq[4 * t];
       double lam[3] = {0, 0, 0};
       for (int ib = 0; ib < nb; ++ib) {
         const double* Y = bx + 6 * ib;
@@ -870,11 +891,8 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
         const double wr = w1[(size_t)t * nb + ib] / sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
         for (int c = 0; c < 3; ++c) lam[c] += wr * Y[3 + c];
       }
-      const double xZ = X[0];
-      const cd w = cmk(X[1], X[2]), wc = cmk(X[1], -X[2]);
-      M[0] = cmk(om, 0.0);  // W~_00
+      M[0] = cmk(om, 0.0);
       for (int c = 1; c < 5; ++c) M[c] = cmk(0.0, 0.0);
-      // degree-1 Q: Lambda (x) grad H'_1^m, Lambda = -omega + lam
       for (int m = 0; m <= 1; ++m) {
         cd g[3];
         if (m == 0) { g[0] = cmk(1, 0); g[1] = cmk(0, 0); g[2] = cmk(0, 0); }
@@ -885,53 +903,67 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
         Q[2] = cadd(cscale(g[1], -om), csub(cscale(g[0], lam[2]), cscale(g[2], lam[0])));
         Q[3] = cadd(cscale(g[2], -om), csub(cscale(g[1], lam[0]), cscale(g[0], lam[1])));
       }
-      for (int n = 1; n <= p; ++n) {
-        const double dn = (double)n;
-        for (int m = 0; m <= n; ++m) {
-          // entries of degree n-1 at m-1, m, m+1 (negative m by conjugation)
-          auto get = [&](int mm, int c) -> cd {
-            const int n1 = n - 1;
-            if (mm > n1 || mm < -n1) return cmk(0, 0);
-            if (mm >= 0) return M[(size_t)hidx(n1, mm) * 5 + c];
-            cd v = cconj(M[(size_t)hidx(n1, -mm) * 5 + c]);
-            return ((-mm) & 1) ? cmk(-v.re, -v.im) : v;
-          };
-          cd* E = M + (size_t)hidx(n, m) * 5;
-          {
-            const cd lo = get(m - 1, 0), mid = get(m, 0), hi = get(m + 1, 0);
+    }
+    // recursion in degree, all (target, m) of one degree in parallel
+    for (int n = 1; n <= p; ++n) {
+      __syncthreads();
+      const double dn = (double)n;
+      for (int it = threadIdx.x; it < nt * (n + 1); it += blockDim.x) {
+        const int t = it / (n + 1), m = it % (n + 1);
+        cd* M = mom + (size_t)t * ncoef * 5;
+        const double* X = tX + 6 * t;
+        const double xZ = X[0];
+        const cd w = cmk(X[1], X[2]), wc = cmk(X[1], -X[2]);
+        // entries of degree n-1 at m-1, m, m+1 (negative m by conjugation)
+        auto get = [&](int mm, int c) -> cd {
+          const int n1 = n - 1;
+          if (mm > n1 || mm < -n1) return cmk(0, 0);
+          if (mm >= 0) return M[(size_t)hidx(n1, mm) * 5 + c];
+          cd v = cconj(M[(size_t)hidx(n1, -mm) * 5 + c]);
+          return ((-mm) & 1) ? cmk(-v.re, -v.im) : v;
+        };
+        cd* E = M + (size_t)hidx(n, m) * 5;
+        {
+          const cd lo = get(m - 1, 0), mid = get(m, 0), hi = get(m + 1, 0);
+          const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
+          E[0] = cadd(cadd(cscale(E[0], 1.0 / dn), cscale(mid, xZ)), t2);
+        }
+        if (n >= 2)
+          for (int c = 1; c < 5; ++c) {
+            const cd lo = get(m - 1, c), mid = get(m, c), hi = get(m + 1, c);
             const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
-            E[0] = cadd(cadd(cscale(E[0], 1.0 / dn), cscale(mid, xZ)), t2);
+            E[c] = cscale(cadd(E[c], cscale(cadd(cscale(mid, xZ), t2), dn)), 1.0 / (dn - 1.0));
           }
-          if (n >= 2)
-            for (int c = 1; c < 5; ++c) {
-              const cd lo = get(m - 1, c), mid = get(m, c), hi = get(m + 1, c);
-              const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
-              E[c] = cscale(cadd(E[c], cscale(cadd(cscale(mid, xZ), t2), dn)), 1.0 / (dn - 1.0));
-            }
-        }
       }
-      double* v = vv + (size_t)t * H5;
-      int kk = 0;
-      for (int n = 1; n <= p; ++n)
-        for (int m = 1; m <= n; ++m, ++kk) {
-          const cd* E = M + (size_t)hidx(n, m) * 5;
-          v[kk] = sigma * imrot(m, E[1]);
-          v[h + kk] = -sigma * imrot(m, E[2]);
-          v[2 * h + kk] = -sigma * imrot(m, E[3]);
-          v[3 * h + kk] = -sigma * imrot(m, E[4]);
-          v[4 * h + kk] = sigma * imrot(m, E[0]);
-        }
-      if (omega_out) {
-        const double f = 1.0 / (16.0 * atan(1.0)) / alpha;
-        for (int i = 0; i < H4; ++i) omega_out[(size_t)plist[t] * H4 + i] = v[i] * f;
-      }
-      if (ias_out) ias_out[plist[t]] = sigma * om;
     }
     __syncthreads();
+    // moment vector v = [rhs (4h); W (h)] into the (now free) a,b slots
+    for (int it = threadIdx.x; it < nt * h; it += blockDim.x) {
+      const int t = it / h, kk = it % h;
+      int n = 1;
+      while ((n + 1) * n / 2 <= kk) ++n;
+      const int m = kk - (n - 1) * n / 2 + 1;
+      const cd* E = mom + ((size_t)t * ncoef + hidx(n, m)) * 5;
+      double* v = ab + (size_t)t * big;
+      v[kk] = sigma * imrot(m, E[1]);
+      v[h + kk] = -sigma * imrot(m, E[2]);
+      v[2 * h + kk] = -sigma * imrot(m, E[3]);
+      v[3 * h + kk] = -sigma * imrot(m, E[4]);
+      v[4 * h + kk] = sigma * imrot(m, E[0]);
+    }
+    __syncthreads();
+    if (omega_out) {
+      const double f = 1.0 / (16.0 * atan(1.0)) / alpha;
+      for (int it = threadIdx.x; it < nt * H4; it += blockDim.x) {
+        const int t = it / H4, i = it % H4;
+        omega_out[(size_t)plist[t] * H4 + i] = ab[(size_t)t * big + i] * f;
+      }
+      for (int t = threadIdx.x; t < nt; t += blockDim.x) ias_out[plist[t]] = sigma * tq[4 * t];
+    }
     // weights: out(j) = sum_r G(j, r) v(r)
     for (int it = threadIdx.x; it < nt * 2 * h; it += blockDim.x) {
       const int t = it / (2 * h), j = it % (2 * h);
-      const double* v = vv + (size_t)t * H5;
+      const double* v = ab + (size_t)t * big;
       const double* g = G + (size_t)j * H5;
       double s = 0.0;
       for (int r = 0; r < H5; ++r) s += g[r] * v[r];
