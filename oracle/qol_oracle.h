@@ -8,6 +8,24 @@
  * x[3*j + c], A(m,n) is A[i + m*j]; index outputs (idxs, CSR) are 1-based.
  * Compiled with -O2 -ffp-contract=off and no fast-math (reference FP semantics,
  * Makefile:95-99), so every sum is a left-to-right IEEE sum.
+ *
+ * How each part is pinned (tests/test_oracle_*.py); the reference itself cannot
+ * be built (its QuatApproximation / LineQuaaadrature submodules are absent):
+ *   gauss, legendre_vals2coefs, bclag_weights   GL-8 of test/test_lap3d_close.f90:35-42,
+ *                                               numpy leggauss / legfit, barycentric exactness
+ *   vr_nodes                                    table of utils/get_vioreanu_nodes.m, exactness
+ *   koorn_pols, lu_inverse, simplex_precomp     orthonormality, vmatr*umatr = I, interpolation
+ *   patch_ref                                   level counts, exact composite rules
+ *   build_patch_levels, lap3dsdlpmat_levels     brute-force panel quadrature
+ *   lap3dsdlpmat                                closed form, coincident zero, Gauss's law
+ *   ssq_w1                                      adaptive quadrature; exact on straight edges
+ *   solid angle                                 Van Oosterom-Strackee on flat triangles
+ *   rrq                                         brute force on smooth densities (spectral in p)
+ *   nearlist                                    137,259 pairs on the order-4 fixture surface
+ *   close_apply, far_direct, getnearquad        fixture GRF, Gauss's law, COO consistency
+ * Parity partially pinned: the level label rule (which level serves a pair) is
+ * the restatement of patch_refine_mod.f90:516-615 with no reference vector of
+ * labels; the results each label produces are pinned by brute force.
  */
 #pragma once
 #include <cstdint>

@@ -125,7 +125,12 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
   if (p < 2 || p > QOL_MAXP) throw std::runtime_error("patch order out of range for the device kernels");
   if (npan <= 0) throw std::runtime_error("npan must be positive");
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  for (auto& st : tev)
+    for (auto& e : st) HIPCHK(hipEventCreate(&e));
+  for (hipEvent_t* e : {&ev_lists, &ev_direct, &ev_setup, &ev_edge})
+    HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   tab = std::make_unique<DeviceTables>(p, stream);
   h = tab->h;
   nq = p + 2;
@@ -150,7 +155,11 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
 }
 
 Engine::~Engine() {
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& st : tev)
+    for (auto& e : st) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ev_lists, ev_direct, ev_setup, ev_edge}) (void)hipEventDestroy(e);
+  if (s1) (void)hipStreamDestroy(s1);
+  if (s2) (void)hipStreamDestroy(s2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -176,8 +185,8 @@ void Engine::alloc_pairs() {
   d_lpairs.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
 }
 
-// classification, per-label pair lists and the direct levels
-void Engine::classify_and_direct(bool timed) {
+// classification and per-label pair lists (main stream)
+void Engine::classify_lists() {
   const size_t lds_b = sizeof(double) * 4 * (size_t)tab->nsubtot;
   hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, (const int*)d_offs.ptr,
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_balls.ptr,
@@ -187,10 +196,13 @@ void Engine::classify_and_direct(bool timed) {
   hipLaunchKernelGGL(compact_kernel, dim3(npan), dim3(256), 0, stream, (const int*)d_offs.ptr,
                      (const int8_t*)d_label.ptr, (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan,
                      (int*)d_lpairs.ptr);
-  if (timed) HIPCHK(hipEventRecord(ev[2], stream));
+}
+
+// direct S/D of every level-served pair
+void Engine::direct(hipStream_t s) {
   launch_direct_mfma(tab->LA, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr,
-                     nnz, stream);
+                     nnz, s);
 }
 
 i64 Engine::nearlist() {
@@ -252,11 +264,11 @@ void Engine::get_pairs(i64* offs, i64* idx1) {
 void Engine::build_levels() {
   d_lgeo.alloc(sizeof(double) * 7 * (size_t)tab->ntot * npan);
   d_balls.alloc(sizeof(double) * 4 * (size_t)tab->nsubtot * npan);
-  const int nmax = std::max(tab->LA.lev_n[2], tab->LA.lev_n[3]);
-  const size_t lds = sizeof(double) * (9 * (size_t)h + 4 * (size_t)nmax);
+  const size_t lds = level_geom_smem(h);
   set_lds((const void*)level_geom_kernel, lds);
-  hipLaunchKernelGGL(level_geom_kernel, dim3(npan), dim3(256), lds, stream, tab->LA, (const double*)d_sx.ptr,
-                     (const double*)d_rts.ptr, (const double*)d_rps.ptr, (double*)d_lgeo.ptr, (double*)d_balls.ptr);
+  hipLaunchKernelGGL(level_geom_kernel, dim3(level_geom_blocks(npan)), dim3(256), lds, stream, tab->LA, npan,
+                     (const double*)d_sx.ptr, (const double*)d_rts.ptr, (const double*)d_rps.ptr,
+                     (double*)d_lgeo.ptr, (double*)d_balls.ptr);
   HIPCHK(hipGetLastError());
   levels_built = true;
 }
@@ -273,90 +285,116 @@ void Engine::alloc_rrq() {
   d_Msl.alloc(sizeof(double) * (size_t)h * h * np_);
   d_Fn.alloc(sizeof(double) * (size_t)h * h * np_);
   d_G.alloc(sizeof(double) * 10 * (size_t)h * h * np_);
-  d_scr.alloc(sizeof(double) * 9 * (size_t)h * h * np_);
+  if (!rrq_solve_g1_in_lds(h)) d_scr.alloc(sizeof(double) * 4 * (size_t)h * h * np_);
   RP.frame = (double*)d_frame.ptr; RP.edge = (double*)d_edge.ptr; RP.bX = (double*)d_bX.ptr;
   RP.cx = (double*)d_cx.ptr; RP.basis = (double*)d_basis.ptr; RP.Fq = (double*)d_Fq.ptr;
   RP.Msl = (double*)d_Msl.ptr; RP.Fn = (double*)d_Fn.ptr;
-  RP.G = (double*)d_G.ptr; RP.scr = (double*)d_scr.ptr;
+  RP.G = (double*)d_G.ptr; RP.scr = rrq_solve_g1_in_lds(h) ? nullptr : (double*)d_scr.ptr;
   rrq_alloc = true;
 }
 
 void Engine::eval_levels_only() {
   if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
   build_levels();
-  if (nnz > 0) classify_and_direct(false);
+  if (nnz > 0) {
+    classify_lists();
+    direct(stream);
+  }
   HIPCHK(hipGetLastError());
   evaluated = true;
   have_omega = false;
 }
 
+// Stages and streams (default priority).  After the pair lists, three streams
+// run side by side: the direct levels (s1: matrix cores), the per-panel RRQ
+// set-up and solve (main), and the per-pair projection and edge work (s2, after
+// the set-up); the moment/weights kernel joins them.  tev[i] brackets stage i:
+// levels, classify+lists, direct, rrq setup, rrq solve, projection, edges, moments.
 void Engine::eval(int exterior, double side_tol, bool want_omega) {
   if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
   if (p > 12) throw std::runtime_error("the RRQ kernels support patch orders up to 12");
-  HIPCHK(hipEventRecord(ev[0], stream));
+  auto rec = [&](int i, int j, hipStream_t s) { HIPCHK(hipEventRecord(tev[i][j], s)); };
+  rec(0, 0, stream);
   build_levels();
-  HIPCHK(hipEventRecord(ev[1], stream));
-  if (nnz > 0) {
-    classify_and_direct(true);
-  } else {
+  rec(0, 1, stream);
+  if (nnz <= 0) {
     HIPCHK(hipMemsetAsync(d_nrrq.ptr, 0, sizeof(int) * npan, stream));
     HIPCHK(hipMemsetAsync(d_lcount.ptr, 0, sizeof(int) * 5 * npan, stream));
-    HIPCHK(hipEventRecord(ev[2], stream));
+    for (int i = 1; i < 8; ++i) { rec(i, 0, stream); rec(i, 1, stream); }
+    HIPCHK(hipGetLastError());
+    evaluated = true;
+    have_omega = want_omega;
+    return;
   }
-  HIPCHK(hipEventRecord(ev[3], stream));
-  if (nnz > 0) {
-    alloc_rrq();
-    const size_t lds1 = rrq_setup_smem(h, nq, ncoef);
-    set_lds((const void*)rrq_panel_setup_kernel, lds1);
-    hipLaunchKernelGGL(rrq_panel_setup_kernel, dim3(npan), dim3(256), lds1, stream, tab->RT, RP, npan,
-                       (const int*)d_nrrq.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
-                       (const double*)d_sw.ptr, ichart, (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
-    HIPCHK(hipEventRecord(ev[4], stream));
-    launch_rrq_panel_solve(tab->RT, RP, npan, (const int*)d_nrrq.ptr, stream);
-    HIPCHK(hipEventRecord(ev[5], stream));
-    if (want_omega) {
-      d_omega.alloc(sizeof(double) * 4 * (size_t)h * nnz);
-      d_ias.alloc(sizeof(double) * (size_t)nnz);
-      HIPCHK(hipMemsetAsync(d_omega.ptr, 0, sizeof(double) * 4 * (size_t)h * nnz, stream));
-      HIPCHK(hipMemsetAsync(d_ias.ptr, 0, sizeof(double) * (size_t)nnz, stream));
-    }
-    const int ng = blocks((size_t)nnz, 256);
-    d_pq.alloc(sizeof(double) * 4 * (size_t)nnz);
-    d_w1.alloc(sizeof(double) * 3 * (size_t)nq * nnz);
-    d_oe.alloc(sizeof(double) * 3 * (size_t)nnz);
-    hipLaunchKernelGGL(rrq_project_kernel, dim3(ng), dim3(256), 0, stream, tab->RT, RP, (const int*)d_lpairs.ptr,
-                       (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr, (const int*)d_tidx.ptr,
-                       (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol, (double*)d_pq.ptr);
-    HIPCHK(hipEventRecord(ev[6], stream));
-    hipLaunchKernelGGL(rrq_edge_kernel, dim3(blocks(3 * (size_t)nnz, 256)), dim3(256), 0, stream, tab->RT, RP,
-                       (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
-                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
-                       (double*)d_oe.ptr);
-    HIPCHK(hipEventRecord(ev[7], stream));
-    const size_t lds3 = rrq_target_smem(h, nq, ncoef);
-    set_lds((const void*)rrq_target_kernel, lds3);
-    hipLaunchKernelGGL(rrq_target_kernel, dim3(npan), dim3(256), lds3, stream, tab->RT, RP, (const int*)d_lpairs.ptr,
-                       (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan, (const int*)d_tidx.ptr,
-                       (const double*)d_tx.ptr, (const int*)d_nrrq.ptr, (const double*)d_w1.ptr,
-                       (const double*)d_oe.ptr, (double*)d_wout.ptr, want_omega ? (double*)d_omega.ptr : nullptr,
-                       want_omega ? (double*)d_ias.ptr : nullptr);
-  } else {
-    HIPCHK(hipEventRecord(ev[4], stream));
-    HIPCHK(hipEventRecord(ev[5], stream));
-    HIPCHK(hipEventRecord(ev[6], stream));
-    HIPCHK(hipEventRecord(ev[7], stream));
+  alloc_rrq();
+  if (want_omega) {
+    d_omega.alloc(sizeof(double) * 4 * (size_t)h * nnz);
+    d_ias.alloc(sizeof(double) * (size_t)nnz);
+    HIPCHK(hipMemsetAsync(d_omega.ptr, 0, sizeof(double) * 4 * (size_t)h * nnz, stream));
+    HIPCHK(hipMemsetAsync(d_ias.ptr, 0, sizeof(double) * (size_t)nnz, stream));
   }
-  HIPCHK(hipEventRecord(ev[8], stream));
+  d_pq.alloc(sizeof(double) * 4 * (size_t)nnz);
+  d_w1.alloc(sizeof(double) * 3 * (size_t)nq * nnz);
+  d_oe.alloc(sizeof(double) * 3 * (size_t)nnz);
+  rec(1, 0, stream);
+  classify_lists();
+  rec(1, 1, stream);
+  HIPCHK(hipEventRecord(ev_lists, stream));
+  // s1: direct levels
+  HIPCHK(hipStreamWaitEvent(s1, ev_lists, 0));
+  rec(2, 0, s1);
+  direct(s1);
+  rec(2, 1, s1);
+  HIPCHK(hipEventRecord(ev_direct, s1));
+  // main: RRQ set-up
+  rec(3, 0, stream);
+  const size_t lds1 = rrq_setup_smem(h, nq, ncoef);
+  set_lds((const void*)rrq_panel_setup_kernel, lds1);
+  hipLaunchKernelGGL(rrq_panel_setup_kernel, dim3(npan), dim3(256), lds1, stream, tab->RT, RP, npan,
+                     (const int*)d_nrrq.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
+                     (const double*)d_sw.ptr, ichart, (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
+  rec(3, 1, stream);
+  HIPCHK(hipEventRecord(ev_setup, stream));
+  // s2: projection and edges
+  HIPCHK(hipStreamWaitEvent(s2, ev_setup, 0));
+  rec(5, 0, s2);
+  hipLaunchKernelGGL(rrq_project_kernel, dim3(blocks((size_t)nnz, 256)), dim3(256), 0, s2, tab->RT, RP,
+                     (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                     (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol,
+                     (double*)d_pq.ptr);
+  rec(5, 1, s2);
+  rec(6, 0, s2);
+  launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                  (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
+                  (double*)d_oe.ptr, nnz, s2);
+  rec(6, 1, s2);
+  HIPCHK(hipEventRecord(ev_edge, s2));
+  // main: solve, then the moments/weights once the edges are in
+  rec(4, 0, stream);
+  launch_rrq_panel_solve(tab->RT, RP, npan, (const int*)d_nrrq.ptr, stream);
+  rec(4, 1, stream);
+  HIPCHK(hipStreamWaitEvent(stream, ev_edge, 0));
+  rec(7, 0, stream);
+  const size_t lds3 = rrq_target_smem(h, nq, ncoef);
+  set_lds((const void*)rrq_target_kernel, lds3);
+  hipLaunchKernelGGL(rrq_target_kernel, dim3(npan), dim3(256), lds3, stream, tab->RT, RP, (const int*)d_lpairs.ptr,
+                     (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan, (const int*)d_tidx.ptr,
+                     (const double*)d_tx.ptr, (const int*)d_nrrq.ptr, (const double*)d_w1.ptr,
+                     (const double*)d_oe.ptr, (double*)d_wout.ptr, want_omega ? (double*)d_omega.ptr : nullptr,
+                     want_omega ? (double*)d_ias.ptr : nullptr);
+  rec(7, 1, stream);
+  // join: everything after eval on the main stream sees every weight
+  HIPCHK(hipStreamWaitEvent(stream, ev_direct, 0));
   HIPCHK(hipGetLastError());
   evaluated = true;
   have_omega = want_omega;
 }
 
 void Engine::stage_ms(double* ms) {
-  HIPCHK(hipEventSynchronize(ev[8]));
+  HIPCHK(hipStreamSynchronize(stream));
   for (int i = 0; i < 8; ++i) {
     float t = 0.0f;
-    HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+    HIPCHK(hipEventElapsedTime(&t, tev[i][0], tev[i][1]));
     ms[i] = t;
   }
 }

@@ -11,81 +11,104 @@
 
 namespace qol {
 
-// One workgroup per panel.  Level nodes are interpolated in the reference's
-// summation order (sequential over the h panel nodes, no FMA) so that the ball
-// centres and radii -- and hence every classification -- match the host oracle
-// bit for bit.  Output: lgeo[panel][7][ntot] (x,y,z,nx,ny,nz,w), balls[panel][nsubtot][4].
-__global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, const double* __restrict__ sx,
+// kLP panels per workgroup, so each interpolation-matrix entry is loaded once
+// for all of them.  Level nodes are interpolated in the reference's summation
+// order (sequential over the h panel nodes, no FMA) so that the ball centres and
+// radii -- and hence every classification -- match the host oracle bit for bit.
+// Output: lgeo[panel][7][ntot] (x,y,z,nx,ny,nz,w), balls[panel][nsubtot][4].
+constexpr int kLP = 4;
+
+size_t level_geom_smem(int h) { return sizeof(double) * 9 * (size_t)h * kLP; }
+int level_geom_blocks(int npan) { return (npan + kLP - 1) / kLP; }
+
+__global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, const double* __restrict__ sx,
                                                          const double* __restrict__ rts,
                                                          const double* __restrict__ rps,
                                                          double* __restrict__ lgeo, double* __restrict__ balls) {
   extern __shared__ double smem[];
-  const int h = A.h, k = blockIdx.x;
-  double* srcT = smem;              // [9][h]
-  double* lx = srcT + 9 * h;        // [4][nmax] x,y,z,w of the current level
-  const int nmax = A.lev_n[3] > A.lev_n[2] ? A.lev_n[3] : A.lev_n[2];
-  for (int i = threadIdx.x; i < h; i += blockDim.x) {
-    const size_t g = (size_t)k * h + i;
-    for (int c = 0; c < 3; ++c) {
-      srcT[c * h + i] = sx[3 * g + c];
-      srcT[(3 + c) * h + i] = rts[3 * g + c];
-      srcT[(6 + c) * h + i] = rps[3 * g + c];
+  const int h = A.h, k0 = blockIdx.x * kLP;
+  const int np_ = min(kLP, npan - k0);
+  double* srcT = smem;  // [kLP][9][h]
+  for (int e = threadIdx.x; e < kLP * h; e += blockDim.x) {
+    const int pp = e / h, i = e % h;
+    double* S = srcT + (size_t)pp * 9 * h;
+    if (pp < np_) {
+      const size_t g = (size_t)(k0 + pp) * h + i;
+      for (int c = 0; c < 3; ++c) {
+        S[c * h + i] = sx[3 * g + c];
+        S[(3 + c) * h + i] = rts[3 * g + c];
+        S[(6 + c) * h + i] = rps[3 * g + c];
+      }
+    } else {
+      for (int c = 0; c < 9; ++c) S[c * h + i] = 0.0;
     }
   }
   __syncthreads();
-  double* G = lgeo + (size_t)k * 7 * A.ntot;
-  double* B = balls + (size_t)k * 4 * A.nsubtot;
   for (int l = 0; l < 4; ++l) {
     const int n = A.lev_n[l], off = A.lev_off[l];
     const double* IM = A.interp + (size_t)h * off;  // column-major n x h
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      double geo[9];
-      for (int c = 0; c < 9; ++c) geo[c] = 0.0;
+      double geo[kLP][9];
+#pragma unroll
+      for (int pp = 0; pp < kLP; ++pp)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) geo[pp][c] = 0.0;
       for (int q = 0; q < h; ++q) {
         const double v = IM[i + (size_t)n * q];
-        for (int c = 0; c < 9; ++c) geo[c] = geo[c] + v * srcT[c * h + q];
+#pragma unroll
+        for (int pp = 0; pp < kLP; ++pp)
+#pragma unroll
+          for (int c = 0; c < 9; ++c) geo[pp][c] = geo[pp][c] + v * srcT[((size_t)pp * 9 + c) * h + q];
       }
-      const double n0 = geo[7] * geo[5] - geo[8] * geo[4];
-      const double n1 = geo[8] * geo[3] - geo[6] * geo[5];
-      const double n2 = geo[6] * geo[4] - geo[7] * geo[3];
-      const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-      const double w = sp * A.wt[off + i];
-      G[0 * A.ntot + off + i] = geo[0];
-      G[1 * A.ntot + off + i] = geo[1];
-      G[2 * A.ntot + off + i] = geo[2];
-      G[3 * A.ntot + off + i] = n0 / sp;
-      G[4 * A.ntot + off + i] = n1 / sp;
-      G[5 * A.ntot + off + i] = n2 / sp;
-      G[6 * A.ntot + off + i] = w;
-      lx[i] = geo[0];
-      lx[nmax + i] = geo[1];
-      lx[2 * nmax + i] = geo[2];
-      lx[3 * nmax + i] = w;
+#pragma unroll
+      for (int pp = 0; pp < kLP; ++pp) {
+        if (pp >= np_) break;
+        const double* gp = geo[pp];
+        const double n0 = gp[7] * gp[5] - gp[8] * gp[4];
+        const double n1 = gp[8] * gp[3] - gp[6] * gp[5];
+        const double n2 = gp[6] * gp[4] - gp[7] * gp[3];
+        const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        double* G = lgeo + (size_t)(k0 + pp) * 7 * A.ntot + off + i;
+        G[0] = gp[0];
+        G[A.ntot] = gp[1];
+        G[2 * (size_t)A.ntot] = gp[2];
+        G[3 * (size_t)A.ntot] = n0 / sp;
+        G[4 * (size_t)A.ntot] = n1 / sp;
+        G[5 * (size_t)A.ntot] = n2 / sp;
+        G[6 * (size_t)A.ntot] = sp * A.wt[off + i];
+      }
     }
-    __syncthreads();
-    const int ns = A.lev_nsub[l], sb = A.sub_base[l];
-    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-      const int i0 = A.suboff[sb + l + s], i1 = A.suboff[sb + l + s + 1];
-      double swsum = 0.0;
-      for (int i = i0; i < i1; ++i) swsum += lx[3 * nmax + i];
-      double qp[3];
-      for (int c = 0; c < 3; ++c) {
-        double acc = 0.0;
-        for (int i = i0; i < i1; ++i) acc += lx[c * nmax + i] * lx[3 * nmax + i];
-        qp[c] = acc / swsum;
-      }
-      double rmax2 = 0.0;
-      for (int i = i0; i < i1; ++i) {
-        const double d0 = lx[i] - qp[0], d1 = lx[nmax + i] - qp[1], d2 = lx[2 * nmax + i] - qp[2];
-        rmax2 = fmax(rmax2, d0 * d0 + d1 * d1 + d2 * d2);
-      }
-      const double r2 = A.subtyp[sb + s] == 1 ? A.dist2 * fmax(rmax2, 0.5 * swsum) : A.dist2 * fmax(rmax2, swsum);
-      B[4 * (sb + s) + 0] = qp[0];
-      B[4 * (sb + s) + 1] = qp[1];
-      B[4 * (sb + s) + 2] = qp[2];
-      B[4 * (sb + s) + 3] = r2;
+  }
+  __syncthreads();
+  // sub-patch balls from the level nodes just written (this workgroup's panels)
+  const int ns_all = A.nsubtot;
+  for (int e = threadIdx.x; e < np_ * ns_all; e += blockDim.x) {
+    const int pp = e / ns_all, s_all = e % ns_all;
+    int l = 0;
+    while (l < 3 && s_all >= A.sub_base[l + 1]) ++l;
+    const int s = s_all - A.sub_base[l], off = A.lev_off[l];
+    const double* G = lgeo + (size_t)(k0 + pp) * 7 * A.ntot + off;
+    const size_t nt_ = A.ntot;
+    const int i0 = A.suboff[A.sub_base[l] + l + s], i1 = A.suboff[A.sub_base[l] + l + s + 1];
+    double swsum = 0.0;
+    for (int i = i0; i < i1; ++i) swsum += G[6 * nt_ + i];
+    double qp[3];
+    for (int c = 0; c < 3; ++c) {
+      double acc = 0.0;
+      for (int i = i0; i < i1; ++i) acc += G[c * nt_ + i] * G[6 * nt_ + i];
+      qp[c] = acc / swsum;
     }
-    __syncthreads();
+    double rmax2 = 0.0;
+    for (int i = i0; i < i1; ++i) {
+      const double d0 = G[i] - qp[0], d1 = G[nt_ + i] - qp[1], d2 = G[2 * nt_ + i] - qp[2];
+      rmax2 = fmax(rmax2, d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    const double r2 = A.subtyp[s_all] == 1 ? A.dist2 * fmax(rmax2, 0.5 * swsum) : A.dist2 * fmax(rmax2, swsum);
+    double* B = balls + (size_t)(k0 + pp) * 4 * A.nsubtot + 4 * s_all;
+    B[0] = qp[0];
+    B[1] = qp[1];
+    B[2] = qp[2];
+    B[3] = r2;
   }
 }
 

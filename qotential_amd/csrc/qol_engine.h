@@ -34,8 +34,9 @@ struct Engine {
   i64 nnz = -1;
   bool levels_built = false, evaluated = false, have_omega = false, rrq_alloc = false;
   i64 tlists_nnz = -1;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[9] = {};
+  hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr;
+  hipEvent_t tev[8][2] = {};
+  hipEvent_t ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_edge = nullptr;
   std::unique_ptr<DeviceTables> tab;
   DevBuf d_sx, d_snx, d_sw, d_rts, d_rps, d_sxbd, d_rv;
   DevBuf d_tx, d_nball, d_cnt, d_offs, d_tidx;
@@ -57,7 +58,8 @@ struct Engine {
   void alloc_rrq();
   void eval(int exterior, double side_tol, bool want_omega);
   void eval_levels_only();
-  void classify_and_direct(bool timed);
+  void classify_lists();
+  void direct(hipStream_t s);
   void stage_ms(double* ms);
   void get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias);
   void apply(const double* ub, const double* ubn, double* u, double* val, bool device_ptrs);

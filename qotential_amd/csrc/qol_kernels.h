@@ -41,11 +41,13 @@ struct RRQPanels {
   double* Msl;     // [npan][h][h] Msl^T (column-major)
   double* Fn;      // [npan][h][h] F_k at the nodes, F(i,k)
   double* G;       // [npan][2h][5h] weights operator
-  double* scr;     // [npan][9h^2] scratch for the weights operator
+  double* scr;     // [npan][4h^2] G1 when it does not fit in LDS, else null
 };
 
-__global__ void level_geom_kernel(LevelArgs A, const double* sx, const double* rts, const double* rps,
+__global__ void level_geom_kernel(LevelArgs A, int npan, const double* sx, const double* rts, const double* rps,
                                   double* lgeo, double* balls);
+size_t level_geom_smem(int h);
+int level_geom_blocks(int npan);
 __global__ void classify_kernel(LevelArgs A, const int* offs, const int* tidx, const double* tx,
                                 const double* balls, int8_t* label, int* ppanel, int* nrrq, int* lcount);
 
@@ -79,11 +81,13 @@ __global__ void rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan, const
                                        const double* snx, const double* sw, int ichart, const double* sxbd_chart,
                                        const double* rv_chart);
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s);
+bool rrq_solve_g1_in_lds(int h);
 __global__ void rrq_project_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
                                    const int* tidx, const double* tx, const double* sx, int exterior,
                                    double side_tol, double* pq);
-__global__ void rrq_edge_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
-                                const int* tidx, const double* tx, const double* pq, double* w1, double* oe);
+void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
+                     const int* tidx, const double* tx, const double* pq, double* w1, double* oe, int64_t nnz,
+                     hipStream_t s);
 __global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot,
                                   int npan, const int* tidx, const double* tx, const int* nrrq, const double* w1g,
                                   const double* oeg, double* wout, double* omega_out, double* ias_out);

@@ -19,21 +19,26 @@ namespace qol {
 // accumulated while the Koornwinder basis is generated (no h-sized arrays)
 __device__ void geom_eval(const double* cx, int h, int korder, double u, double v, double* x, double* xu, double* xv) {
   const int N = korder + 1;
-  double Q[QOL_MAXP + 1], Qu[QOL_MAXP + 1], Qv[QOL_MAXP + 1];
   const double a = 2.0 * u + v - 1.0, b = 1.0 - v, bb = b * b;
-  Q[0] = 1.0; Qu[0] = 0.0; Qv[0] = 0.0;
-  if (N > 1) { Q[1] = a; Qu[1] = 2.0; Qv[1] = 1.0; }
-  for (int m = 1; m + 1 < N; ++m) {
-    const double c1 = 2.0 * m + 1.0, c2 = (double)m, inv = 1.0 / (double)(m + 1);
-    Q[m + 1] = (c1 * a * Q[m] - c2 * bb * Q[m - 1]) * inv;
-    Qu[m + 1] = (c1 * (2.0 * Q[m] + a * Qu[m]) - c2 * bb * Qu[m - 1]) * inv;
-    Qv[m + 1] = (c1 * (Q[m] + a * Qv[m]) - c2 * (bb * Qv[m - 1] - 2.0 * b * Q[m - 1])) * inv;
-  }
   for (int c = 0; c < 3; ++c) { x[c] = 0.0; if (xu) { xu[c] = 0.0; xv[c] = 0.0; } }
   const double xx = 2.0 * v - 1.0;
-  // Jacobi P_bd^(2am+1,0)(xx) by its three-term recurrence in bd, one sweep per am;
-  // basis index of (degree d = am + bd, bd) is d(d+1)/2 + bd
+  // Q_am(a, b) (scaled Legendre in a/b) and its u, v derivatives advance with am;
+  // for each am, Jacobi P_bd^(2am+1,0)(xx) by its three-term recurrence in bd.
+  // Basis index of (degree d = am + bd, bd) is d(d+1)/2 + bd.
+  double Q = 1.0, Qu = 0.0, Qv = 0.0, Qm = 0.0, Qum = 0.0, Qvm = 0.0;
   for (int am = 0; am < N; ++am) {
+    if (am == 1) {
+      Qm = Q; Qum = Qu; Qvm = Qv;
+      Q = a; Qu = 2.0; Qv = 1.0;
+    } else if (am >= 2) {
+      const int m = am - 1;
+      const double c1 = 2.0 * m + 1.0, c2 = (double)m, inv = 1.0 / (double)(m + 1);
+      const double Qn = (c1 * a * Q - c2 * bb * Qm) * inv;
+      const double Qun = (c1 * (2.0 * Q + a * Qu) - c2 * bb * Qum) * inv;
+      const double Qvn = (c1 * (Q + a * Qv) - c2 * (bb * Qvm - 2.0 * b * Qm)) * inv;
+      Qm = Q; Qum = Qu; Qvm = Qv;
+      Q = Qn; Qu = Qun; Qv = Qvn;
+    }
     const double al = 2.0 * am + 1.0;
     double p = 1.0, dp = 0.0, pm = 1.0, dpm = 0.0;
     for (int bd = 0; am + bd < N; ++bd) {
@@ -52,10 +57,10 @@ __device__ void geom_eval(const double* cx, int h, int korder, double u, double 
       }
       const int d = am + bd, idx = d * (d + 1) / 2 + bd;
       const double sc = sqrt((2.0 * am + 1.0) * 2.0 * (double)(d + 1));
-      const double f = sc * Q[am] * p;
+      const double f = sc * Q * p;
       for (int c = 0; c < 3; ++c) x[c] += f * cx[c * h + idx];
       if (xu) {
-        const double fu = sc * Qu[am] * p, fv = sc * (Qv[am] * p + Q[am] * dp);
+        const double fu = sc * Qu * p, fv = sc * (Qv * p + Q * dp);
         for (int c = 0; c < 3; ++c) { xu[c] += fu * cx[c * h + idx]; xv[c] += fv * cx[c * h + idx]; }
       }
     }
@@ -433,7 +438,8 @@ __device__ void gauss_jordan(double (&A)[RPT][CPT][Q], int n, int m, double* sro
 #pragma unroll
       for (int b = 0; b < CPT; ++b) {
         const int c = tx + 16 * b;
-        if (c >= m) continue;
+        // left-block columns up to k are finished (never read again)
+        if (c >= m || (c <= k && c < n)) continue;
         if (r == k) {
           for (int q = 0; q < Q; ++q) A[a][b][q] = nrow[Q * c + q];
         } else {
@@ -452,8 +458,14 @@ __device__ void gauss_jordan(double (&A)[RPT][CPT][Q], int n, int m, double* sro
 //   X = Fq^{-1} (quaternion Gauss-Jordan on [Fq | I]); G1(i, (b,k)) = (X_ki)_b,
 //   so that Ad = G1 rhs / 4pi solves M^T w = rhs (first h components);
 //   B = (Msl^T)^{-1} (real Gauss-Jordan on [Msl^T | I]);
-//   As = B (W/4pi + Fn^T Ad) / alpha.
-// Rows j<h: As, rows h..2h-1: Ad.  Scratch per panel: G1 (h x 4h), T1 (h x 4h), B (h x h).
+//   As = B (W/4pi + Fn^T Ad) / alpha = (C rhs + B W) / (4pi alpha),  C = B Fn^T G1.
+// Rows j<h: As, rows h..2h-1: Ad.  B and B Fn^T (h x h) live in LDS, and G1
+// (h x 4h) too when it fits (P.scr, 4h^2 per panel, is used otherwise).
+bool rrq_solve_g1_in_lds(int h) { return (size_t)6 * h * h * sizeof(double) <= 120 * 1024; }
+size_t rrq_solve_smem(int h) {
+  return sizeof(double) * (size_t)((rrq_solve_g1_in_lds(h) ? 4 * h * h : 0) + 2 * h * h);
+}
+
 template <int RPT, int CPT>
 __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPanels P, int npan,
                                                               const int* __restrict__ nrrq) {
@@ -463,18 +475,20 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
   __shared__ double srow[4 * 16 * CPT], prow[4 * 16 * CPT], nrow[4 * 16 * CPT], colk[4 * 16 * RPT];
   __shared__ double rv[16];
   __shared__ int ri[16];
+  extern __shared__ double dyn[];
+  double* Bm = dyn;                      // [h][h] B(r,c) at r*h+c
+  double* Cm = Bm + (size_t)h * h;       // [h][h] (B Fn^T)(j,i) at j*h+i
+  double* G1 = P.scr ? P.scr + (size_t)k * 4 * h * h : Cm + (size_t)h * h;  // [h][4h]
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const double* FQ = P.Fq + (size_t)k * 3 * h * h;
   const double* MS = P.Msl + (size_t)k * h * h;
   const double* FN = P.Fn + (size_t)k * h * h;
   double* G = P.G + (size_t)k * 2 * h * H5;
-  double* G1 = P.scr + (size_t)k * 9 * h * h;  // [h][4h]
-  double* T1 = G1 + (size_t)4 * h * h;          // [h][4h]
-  double* Bm = T1 + (size_t)4 * h * h;          // [h][h] B(r,c) at r*h+c
   const double alpha = P.frame[(size_t)k * 16 + 12];
   const double fac = 1.0 / (16.0 * atan(1.0));
   {
     double A[RPT][CPT][4];
+#pragma unroll
     for (int a = 0; a < RPT; ++a)
 #pragma unroll
       for (int b = 0; b < CPT; ++b) {
@@ -489,6 +503,7 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
         }
       }
     gauss_jordan<4, RPT, CPT>(A, h, 2 * h, srow, prow, nrow, colk, rv, ri);
+#pragma unroll
     for (int a = 0; a < RPT; ++a)
 #pragma unroll
       for (int b = 0; b < CPT; ++b) {
@@ -501,6 +516,7 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
   }
   {
     double A[RPT][CPT][1];
+#pragma unroll
     for (int a = 0; a < RPT; ++a)
 #pragma unroll
       for (int b = 0; b < CPT; ++b) {
@@ -508,6 +524,7 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
         A[a][b][0] = (r < h && c < h) ? MS[r + (size_t)h * c] : ((r < h && c - h == r) ? 1.0 : 0.0);
       }
     gauss_jordan<1, RPT, CPT>(A, h, 2 * h, srow, prow, nrow, colk, rv, ri);
+#pragma unroll
     for (int a = 0; a < RPT; ++a)
 #pragma unroll
       for (int b = 0; b < CPT; ++b) {
@@ -515,55 +532,113 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
         if (r < h && c >= h && c < 2 * h) Bm[(size_t)r * h + (c - h)] = A[a][b][0];
       }
   }
-  __threadfence_block();
   __syncthreads();
-  for (int e = threadIdx.x; e < h * H4; e += blockDim.x) {
-    const int r = e / H4, c = e % H4;
-    double t = 0.0;
-    for (int i = 0; i < h; ++i) t += FN[i + (size_t)h * r] * G1[(size_t)i * H4 + c];
-    T1[(size_t)r * H4 + c] = t;
+  // C = B Fn^T
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+    const int j = e / h, i = e % h;
+    double s = 0.0;
+    for (int r = 0; r < h; ++r) s += Bm[(size_t)j * h + r] * FN[i + (size_t)h * r];
+    Cm[(size_t)j * h + i] = s;
   }
-  __threadfence_block();
   __syncthreads();
-  for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
-    const int r = e / H5, c = e % H5;
-    G[(size_t)(h + r) * H5 + c] = c < H4 ? G1[(size_t)r * H4 + c] * fac : 0.0;
-  }
+  const double fa = fac / alpha;
   for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
     const int j = e / H5, c = e % H5;
+    // Ad rows
+    G[(size_t)(h + j) * H5 + c] = c < H4 ? G1[(size_t)j * H4 + c] * fac : 0.0;
+    // As rows
     double s = 0.0;
     if (c < H4) {
-      for (int r = 0; r < h; ++r) s += Bm[(size_t)j * h + r] * T1[(size_t)r * H4 + c];
+      for (int i = 0; i < h; ++i) s += Cm[(size_t)j * h + i] * G1[(size_t)i * H4 + c];
     } else {
       s = Bm[(size_t)j * h + (c - H4)];
     }
-    G[(size_t)j * H5 + c] = s * fac / alpha;
+    G[(size_t)j * H5 + c] = s * fa;
   }
 }
 
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s) {
   const int h = T.h;
   const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
-  if (rpt <= 3 && cpt <= 5)
-    hipLaunchKernelGGL((rrq_panel_solve_kernel<3, 5>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
-  else if (rpt <= 4 && cpt <= 7)
-    hipLaunchKernelGGL((rrq_panel_solve_kernel<4, 7>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
-  else if (rpt <= 5 && cpt <= 10)
-    hipLaunchKernelGGL((rrq_panel_solve_kernel<5, 10>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
-  else
+  const size_t lds = rrq_solve_smem(h);
+  if (!rrq_solve_g1_in_lds(h) && !P.scr) throw std::runtime_error("RRQ solve: scratch for G1 not allocated");
+  if (rpt <= 3 && cpt <= 5) {
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)rrq_panel_solve_kernel<3, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((rrq_panel_solve_kernel<3, 5>), dim3(npan), dim3(256), lds, s, T, P, npan, nrrq);
+  } else if (rpt <= 4 && cpt <= 7) {
+    (void)hipFuncSetAttribute((const void*)rrq_panel_solve_kernel<4, 7>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((rrq_panel_solve_kernel<4, 7>), dim3(npan), dim3(256), lds, s, T, P, npan, nrrq);
+  } else if (rpt <= 5 && cpt <= 10) {
+    (void)hipFuncSetAttribute((const void*)rrq_panel_solve_kernel<5, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((rrq_panel_solve_kernel<5, 10>), dim3(npan), dim3(256), lds, s, T, P, npan, nrrq);
+  } else {
     throw std::runtime_error("RRQ solve supports patch orders up to 12");
+  }
 }
 
 // ---------------------------------------------------------------------------
 // per-target pieces
 
-// complex root of |X_e(s) - x|^2 near [-1,1] by Newton from the chord guess
-__device__ bool edge_root(const double* cf, int nq, const double* x, cd& s0) {
-  double A[3], B[3], D[3], C[3];
+// Edge helpers, specialised on the number of edge nodes NQ so that every
+// nq-sized array is a register array and the Legendre recurrence constants
+// (2k+1)/(k+1), k/(k+1) fold at compile time.  cf = 3 x NQ Legendre
+// coefficients of one edge, cf[c*NQ + k].
+
+// X(s), X'(s) of all three coordinates at real s
+template <int NQ>
+__device__ __forceinline__ void leg3_r(const double* cf, double s, double* val, double* der) {
+  double p0 = 1.0, p1 = s, d0 = 0.0, d1 = 1.0;
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
-    double v, d;
-    leg_eval_r(cf + c * nq, nq, -1.0, v, d); A[c] = v;
-    leg_eval_r(cf + c * nq, nq, 1.0, v, d); B[c] = v;
+    val[c] = cf[c * NQ] + cf[c * NQ + 1] * s;
+    der[c] = cf[c * NQ + 1];
+  }
+#pragma unroll
+  for (int k = 1; k + 1 < NQ; ++k) {
+    const double c1 = (2.0 * k + 1.0) / (k + 1.0), c2 = (double)k / (k + 1.0);
+    const double p2 = c1 * s * p1 - c2 * p0;
+    const double d2 = c1 * (p1 + s * d1) - c2 * d0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      val[c] += cf[c * NQ + k + 1] * p2;
+      der[c] += cf[c * NQ + k + 1] * d2;
+    }
+    p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+  }
+}
+
+// the same at complex s
+template <int NQ>
+__device__ __forceinline__ void leg3_c(const double* cf, cd s, cd* val, cd* der) {
+  cd p0 = cmk(1.0, 0.0), p1 = s, d0 = cmk(0.0, 0.0), d1 = cmk(1.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    val[c] = cadd(cmk(cf[c * NQ], 0.0), cscale(s, cf[c * NQ + 1]));
+    der[c] = cmk(cf[c * NQ + 1], 0.0);
+  }
+#pragma unroll
+  for (int k = 1; k + 1 < NQ; ++k) {
+    const double c1 = (2.0 * k + 1.0) / (k + 1.0), c2 = (double)k / (k + 1.0);
+    const cd p2 = csub(cscale(cmul(s, p1), c1), cscale(p0, c2));
+    const cd d2 = csub(cscale(cadd(p1, cmul(s, d1)), c1), cscale(d0, c2));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      val[c] = cadd(val[c], cscale(p2, cf[c * NQ + k + 1]));
+      der[c] = cadd(der[c], cscale(d2, cf[c * NQ + k + 1]));
+    }
+    p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+  }
+}
+
+// complex root of |X_e(s) - x|^2 near [-1,1] by Newton from the chord guess
+template <int NQ>
+__device__ bool edge_root(const double* cf, const double* x, cd& s0) {
+  double A[3], B[3], dA[3], dB[3], D[3], C[3];
+  leg3_r<NQ>(cf, -1.0, A, dA);
+  leg3_r<NQ>(cf, 1.0, B, dB);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
     D[c] = 0.5 * (B[c] - A[c]);
     C[c] = 0.5 * (A[c] + B[c]) - x[c];
   }
@@ -572,13 +647,14 @@ __device__ bool edge_root(const double* cf, int nq, const double* x, cd& s0) {
   const double cc = C[0] * C[0] + C[1] * C[1] + C[2] * C[2];
   cd s = cmk(-cdv / dd, sqrt(fmax(0.0, cc * dd - cdv * cdv)) / dd);
   for (int it = 0; it < 30; ++it) {
+    cd v[3], d[3];
+    leg3_c<NQ>(cf, s, v, d);
     cd f = cmk(0, 0), fp = cmk(0, 0);
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-      cd v, d;
-      leg_eval_c(cf + c * nq, nq, s, v, d);
-      v.re -= x[c];
-      f = cadd(f, cmul(v, v));
-      fp = cadd(fp, cscale(cmul(v, d), 2.0));
+      v[c].re -= x[c];
+      f = cadd(f, cmul(v[c], v[c]));
+      fp = cadd(fp, cscale(cmul(v[c], d[c]), 2.0));
     }
     const cd ds = cdiv(f, fp);
     s = csub(s, ds);
@@ -589,51 +665,63 @@ __device__ bool edge_root(const double* cf, int nq, const double* x, cd& s0) {
 }
 
 // singularity-swap weights for the 1/|X(s)-x| kernel on one edge; w applies to f_i/R_i
-__device__ void ssq_w1(const double* cf, int nq, const double* tgl, const double* wgl, const double* x,
-                       double rho_thr, double* w, cd* root, bool* rok) {
+template <int NQ>
+__device__ void ssq_w1(const double* cf, const double* tgl, const double* wgl, const double* x, double rho_thr,
+                       double* w, cd* root, bool* rok) {
   cd s0;
-  const bool ok = edge_root(cf, nq, x, s0);
+  const bool ok = edge_root<NQ>(cf, x, s0);
   *root = s0;
   *rok = ok;
   if (!ok || bernstein_rho(s0) >= rho_thr) {
-    for (int i = 0; i < nq; ++i) w[i] = wgl[i];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) w[i] = wgl[i];
     return;
   }
   const double a = s0.re, b = fabs(s0.im), sg = 1.0 + fabs(a);
   const double x1 = 1.0 - a, x2 = -1.0 - a;
   const double r1 = sqrt(x1 * x1 + b * b), r2 = sqrt(x2 * x2 + b * b);
-  double I[QOL_MAXNQ];
+  double I[NQ];
   if (x2 >= 0.0) I[0] = log((x1 + r1) / (x2 + r2));
   else if (x1 <= 0.0) I[0] = log((r2 - x2) / (r1 - x1));
   else I[0] = log((x1 + r1) * (r2 - x2) / (b * b));
-  if (nq > 1) I[1] = r1 - r2;
+  if (NQ > 1) I[1] = r1 - r2;
   double p1 = x1, p2 = x2;
-  for (int kk = 2; kk < nq; ++kk) {
+#pragma unroll
+  for (int kk = 2; kk < NQ; ++kk) {
     const double dk = (double)kk;
     I[kk] = (p1 * r1 - p2 * r2 - (dk - 1.0) * b * b * I[kk - 2]) / dk;
     p1 *= x1;
     p2 *= x2;
   }
   double sc = 1.0;
-  for (int kk = 0; kk < nq; ++kk) { I[kk] *= sc; sc /= sg; }
+#pragma unroll
+  for (int kk = 0; kk < NQ; ++kk) { I[kk] *= sc; sc /= sg; }
   // dual Vandermonde system sum_i lam_i t_i^k = I_k (Bjorck-Pereyra), t_i = (s_i - a)/sg
-  double t[QOL_MAXNQ];
-  for (int i = 0; i < nq; ++i) t[i] = (tgl[i] - a) / sg;
-  const int N = nq - 1;
+  double t[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) t[i] = (tgl[i] - a) / sg;
+  constexpr int N = NQ - 1;
+#pragma unroll
   for (int kk = 0; kk < N; ++kk)
+#pragma unroll
     for (int i = N; i > kk; --i) I[i] -= t[kk] * I[i - 1];
+#pragma unroll
   for (int kk = N - 1; kk >= 0; --kk) {
+#pragma unroll
     for (int i = kk + 1; i <= N; ++i) I[i] /= (t[i] - t[i - kk - 1]);
+#pragma unroll
     for (int i = kk; i < N; ++i) I[i] -= I[i + 1];
   }
-  for (int i = 0; i < nq; ++i) {
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
     const double ds = tgl[i] - a;
     w[i] = I[i] * sqrt(ds * ds + b * b);
   }
 }
 
 // Dirac-string line integral of one edge, adaptive march toward the near point
-__device__ double solid_angle_edge(const double* cf, int nq, const double* tgl, const double* wgl, const double* x,
+template <int NQ>
+__device__ double solid_angle_edge(const double* cf, const double* tgl, const double* wgl, const double* x,
                                    const double* q, cd s0, bool ok, double rho_thr) {
   double total = 0.0, lo = -1.0, len = 2.0;
   int guard = 0;
@@ -649,10 +737,11 @@ __device__ double solid_angle_edge(const double* cf, int nq, const double* tgl, 
       }
     }
     const double c = 0.5 * (lo + hi), hh = 0.5 * (hi - lo);
-    for (int i = 0; i < nq; ++i) {
+#pragma unroll 2
+    for (int i = 0; i < NQ; ++i) {
       const double s = c + hh * tgl[i];
       double X[3], dX[3];
-      for (int cc = 0; cc < 3; ++cc) leg_eval_r(cf + cc * nq, nq, s, X[cc], dX[cc]);
+      leg3_r<NQ>(cf, s, X, dX);
       const double z[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
       double zx[3];
       cross3(z, dX, zx);
@@ -748,6 +837,7 @@ __global__ void __launch_bounds__(256) rrq_project_kernel(RRQTables T, RRQPanels
   o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = side;
 }
 
+template <int NQ>
 __global__ void __launch_bounds__(256) rrq_edge_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
                                                        const int* __restrict__ ltot, const int* __restrict__ ppanel,
                                                        const int* __restrict__ tidx, const double* __restrict__ tx,
@@ -755,7 +845,7 @@ __global__ void __launch_bounds__(256) rrq_edge_kernel(RRQTables T, RRQPanels P,
                                                        double* __restrict__ oe) {
   const int it = blockIdx.x * blockDim.x + threadIdx.x;
   if (it >= 3 * ltot[4]) return;
-  const int r = it / 3, e = it % 3, nq = T.nq;
+  const int r = it / 3, e = it % 3;
   const int pair = lpairs[rrq_list_base(ltot) + r];
   const int k = ppanel[pair];
   const double* fr = P.frame + (size_t)k * 16;
@@ -763,13 +853,27 @@ __global__ void __launch_bounds__(256) rrq_edge_kernel(RRQTables T, RRQPanels P,
   const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
   double X[3];
   to_frame(fr, x, X);
-  const double* cf = P.edge + (size_t)k * 9 * nq + e * 3 * nq;
-  const double rho_thr = pow(8.0, 8.0 / (double)nq);
+  const double* cf = P.edge + (size_t)k * 9 * NQ + e * 3 * NQ;
+  const double rho_thr = pow(8.0, 8.0 / (double)NQ);
   cd root;
   bool ok;
-  ssq_w1(cf, nq, T.tgl, T.wgl, X, rho_thr, w1 + (size_t)r * 3 * nq + e * nq, &root, &ok);
+  ssq_w1<NQ>(cf, T.tgl, T.wgl, X, rho_thr, w1 + (size_t)r * 3 * NQ + e * NQ, &root, &ok);
   const double q[3] = {pq[4 * (size_t)r], pq[4 * (size_t)r + 1], pq[4 * (size_t)r + 2]};
-  oe[3 * (size_t)r + e] = solid_angle_edge(cf, nq, T.tgl, T.wgl, X, q, root, ok, rho_thr);
+  oe[3 * (size_t)r + e] = solid_angle_edge<NQ>(cf, T.tgl, T.wgl, X, q, root, ok, rho_thr);
+}
+
+void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
+                     const int* tidx, const double* tx, const double* pq, double* w1, double* oe, int64_t nnz,
+                     hipStream_t s) {
+  const dim3 grid((unsigned)((3 * nnz + 255) / 256)), blk(256);
+#define QOL_EDGE(NQ_) \
+  case NQ_: hipLaunchKernelGGL(rrq_edge_kernel<NQ_>, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, pq, w1, oe); break;
+  switch (T.nq) {
+    QOL_EDGE(4) QOL_EDGE(5) QOL_EDGE(6) QOL_EDGE(7) QOL_EDGE(8) QOL_EDGE(9) QOL_EDGE(10) QOL_EDGE(11)
+    QOL_EDGE(12) QOL_EDGE(13) QOL_EDGE(14)
+    default: throw std::runtime_error("RRQ edge kernel: patch orders 2..12 only");
+  }
+#undef QOL_EDGE
 }
 
 // LDS plan of rrq_target_kernel: fixed panel data plus per-target slots

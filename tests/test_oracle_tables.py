@@ -66,3 +66,21 @@ def test_product_tables_bitwise_equal_oracle(oracle):
             assert np.array_equal(uv, L["uv"]) and np.array_equal(wt, L["wt"])
             assert np.array_equal(so, L["suboff"]) and np.array_equal(st, L["subtyp"])
             assert np.array_equal(im, L["interpmat"]), (p, lev, np.abs(im - L["interpmat"]).max())
+
+
+def test_level_rules_exact_on_monomials(oracle):
+    """Every refinement level is a composite rule over sub-patches that tile the
+    reference triangle (VR rules of order orderff-1 on sub-triangles, Gauss
+    squares on merged pairs), so each integrates u^a v^b exactly up to degree
+    p+1.  A missing, doubled or misplaced sub-patch breaks this."""
+    from math import factorial
+    for p in (4, 8, 12):
+        for lev in (1, 2, 3, 4):
+            L = oracle.patch_ref_level(p, lev)
+            u, v, w = L["uv"][:, 0], L["uv"][:, 1], L["wt"]
+            for a in range(p + 2):
+                for b in range(p + 2 - a):
+                    ex = factorial(a) * factorial(b) / factorial(a + b + 2)
+                    assert abs((w * u ** a * v ** b).sum() - ex) < 1e-14, (p, lev, a, b)
+            # nodes inside the triangle
+            assert (u >= -1e-15).all() and (v >= -1e-15).all() and (u + v <= 1 + 1e-15).all()
