@@ -38,22 +38,33 @@ def parse():
     ap.add_argument("--targets", type=int, default=200000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-pairs", type=int, default=600000)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the timed steps, one device-to-device copy of this many MiB (a plain copy "
+                         "kernel of known size that scales the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
-def algorithmic_flops(p, counts, nrrq):
-    """FP64 FLOPs the math needs per stage (counted from the formulas, not from
-    executed instructions).  counts[l] = pairs served by level l (l = 0..3)."""
+def kernel_flops(p, npan, counts, nrrq, nrrq_panels):
+    """Algorithmic FP64 FLOPs per launch of the kernels that have a fixed count
+    (from the formulas, not from executed instructions).  counts[l] = pairs
+    served by level l (l = 0..3), nrrq = RRQ pairs, nrrq_panels = panels with
+    at least one RRQ pair."""
     from qotential_amd.engine import order_info
     info = order_info(p)
     h, nq, ncoef = info["h"], info["nq"], info["ncoef"]
     nb = 3 * nq
-    # direct level L: per node 19 (kernel incl. 1 rsqrt) + 4h (two contractions)
-    direct = sum(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h) for l in range(4))
-    # RRQ per pair: boundary moments 44 per (node, coefficient), recursion
-    # 5 complex updates per coefficient (~60), weights operator 2 (2h)(5h)
-    rrq = nrrq * (44 * nb * ncoef + 60 * ncoef + 20 * h * h)
-    return direct, rrq
+    ntot = int(sum(info["nl"]))
+    return {
+        # 9 products + 9 sums per (level node, panel node), sequential (no FMA)
+        "levels": ("level_geom_kernel", 18.0 * h * ntot * npan),
+        # per level node: 19 for the S/D entry (incl. one rsqrt), 4h for the two contractions
+        "direct": ("direct_mfma_kernel", float(sum(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h)
+                                                   for l in range(4)))),
+        # quaternion Gauss-Jordan 32h^3, real Gauss-Jordan 2h^3, B Fn^T 2h^3, C G1 8h^3
+        "rrq_solve": ("rrq_panel_solve_kernel", 44.0 * h ** 3 * nrrq_panels),
+        # boundary moments 48 nb ncoef, degree recursion ~200 ncoef, weights GEMV 20 h^2
+        "rrq_targets": ("rrq_target_kernel", float(nrrq) * (48 * nb * ncoef + 200 * ncoef + 20 * h * h)),
+    }
 
 
 def cpu_baseline(geo, tx, offs, idx, ub, ubn, sample_pairs):
@@ -141,6 +152,16 @@ def main():
     value = nnz * a.steps / elapsed
     print(f"[bench] {a.steps} steps: {ms_step:.2f} ms/step, {value:.4g} pairs/s", flush=True)
 
+    if a.calib_copy_mib > 0:
+        # plain copy of known size for the HBM-traffic counter calibration (not timed)
+        import torch
+        n = a.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {a.calib_copy_mib} MiB read, {a.calib_copy_mib} MiB written", flush=True)
+
     # GRF identity on every target
     u, _ = eng.apply(ub, ubn, far)
     uex, _ = W.charge_field(tx, pos, q)
@@ -149,21 +170,26 @@ def main():
     repeat = float(np.abs(u - u0).max() / np.abs(u0).max())
     print(f"[bench] GRF rel err {grf:.3e}; repeat-run rel diff {repeat:.1e}", flush=True)
 
-    # roofline of the dominant stage (FP64 compute bound)
-    direct_f, rrq_f = algorithmic_flops(a.order, labels[:4], int(labels[4]))
+    # roofline of the dominant kernel among those with an algorithmic count
+    # (FP64 compute bound: MI355X FP64 matrix and vector peaks are both 78.6 TFLOP/s)
+    offs, idx = eng.pairs()
+    lab = w["label"]
+    csum = np.concatenate([[0], np.cumsum(lab == 5)])
+    rrq_per_panel = csum[offs[1:]] - csum[offs[:-1]]
+    kf = kernel_flops(a.order, geo["ntri"], labels[:4], int(labels[4]), int((rrq_per_panel > 0).sum()))
     per = {k: v / a.steps for k, v in stage_tot.items()}
-    cand = {"direct_mfma_kernel": (direct_f, per["direct"]),
-            "rrq_project+edge+target": (rrq_f, per["rrq_project"] + per["rrq_edge"] + per["rrq_targets"])}
-    dom = max(cand, key=lambda k: cand[k][1])
-    fl, ms = cand[dom]
-    achieved = fl / (ms * 1e-3) / 1e12
-    roof = {"kernel": dom, "bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    kernels = {}
+    for stage, (kname, fl) in kf.items():
+        ms = per[stage]
+        kernels[kname] = {"ms": ms, "gflop": fl / 1e9, "tflops": fl / (ms * 1e-3) / 1e12}
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    achieved = kernels[dom]["tflops"]
+    roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-            "flops_per_launch": fl, "ms_per_launch": ms}
+            "flops_per_launch": kernels[dom]["gflop"] * 1e9, "ms_per_launch": kernels[dom]["ms"]}
 
     base = None
     if not a.no_cpu_baseline:
-        offs, idx = eng.pairs()
         base = cpu_baseline(geo, tx, offs, idx, ub, ubn, a.cpu_sample_pairs)
         print(f"[bench] cpu baseline {base['value']:.4g} pairs/s ({base['sample']})", flush=True)
 
@@ -175,7 +201,7 @@ def main():
         "config": {"workload": f"plain FP64 RRQ, stellarator {a.mp}x{a.np} charts, order {a.order}, "
                                f"{geo['ntri']} triangles, {a.targets} near-surface targets both sides "
                                f"U[-6,-1], reference ball near list",
-                   "pairs": int(nnz), "labels_L1_L4_RRQ": labels.tolist(), "stage_ms": per,
+                   "pairs": int(nnz), "labels_L1_L4_RRQ": labels.tolist(), "stage_ms": per, "kernels": kernels,
                    "grf_rel_err": grf, "rel_err_vs_plain_fp64": 0.0, "repeat_rel_diff": repeat},
         "roofline": roof,
         "cpu_baseline": base,

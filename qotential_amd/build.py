@@ -24,6 +24,8 @@ FLAGS = [
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # MFMA accumulators in VGPRs (gfx950 allows it): no AGPR copies in the K loops
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]

@@ -127,9 +127,10 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
   for (auto& st : tev)
     for (auto& e : st) HIPCHK(hipEventCreate(&e));
-  for (hipEvent_t* e : {&ev_lists, &ev_direct, &ev_setup, &ev_edge})
+  for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_solve, &ev_edge})
     HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   tab = std::make_unique<DeviceTables>(p, stream);
   h = tab->h;
@@ -157,9 +158,10 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
 Engine::~Engine() {
   for (auto& st : tev)
     for (auto& e : st) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {ev_lists, ev_direct, ev_setup, ev_edge}) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ev_start, ev_lists, ev_direct, ev_setup, ev_solve, ev_edge}) (void)hipEventDestroy(e);
   if (s1) (void)hipStreamDestroy(s1);
   if (s2) (void)hipStreamDestroy(s2);
+  if (s3) (void)hipStreamDestroy(s3);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -305,22 +307,25 @@ void Engine::eval_levels_only() {
   have_omega = false;
 }
 
-// Stages and streams (default priority).  After the pair lists, three streams
-// run side by side: the direct levels (s1: matrix cores), the per-panel RRQ
-// set-up and solve (main), and the per-pair projection and edge work (s2, after
-// the set-up); the moment/weights kernel joins them.  tev[i] brackets stage i:
-// levels, classify+lists, direct, rrq setup, rrq solve, projection, edges, moments.
+// Stages and streams (all at the default priority).  The per-panel RRQ set-up
+// and solve need only the surface, so they run for every panel on s2 from the
+// start, beside the level geometry and classification on the main stream.
+// After the pair lists: the direct levels on s1 (matrix cores) and the per-pair
+// projection, root/SSQ and solid angle on s3 (after the set-up); the
+// moment/weights kernel joins s2 and s3, and the main stream joins s1 last.
+// tev[i] brackets stage i: levels, classify+lists, direct, rrq setup, rrq solve,
+// projection, root + SSQ (solid angle from tev[8][0]), moments.
 void Engine::eval(int exterior, double side_tol, bool want_omega) {
   if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
   if (p > 12) throw std::runtime_error("the RRQ kernels support patch orders up to 12");
   auto rec = [&](int i, int j, hipStream_t s) { HIPCHK(hipEventRecord(tev[i][j], s)); };
-  rec(0, 0, stream);
-  build_levels();
-  rec(0, 1, stream);
   if (nnz <= 0) {
+    rec(0, 0, stream);
+    build_levels();
+    rec(0, 1, stream);
     HIPCHK(hipMemsetAsync(d_nrrq.ptr, 0, sizeof(int) * npan, stream));
     HIPCHK(hipMemsetAsync(d_lcount.ptr, 0, sizeof(int) * 5 * npan, stream));
-    for (int i = 1; i < 8; ++i) { rec(i, 0, stream); rec(i, 1, stream); }
+    for (int i = 1; i < 9; ++i) { rec(i, 0, stream); rec(i, 1, stream); }
     HIPCHK(hipGetLastError());
     evaluated = true;
     have_omega = want_omega;
@@ -336,6 +341,26 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   d_pq.alloc(sizeof(double) * 4 * (size_t)nnz);
   d_w1.alloc(sizeof(double) * 3 * (size_t)nq * nnz);
   d_oe.alloc(sizeof(double) * 3 * (size_t)nnz);
+  d_root.alloc(sizeof(double) * 9 * (size_t)nnz);
+  HIPCHK(hipEventRecord(ev_start, stream));
+  // s2: RRQ set-up and solve for every panel
+  HIPCHK(hipStreamWaitEvent(s2, ev_start, 0));
+  rec(3, 0, s2);
+  const size_t lds1 = rrq_setup_smem(h, nq, ncoef);
+  set_lds((const void*)rrq_panel_setup_kernel, lds1);
+  hipLaunchKernelGGL(rrq_panel_setup_kernel, dim3(npan), dim3(256), lds1, s2, tab->RT, RP, npan, (const int*)nullptr,
+                     (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr, ichart,
+                     (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
+  rec(3, 1, s2);
+  HIPCHK(hipEventRecord(ev_setup, s2));
+  rec(4, 0, s2);
+  launch_rrq_panel_solve(tab->RT, RP, npan, nullptr, s2);
+  rec(4, 1, s2);
+  HIPCHK(hipEventRecord(ev_solve, s2));
+  // main: levels, classification, pair lists
+  rec(0, 0, stream);
+  build_levels();
+  rec(0, 1, stream);
   rec(1, 0, stream);
   classify_lists();
   rec(1, 1, stream);
@@ -346,33 +371,23 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   direct(s1);
   rec(2, 1, s1);
   HIPCHK(hipEventRecord(ev_direct, s1));
-  // main: RRQ set-up
-  rec(3, 0, stream);
-  const size_t lds1 = rrq_setup_smem(h, nq, ncoef);
-  set_lds((const void*)rrq_panel_setup_kernel, lds1);
-  hipLaunchKernelGGL(rrq_panel_setup_kernel, dim3(npan), dim3(256), lds1, stream, tab->RT, RP, npan,
-                     (const int*)d_nrrq.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
-                     (const double*)d_sw.ptr, ichart, (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
-  rec(3, 1, stream);
-  HIPCHK(hipEventRecord(ev_setup, stream));
-  // s2: projection and edges
-  HIPCHK(hipStreamWaitEvent(s2, ev_setup, 0));
-  rec(5, 0, s2);
-  hipLaunchKernelGGL(rrq_project_kernel, dim3(blocks((size_t)nnz, 256)), dim3(256), 0, s2, tab->RT, RP,
+  // s3: projection, root + SSQ, solid angle (needs the lists and the set-up)
+  HIPCHK(hipStreamWaitEvent(s3, ev_lists, 0));
+  HIPCHK(hipStreamWaitEvent(s3, ev_setup, 0));
+  rec(5, 0, s3);
+  hipLaunchKernelGGL(rrq_project_kernel, dim3(blocks((size_t)nnz, 256)), dim3(256), 0, s3, tab->RT, RP,
                      (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol,
                      (double*)d_pq.ptr);
-  rec(5, 1, s2);
-  rec(6, 0, s2);
+  rec(5, 1, s3);
+  rec(6, 0, s3);
   launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                   (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
-                  (double*)d_oe.ptr, nnz, s2);
-  rec(6, 1, s2);
-  HIPCHK(hipEventRecord(ev_edge, s2));
-  // main: solve, then the moments/weights once the edges are in
-  rec(4, 0, stream);
-  launch_rrq_panel_solve(tab->RT, RP, npan, (const int*)d_nrrq.ptr, stream);
-  rec(4, 1, stream);
+                  (double*)d_root.ptr, (double*)d_oe.ptr, nnz, s3, tev[8][0]);
+  rec(6, 1, s3);
+  HIPCHK(hipEventRecord(ev_edge, s3));
+  // main: moments and weights once the solve and the edges are in
+  HIPCHK(hipStreamWaitEvent(stream, ev_solve, 0));
   HIPCHK(hipStreamWaitEvent(stream, ev_edge, 0));
   rec(7, 0, stream);
   const size_t lds3 = rrq_target_smem(h, nq, ncoef);
@@ -392,10 +407,18 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
 
 void Engine::stage_ms(double* ms) {
   HIPCHK(hipStreamSynchronize(stream));
-  for (int i = 0; i < 8; ++i) {
+  auto el = [&](hipEvent_t a, hipEvent_t b) {
     float t = 0.0f;
-    HIPCHK(hipEventElapsedTime(&t, tev[i][0], tev[i][1]));
-    ms[i] = t;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    return (double)t;
+  };
+  for (int i = 0; i < 8; ++i) ms[i] = el(tev[i][0], tev[i][1]);
+  // the edge stage is two kernels: root + SSQ, then the solid angle (tev[8][0] between)
+  if (nnz > 0) {
+    ms[6] = el(tev[6][0], tev[8][0]);
+    ms[8] = el(tev[8][0], tev[6][1]);
+  } else {
+    ms[8] = 0.0;
   }
 }
 

@@ -28,19 +28,20 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
   extern __shared__ double smem[];
   const int h = A.h, k0 = blockIdx.x * kLP;
   const int np_ = min(kLP, npan - k0);
-  double* srcT = smem;  // [kLP][9][h]
+  constexpr int NV = kLP * 9;
+  double* srcT = smem;  // [h][kLP][9]: all values of panel node q together
   for (int e = threadIdx.x; e < kLP * h; e += blockDim.x) {
     const int pp = e / h, i = e % h;
-    double* S = srcT + (size_t)pp * 9 * h;
+    double* S = srcT + (size_t)i * NV + pp * 9;
     if (pp < np_) {
       const size_t g = (size_t)(k0 + pp) * h + i;
       for (int c = 0; c < 3; ++c) {
-        S[c * h + i] = sx[3 * g + c];
-        S[(3 + c) * h + i] = rts[3 * g + c];
-        S[(6 + c) * h + i] = rps[3 * g + c];
+        S[c] = sx[3 * g + c];
+        S[3 + c] = rts[3 * g + c];
+        S[6 + c] = rps[3 * g + c];
       }
     } else {
-      for (int c = 0; c < 9; ++c) S[c * h + i] = 0.0;
+      for (int c = 0; c < 9; ++c) S[c] = 0.0;
     }
   }
   __syncthreads();
@@ -53,12 +54,21 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
       for (int pp = 0; pp < kLP; ++pp)
 #pragma unroll
         for (int c = 0; c < 9; ++c) geo[pp][c] = 0.0;
+      // sequential in q for every accumulator (the oracle's order); the next
+      // interpolation entry is fetched one step ahead and the 36 LDS values of
+      // a step are read together
+      double vnext = IM[i];
       for (int q = 0; q < h; ++q) {
-        const double v = IM[i + (size_t)n * q];
+        const double v = vnext;
+        if (q + 1 < h) vnext = IM[i + (size_t)n * (q + 1)];
+        const double* sq = srcT + (size_t)q * NV;
+        double sv[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sv[c] = sq[c];
 #pragma unroll
         for (int pp = 0; pp < kLP; ++pp)
 #pragma unroll
-          for (int c = 0; c < 9; ++c) geo[pp][c] = geo[pp][c] + v * srcT[((size_t)pp * 9 + c) * h + q];
+          for (int c = 0; c < 9; ++c) geo[pp][c] = geo[pp][c] + v * sv[pp * 9 + c];
       }
 #pragma unroll
       for (int pp = 0; pp < kLP; ++pp) {
@@ -268,29 +278,36 @@ __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int*
     accS[q] = qd4{0.0, 0.0, 0.0, 0.0};
     accD[q] = qd4{0.0, 0.0, 0.0, 0.0};
   }
+  // branch-free body (clamped indices, selects) so the accumulators stay put
   for (int k0 = 0; k0 < n; k0 += 4) {
     const int i = k0 + kq;
     const bool in = i < n;
-    double s = 0.0, d = 0.0;
-    if (live && in) {
-      const double dx = x0 - G[i], dy = x1 - G[ntot + i], dz = x2 - G[2 * ntot + i];
-      const double rr = dx * dx + dy * dy + dz * dz;
-      if (rr > 1.0e-28) {
-        // 1/r from the hardware estimate and two Newton steps
-        double y = __builtin_amdgcn_rsq(rr);
-        double t = rr * y;
-        y = fma(0.5 * y, fma(-t, y, 1.0), y);
-        t = rr * y;
-        y = fma(0.5 * y, fma(-t, y, 1.0), y);
-        const double cw = c4 * G[6 * ntot + i];
-        const double dn = dx * G[3 * ntot + i] + dy * G[4 * ntot + i] + dz * G[5 * ntot + i];
-        s = cw * y;
-        d = cw * dn * (y * y * y);
-      }
-    }
+    const int ic = in ? i : n - 1;
+    const double gx = G[ic], gy = G[ntot + ic], gz = G[2 * ntot + ic];
+    const double nx = G[3 * ntot + ic], ny = G[4 * ntot + ic], nz = G[5 * ntot + ic], gw = G[6 * ntot + ic];
+    double bq[NT];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) bq[q] = B[(size_t)ic * pad + 16 * q + m];
+    const double dx = x0 - gx, dy = x1 - gy, dz = x2 - gz;
+    const double rr = dx * dx + dy * dy + dz * dz;
+    const bool ok = live && in && rr > 1.0e-28;
+    const double rs = ok ? rr : 1.0;
+    // 1/r from the hardware estimate and two Newton steps
+    double y = __builtin_amdgcn_rsq(rs);
+    double t = rs * y;
+    y = fma(0.5 * y, fma(-t, y, 1.0), y);
+    t = rs * y;
+    y = fma(0.5 * y, fma(-t, y, 1.0), y);
+    // every operand is finite (clamped index, rs = 1 when masked), so a 0/1 mask
+    // zeroes the masked lanes without a branch
+    const double msk = ok ? 1.0 : 0.0;
+    const double cw = msk * (c4 * gw);
+    const double dn = dx * nx + dy * ny + dz * nz;
+    const double s = cw * y;
+    const double d = cw * dn * (y * y * y);
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
-      const double bv = in ? B[(size_t)i * pad + 16 * q + m] : 0.0;
+      const double bv = in ? bq[q] : 0.0;
       accS[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(s, bv, accS[q], 0, 0, 0);
       accD[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(d, bv, accD[q], 0, 0, 0);
     }

@@ -85,9 +85,10 @@ bool rrq_solve_g1_in_lds(int h);
 __global__ void rrq_project_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
                                    const int* tidx, const double* tx, const double* sx, int exterior,
                                    double side_tol, double* pq);
+// root + SSQ kernel, then (after event `mid`, if given) the solid-angle kernel
 void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
-                     const int* tidx, const double* tx, const double* pq, double* w1, double* oe, int64_t nnz,
-                     hipStream_t s);
+                     const int* tidx, const double* tx, const double* pq, double* w1, double* root, double* oe,
+                     int64_t nnz, hipStream_t s, hipEvent_t mid);
 __global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot,
                                   int npan, const int* tidx, const double* tx, const int* nrrq, const double* w1g,
                                   const double* oeg, double* wout, double* omega_out, double* ias_out);

@@ -123,8 +123,9 @@ size_t rrq_setup_smem(int h, int nq, int ncoef) {
   return a + (b > c ? b : c);
 }
 
-// One workgroup per panel that has RRQ pairs: frame, boundary, basis data at the
-// boundary nodes, and the quaternion matrix M^T and Msl^T at the panel nodes.
+// One workgroup per panel (with RRQ pairs, or every panel when nrrq is null):
+// frame, boundary, basis data at the boundary nodes, and the quaternion matrix
+// and Msl^T at the panel nodes.
 __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan,
                                                               const int* __restrict__ nrrq,
                                                               const double* __restrict__ sx,
@@ -133,7 +134,7 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
                                                               const double* __restrict__ sxbd_chart,
                                                               const double* __restrict__ rv_chart) {
   const int k = blockIdx.x;
-  if (nrrq[k] == 0) return;
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
   extern __shared__ double sm[];
   const int h = T.h, nq = T.nq, p = T.p, nb = 3 * nq, ncoef = T.ncoef, H4 = 4 * h;
   double* ssx = sm;             // [3][h]
@@ -270,8 +271,9 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
     const double* dX = bx + 6 * ib + 3;
     cd g[3];
     harm_grad(H, n, m, g);
-    double* o = P.basis + (((size_t)k * nb + ib) * ncoef + kk) * 12;
-    for (int c = 0; c < 3; ++c) { o[2 * c] = g[c].re; o[2 * c + 1] = g[c].im; }
+    // component-major: o[c * ncoef] is component c of coefficient kk
+    double* o = P.basis + ((size_t)k * nb + ib) * 12 * ncoef + kk;
+    for (int c = 0; c < 3; ++c) { o[(2 * c) * ncoef] = g[c].re; o[(2 * c + 1) * ncoef] = g[c].im; }
     // V = n [dX_Z grad H'_{n-1}^m + i/2 (dW grad H'_{n-1}^{m-1} + conj(dW) grad H'_{n-1}^{m+1})]
     cd V[3] = {cmk(0, 0), cmk(0, 0), cmk(0, 0)};
     if (n >= 2) {
@@ -291,7 +293,7 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
         V[c] = cscale(cadd(cscale(gm[c], dX[0]), cscale(cmuli(t), 0.5)), (double)n);
       }
     }
-    for (int c = 0; c < 3; ++c) { o[6 + 2 * c] = V[c].re; o[6 + 2 * c + 1] = V[c].im; }
+    for (int c = 0; c < 3; ++c) { o[(6 + 2 * c) * ncoef] = V[c].re; o[(6 + 2 * c + 1) * ncoef] = V[c].im; }
   }
   __syncthreads();
   // quaternion system at the nodes.  F_k = Im((-i)^m H'_n^m), n=1..p, m=1..n.
@@ -301,166 +303,72 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
   double* FQ = P.Fq + (size_t)k * 3 * h * h;
   double* MS = P.Msl + (size_t)k * h * h;
   double* FN = P.Fn + (size_t)k * h * h;
-  for (int i0 = 0; i0 < h; i0 += 64) {
-    const int i = i0 + threadIdx.x;
-    if (threadIdx.x < 64 && i < h) {
-      cd* H = Hs + (size_t)threadIdx.x * ncoef;
-      double x[3] = {ssx[i], ssx[h + i], ssx[2 * h + i]}, nn0[3] = {ssn[i], ssn[h + i], ssn[2 * h + i]}, X[3], N[3];
-      to_frame(fr, x, X);
-      rotate(fr, nn0, N);
-      harm_vals(X, p, H);
-      int kk = 0;
-      for (int n = 1; n <= p; ++n)
-        for (int m = 1; m <= n; ++m, ++kk) {
-          cd g[3];
-          harm_grad(H, n, m, g);
-          const double F = imrot(m, H[hidx(n, m)]);
-          const double F1 = imrot(m, g[0]), F2 = imrot(m, g[1]), F3 = imrot(m, g[2]);
-          FN[i + (size_t)h * kk] = F;
-          MS[kk + (size_t)h * i] = N[0] * F1 + N[1] * F2 + N[2] * F3;
-          double* q = FQ + 3 * ((size_t)i * h + kk);
-          q[0] = F1; q[1] = F2; q[2] = F3;
-        }
+  // node positions and normals in the frame (in place), then per chunk of nodes:
+  // harmonics at the nodes, and one thread per (node, basis function)
+  for (int i = threadIdx.x; i < h; i += blockDim.x) {
+    const double x[3] = {ssx[i], ssx[h + i], ssx[2 * h + i]}, nn0[3] = {ssn[i], ssn[h + i], ssn[2 * h + i]};
+    double X[3], N[3];
+    to_frame(fr, x, X);
+    rotate(fr, nn0, N);
+    for (int c = 0; c < 3; ++c) { ssx[c * h + i] = X[c]; ssn[c * h + i] = N[c]; }
+  }
+  __syncthreads();
+  const int chunk = nb > 64 ? nb : 64;
+  for (int i0 = 0; i0 < h; i0 += chunk) {
+    const int ni = min(chunk, h - i0);
+    for (int t = threadIdx.x; t < ni; t += blockDim.x) {
+      const double X[3] = {ssx[i0 + t], ssx[h + i0 + t], ssx[2 * h + i0 + t]};
+      harm_vals(X, p, Hs + (size_t)t * ncoef);
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < ni * h; it += blockDim.x) {
+      const int t = it / h, kk = it % h, i = i0 + t;
+      int n = 1;
+      while ((n + 1) * n / 2 <= kk) ++n;
+      const int m = kk - (n - 1) * n / 2 + 1;
+      const cd* H = Hs + (size_t)t * ncoef;
+      cd g[3];
+      harm_grad(H, n, m, g);
+      const double F = imrot(m, H[hidx(n, m)]);
+      const double F1 = imrot(m, g[0]), F2 = imrot(m, g[1]), F3 = imrot(m, g[2]);
+      FN[i + (size_t)h * kk] = F;
+      MS[kk + (size_t)h * i] = ssn[i] * F1 + ssn[h + i] * F2 + ssn[2 * h + i] * F3;
+      double* q = FQ + 3 * ((size_t)i * h + kk);
+      q[0] = F1; q[1] = F2; q[2] = F3;
     }
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// Register-resident Gauss-Jordan elimination with partial pivoting.  The n x m
-// matrix (elements real, Q=1, or quaternions, Q=4, combined by left
-// multiplication) is spread over a 16 x 16 thread grid, thread (ty,tx) holding
-// rows ty+16a and columns tx+16b.  Rows are exchanged whole; the pivot is the
-// largest |element| in the column (lowest row index on ties).
-template <int Q>
-__device__ __forceinline__ void emul(const double* a, const double* b, double* c) {
-  if constexpr (Q == 1) {
-    c[0] = a[0] * b[0];
-  } else {
-    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
-    c[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
-    c[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
-    c[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
-  }
+// Weights operator of one panel by two Gauss-Jordan eliminations run in one loop:
+//   [Fq | I] -> [I | X],  X = Fq^{-1}   (h x h pure-quaternion matrix, elements
+//                                        combined by left multiplication), and
+//   [Msl^T | I] -> [I | B]              (real).
+// The matrices live in registers on a 16 x 16 thread grid, thread (ty, tx)
+// holding rows ty+16a and columns tx+16b.  Partial pivoting without moving rows:
+// step k picks the unused row with the largest |element| in column k (lowest
+// index on ties), eliminates column k from every other row and normalises the
+// pivot row, so row piv[k] ends as row k of the inverse.  Two barriers per step;
+// LDS buffers alternate between steps.
+//
+// With G1(i, (b,k)) = (X_ki)_b, Ad = G1 rhs / 4pi solves M^T w = rhs (first h
+// components), and As = B (W/4pi + Fn^T Ad) / alpha = (C rhs + B W) / (4pi alpha),
+// C = B Fn^T G1.  Rows j < h of G: As, rows h..2h-1: Ad.
+__device__ __forceinline__ void qmul_fma(const double* a, const double* b, double* c) {
+  c[0] = fma(a[0], b[0], fma(-a[1], b[1], fma(-a[2], b[2], -a[3] * b[3])));
+  c[1] = fma(a[0], b[1], fma(a[1], b[0], fma(a[2], b[3], -a[3] * b[2])));
+  c[2] = fma(a[0], b[2], fma(-a[1], b[3], fma(a[2], b[0], a[3] * b[1])));
+  c[3] = fma(a[0], b[3], fma(a[1], b[2], fma(-a[2], b[1], a[3] * b[0])));
 }
-template <int Q>
-__device__ __forceinline__ double enorm(const double* a) {
-  if constexpr (Q == 1) return fabs(a[0]);
-  else return a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
-}
-template <int Q>
-__device__ __forceinline__ void einv(const double* a, double* r) {
-  if constexpr (Q == 1) {
-    r[0] = 1.0 / a[0];
-  } else {
-    const double n2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
-    r[0] = a[0] / n2; r[1] = -a[1] / n2; r[2] = -a[2] / n2; r[3] = -a[3] / n2;
-  }
+// e -= a * b (quaternions)
+__device__ __forceinline__ void qsub_mul(double* e, const double* a, const double* b) {
+  e[0] = fma(-a[0], b[0], fma(a[1], b[1], fma(a[2], b[2], fma(a[3], b[3], e[0]))));
+  e[1] = fma(-a[0], b[1], fma(-a[1], b[0], fma(-a[2], b[3], fma(a[3], b[2], e[1]))));
+  e[2] = fma(-a[0], b[2], fma(a[1], b[3], fma(-a[2], b[0], fma(-a[3], b[1], e[2]))));
+  e[3] = fma(-a[0], b[3], fma(-a[1], b[2], fma(a[2], b[1], fma(-a[3], b[0], e[3]))));
 }
 
-template <int Q, int RPT, int CPT>
-__device__ void gauss_jordan(double (&A)[RPT][CPT][Q], int n, int m, double* srow, double* prow, double* nrow,
-                             double* colk, double* rv, int* ri) {
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  __shared__ int sip;
-  for (int k = 0; k < n; ++k) {
-    const int kt = k & 15, kb = k >> 4;
-    // pivot search in column k
-    if (tx == kt) {
-      double best = -1.0;
-      int bi = n;
-#pragma unroll
-      for (int a = 0; a < RPT; ++a) {
-        const int r = ty + 16 * a;
-#pragma unroll
-        for (int b = 0; b < CPT; ++b)
-          if (b == kb && r >= k && r < n) {
-            const double v = enorm<Q>(A[a][b]);
-            if (v > best) { best = v; bi = r; }
-          }
-      }
-      rv[ty] = best;
-      ri[ty] = bi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double best = -1.0;
-      int bi = k;
-      for (int t = 0; t < 16; ++t)
-        if (rv[t] > best || (rv[t] == best && ri[t] < bi)) { best = rv[t]; bi = ri[t]; }
-      sip = bi;
-    }
-    __syncthreads();
-    const int ip = sip;
-    // rows k and ip to LDS
-#pragma unroll
-    for (int a = 0; a < RPT; ++a) {
-      const int r = ty + 16 * a;
-      if (r == k || r == ip)
-#pragma unroll
-        for (int b = 0; b < CPT; ++b) {
-          const int c = tx + 16 * b;
-          if (c < m)
-            for (int q = 0; q < Q; ++q) (r == ip ? prow : srow)[Q * c + q] = A[a][b][q];
-        }
-    }
-    __syncthreads();
-    // swap in registers, normalised pivot row, multipliers of column k
-#pragma unroll
-    for (int a = 0; a < RPT; ++a) {
-      const int r = ty + 16 * a;
-      if (ip != k && (r == k || r == ip))
-#pragma unroll
-        for (int b = 0; b < CPT; ++b) {
-          const int c = tx + 16 * b;
-          if (c < m)
-            for (int q = 0; q < Q; ++q) A[a][b][q] = (r == k ? prow : srow)[Q * c + q];
-        }
-    }
-    double inv[Q];
-    einv<Q>(prow + Q * k, inv);
-    for (int c = threadIdx.x; c < m; c += blockDim.x) emul<Q>(inv, prow + Q * c, nrow + Q * c);
-    if (tx == kt)
-#pragma unroll
-      for (int a = 0; a < RPT; ++a) {
-        const int r = ty + 16 * a;
-#pragma unroll
-        for (int b = 0; b < CPT; ++b)
-          if (b == kb && r < n)
-            for (int q = 0; q < Q; ++q) colk[Q * r + q] = A[a][b][q];
-      }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < RPT; ++a) {
-      const int r = ty + 16 * a;
-      if (r >= n) continue;
-      const double* mr = colk + Q * r;
-#pragma unroll
-      for (int b = 0; b < CPT; ++b) {
-        const int c = tx + 16 * b;
-        // left-block columns up to k are finished (never read again)
-        if (c >= m || (c <= k && c < n)) continue;
-        if (r == k) {
-          for (int q = 0; q < Q; ++q) A[a][b][q] = nrow[Q * c + q];
-        } else {
-          double t[Q];
-          emul<Q>(mr, nrow + Q * c, t);
-          for (int q = 0; q < Q; ++q) A[a][b][q] = A[a][b][q] - t[q];
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Weights operator G (2h x 5h) acting on v = [rhs(4h); W(h)] from the quaternion
-// matrix Fq and Msl^T:
-//   X = Fq^{-1} (quaternion Gauss-Jordan on [Fq | I]); G1(i, (b,k)) = (X_ki)_b,
-//   so that Ad = G1 rhs / 4pi solves M^T w = rhs (first h components);
-//   B = (Msl^T)^{-1} (real Gauss-Jordan on [Msl^T | I]);
-//   As = B (W/4pi + Fn^T Ad) / alpha = (C rhs + B W) / (4pi alpha),  C = B Fn^T G1.
-// Rows j<h: As, rows h..2h-1: Ad.  B and B Fn^T (h x h) live in LDS, and G1
-// (h x 4h) too when it fits (P.scr, 4h^2 per panel, is used otherwise).
 bool rrq_solve_g1_in_lds(int h) { return (size_t)6 * h * h * sizeof(double) <= 120 * 1024; }
 size_t rrq_solve_smem(int h) {
   return sizeof(double) * (size_t)((rrq_solve_g1_in_lds(h) ? 4 * h * h : 0) + 2 * h * h);
@@ -470,11 +378,13 @@ template <int RPT, int CPT>
 __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPanels P, int npan,
                                                               const int* __restrict__ nrrq) {
   const int k = blockIdx.x;
-  if (nrrq[k] == 0) return;
-  const int h = T.h, H4 = 4 * h, H5 = 5 * h;
-  __shared__ double srow[4 * 16 * CPT], prow[4 * 16 * CPT], nrow[4 * 16 * CPT], colk[4 * 16 * RPT];
-  __shared__ double rv[16];
-  __shared__ int ri[16];
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
+  const int h = T.h, H4 = 4 * h, H5 = 5 * h, m = 2 * h;
+  constexpr int NR = 16 * RPT, NC = 16 * CPT;
+  __shared__ double colq[2][4 * NR], colr[2][NR], prq[2][4 * NC], prr[2][NC];
+  __shared__ double cvq[2][16], cvr[2][16];
+  __shared__ int ciq[2][16], cir[2][16];
+  __shared__ int pivq[NR], pivr[NR];
   extern __shared__ double dyn[];
   double* Bm = dyn;                      // [h][h] B(r,c) at r*h+c
   double* Cm = Bm + (size_t)h * h;       // [h][h] (B Fn^T)(j,i) at j*h+i
@@ -486,74 +396,149 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
   double* G = P.G + (size_t)k * 2 * h * H5;
   const double alpha = P.frame[(size_t)k * 16 + 12];
   const double fac = 1.0 / (16.0 * atan(1.0));
-  {
-    double A[RPT][CPT][4];
+  double Aq[RPT][CPT][4], Ar[RPT][CPT];
 #pragma unroll
-    for (int a = 0; a < RPT; ++a)
+  for (int a = 0; a < RPT; ++a)
+#pragma unroll
+    for (int b = 0; b < CPT; ++b) {
+      const int r = ty + 16 * a, c = tx + 16 * b;
+      double* e = Aq[a][b];
+      e[0] = e[1] = e[2] = e[3] = 0.0;
+      Ar[a][b] = 0.0;
+      if (r < h && c < h) {
+        const double* q = FQ + 3 * ((size_t)r * h + c);
+        e[1] = q[0]; e[2] = q[1]; e[3] = q[2];
+        Ar[a][b] = MS[r + (size_t)h * c];
+      } else if (r < h && c - h == r) {
+        e[0] = 1.0;
+        Ar[a][b] = 1.0;
+      }
+    }
+  // rows already used as pivots (same in every thread)
+  unsigned long long uq[2] = {0ull, 0ull}, ur[2] = {0ull, 0ull};
+  for (int kk = 0; kk < h; ++kk) {
+    const int buf = kk & 1, kt = kk & 15, kb = kk >> 4;
+    // column kk: pivot candidates and multipliers
+    if (tx == kt) {
+      double bq = -1.0, br = -1.0;
+      int iq = NR, ir = NR;
+#pragma unroll
+      for (int a = 0; a < RPT; ++a) {
+        const int r = ty + 16 * a;
+#pragma unroll
+        for (int b = 0; b < CPT; ++b)
+          if (b == kb && r < h) {
+            const double* e = Aq[a][b];
+            for (int q = 0; q < 4; ++q) colq[buf][4 * r + q] = e[q];
+            colr[buf][r] = Ar[a][b];
+            const bool fq = (uq[r >> 6] >> (r & 63)) & 1ull, fr = (ur[r >> 6] >> (r & 63)) & 1ull;
+            const double vq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3], vr = fabs(Ar[a][b]);
+            if (!fq && vq > bq) { bq = vq; iq = r; }
+            if (!fr && vr > br) { br = vr; ir = r; }
+          }
+      }
+      cvq[buf][ty] = bq; ciq[buf][ty] = iq;
+      cvr[buf][ty] = br; cir[buf][ty] = ir;
+    }
+    __syncthreads();
+    int pq = NR, pr = NR;
+    {
+      double bq = -1.0, br = -1.0;
+      for (int t = 0; t < 16; ++t) {
+        const double vq = cvq[buf][t], vr = cvr[buf][t];
+        const int iq = ciq[buf][t], ir = cir[buf][t];
+        if (vq > bq || (vq == bq && iq < pq)) { bq = vq; pq = iq; }
+        if (vr > br || (vr == br && ir < pr)) { br = vr; pr = ir; }
+      }
+    }
+    uq[pq >> 6] |= 1ull << (pq & 63);
+    ur[pr >> 6] |= 1ull << (pr & 63);
+    if (threadIdx.x == 0) { pivq[kk] = pq; pivr[kk] = pr; }
+    // pivot rows to LDS
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+      const int r = ty + 16 * a;
 #pragma unroll
       for (int b = 0; b < CPT; ++b) {
-        const int r = ty + 16 * a, c = tx + 16 * b;
-        double* e = A[a][b];
-        e[0] = e[1] = e[2] = e[3] = 0.0;
-        if (r < h && c < h) {
-          const double* q = FQ + 3 * ((size_t)r * h + c);
-          e[1] = q[0]; e[2] = q[1]; e[3] = q[2];
-        } else if (r < h && c - h == r) {
-          e[0] = 1.0;
+        const int c = tx + 16 * b;
+        if (c < m) {
+          if (r == pq)
+            for (int q = 0; q < 4; ++q) prq[buf][4 * c + q] = Aq[a][b][q];
+          if (r == pr) prr[buf][c] = Ar[a][b];
         }
       }
-    gauss_jordan<4, RPT, CPT>(A, h, 2 * h, srow, prow, nrow, colk, rv, ri);
+    }
+    __syncthreads();
+    // inverse of the pivots
+    double iq4[4];
+    {
+      const double* e = prq[buf] + 4 * kk;
+      const double n2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
+      const double in2 = 1.0 / n2;
+      iq4[0] = e[0] * in2; iq4[1] = -e[1] * in2; iq4[2] = -e[2] * in2; iq4[3] = -e[3] * in2;
+    }
+    const double irr = 1.0 / prr[buf][kk];
 #pragma unroll
-    for (int a = 0; a < RPT; ++a)
+    for (int b = 0; b < CPT; ++b) {
+      const int c = tx + 16 * b;
+      // left-block columns up to kk are finished (never read again)
+      if (c >= m || (c <= kk && c < h)) continue;
+      double nq[4];
+      qmul_fma(iq4, prq[buf] + 4 * c, nq);
+      const double nr = irr * prr[buf][c];
 #pragma unroll
-      for (int b = 0; b < CPT; ++b) {
-        const int kk = ty + 16 * a, c = tx + 16 * b;
-        if (kk < h && c >= h && c < 2 * h) {
-          const int i = c - h;
-          for (int q = 0; q < 4; ++q) G1[(size_t)i * H4 + q * h + kk] = A[a][b][q];
+      for (int a = 0; a < RPT; ++a) {
+        const int r = ty + 16 * a;
+        if (r >= h) continue;
+        if (r == pq) {
+          for (int q = 0; q < 4; ++q) Aq[a][b][q] = nq[q];
+        } else {
+          qsub_mul(Aq[a][b], colq[buf] + 4 * r, nq);
         }
+        if (r == pr) Ar[a][b] = nr;
+        else Ar[a][b] = fma(-colr[buf][r], nr, Ar[a][b]);
       }
+    }
   }
-  {
-    double A[RPT][CPT][1];
+  __syncthreads();
+  // row piv[kk] holds row kk of the inverse: scatter by the inverse permutation
+  __shared__ int posq[NR], posr[NR];
+  for (int kk = threadIdx.x; kk < h; kk += blockDim.x) { posq[pivq[kk]] = kk; posr[pivr[kk]] = kk; }
+  __syncthreads();
 #pragma unroll
-    for (int a = 0; a < RPT; ++a)
+  for (int a = 0; a < RPT; ++a)
 #pragma unroll
-      for (int b = 0; b < CPT; ++b) {
-        const int r = ty + 16 * a, c = tx + 16 * b;
-        A[a][b][0] = (r < h && c < h) ? MS[r + (size_t)h * c] : ((r < h && c - h == r) ? 1.0 : 0.0);
+    for (int b = 0; b < CPT; ++b) {
+      const int r = ty + 16 * a, c = tx + 16 * b;
+      if (r < h && c >= h && c < m) {
+        const int i = c - h;
+        const int kq = posq[r], kr = posr[r];
+        for (int q = 0; q < 4; ++q) G1[(size_t)i * H4 + q * h + kq] = Aq[a][b][q];
+        Bm[(size_t)kr * h + i] = Ar[a][b];
       }
-    gauss_jordan<1, RPT, CPT>(A, h, 2 * h, srow, prow, nrow, colk, rv, ri);
-#pragma unroll
-    for (int a = 0; a < RPT; ++a)
-#pragma unroll
-      for (int b = 0; b < CPT; ++b) {
-        const int r = ty + 16 * a, c = tx + 16 * b;
-        if (r < h && c >= h && c < 2 * h) Bm[(size_t)r * h + (c - h)] = A[a][b][0];
-      }
-  }
+    }
   __syncthreads();
   // C = B Fn^T
   for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
     const int j = e / h, i = e % h;
     double s = 0.0;
-    for (int r = 0; r < h; ++r) s += Bm[(size_t)j * h + r] * FN[i + (size_t)h * r];
+    for (int r = 0; r < h; ++r) s = fma(Bm[(size_t)j * h + r], FN[i + (size_t)h * r], s);
     Cm[(size_t)j * h + i] = s;
   }
   __syncthreads();
   const double fa = fac / alpha;
   for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
     const int j = e / H5, c = e % H5;
-    // Ad rows
-    G[(size_t)(h + j) * H5 + c] = c < H4 ? G1[(size_t)j * H4 + c] * fac : 0.0;
+    // Ad rows (G stored transposed, G(row, c) at c*2h + row)
+    G[(size_t)c * m + h + j] = c < H4 ? G1[(size_t)j * H4 + c] * fac : 0.0;
     // As rows
     double s = 0.0;
     if (c < H4) {
-      for (int i = 0; i < h; ++i) s += Cm[(size_t)j * h + i] * G1[(size_t)i * H4 + c];
+      for (int i = 0; i < h; ++i) s = fma(Cm[(size_t)j * h + i], G1[(size_t)i * H4 + c], s);
     } else {
       s = Bm[(size_t)j * h + (c - H4)];
     }
-    G[(size_t)j * H5 + c] = s * fa;
+    G[(size_t)c * m + j] = s * fa;
   }
 }
 
@@ -837,12 +822,12 @@ __global__ void __launch_bounds__(256) rrq_project_kernel(RRQTables T, RRQPanels
   o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = side;
 }
 
+// per (pair, edge): complex root and SSQ weights -> w1[r][3NQ], root[r][3][3] (re, im, ok)
 template <int NQ>
-__global__ void __launch_bounds__(256) rrq_edge_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
+__global__ void __launch_bounds__(256) rrq_root_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
                                                        const int* __restrict__ ltot, const int* __restrict__ ppanel,
                                                        const int* __restrict__ tidx, const double* __restrict__ tx,
-                                                       const double* __restrict__ pq, double* __restrict__ w1,
-                                                       double* __restrict__ oe) {
+                                                       double* __restrict__ w1, double* __restrict__ root) {
   const int it = blockIdx.x * blockDim.x + threadIdx.x;
   if (it >= 3 * ltot[4]) return;
   const int r = it / 3, e = it % 3;
@@ -855,23 +840,51 @@ __global__ void __launch_bounds__(256) rrq_edge_kernel(RRQTables T, RRQPanels P,
   to_frame(fr, x, X);
   const double* cf = P.edge + (size_t)k * 9 * NQ + e * 3 * NQ;
   const double rho_thr = pow(8.0, 8.0 / (double)NQ);
-  cd root;
+  cd s0;
   bool ok;
-  ssq_w1<NQ>(cf, T.tgl, T.wgl, X, rho_thr, w1 + (size_t)r * 3 * NQ + e * NQ, &root, &ok);
+  ssq_w1<NQ>(cf, T.tgl, T.wgl, X, rho_thr, w1 + (size_t)r * 3 * NQ + e * NQ, &s0, &ok);
+  double* o = root + 3 * (size_t)it;
+  o[0] = s0.re; o[1] = s0.im; o[2] = ok ? 1.0 : 0.0;
+}
+
+// per (pair, edge): the edge's Dirac-string line integral of the solid angle -> oe[r][3]
+template <int NQ>
+__global__ void __launch_bounds__(256) rrq_solid_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
+                                                        const int* __restrict__ ltot, const int* __restrict__ ppanel,
+                                                        const int* __restrict__ tidx, const double* __restrict__ tx,
+                                                        const double* __restrict__ pq, const double* __restrict__ root,
+                                                        double* __restrict__ oe) {
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= 3 * ltot[4]) return;
+  const int r = it / 3, e = it % 3;
+  const int pair = lpairs[rrq_list_base(ltot) + r];
+  const int k = ppanel[pair];
+  const double* fr = P.frame + (size_t)k * 16;
+  const size_t g = (size_t)tidx[pair];
+  const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
+  double X[3];
+  to_frame(fr, x, X);
+  const double* cf = P.edge + (size_t)k * 9 * NQ + e * 3 * NQ;
+  const double rho_thr = pow(8.0, 8.0 / (double)NQ);
+  const double* rt = root + 3 * (size_t)it;
   const double q[3] = {pq[4 * (size_t)r], pq[4 * (size_t)r + 1], pq[4 * (size_t)r + 2]};
-  oe[3 * (size_t)r + e] = solid_angle_edge<NQ>(cf, T.tgl, T.wgl, X, q, root, ok, rho_thr);
+  oe[it] = solid_angle_edge<NQ>(cf, T.tgl, T.wgl, X, q, cmk(rt[0], rt[1]), rt[2] != 0.0, rho_thr);
 }
 
 void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
-                     const int* tidx, const double* tx, const double* pq, double* w1, double* oe, int64_t nnz,
-                     hipStream_t s) {
+                     const int* tidx, const double* tx, const double* pq, double* w1, double* root, double* oe,
+                     int64_t nnz, hipStream_t s, hipEvent_t mid) {
   const dim3 grid((unsigned)((3 * nnz + 255) / 256)), blk(256);
-#define QOL_EDGE(NQ_) \
-  case NQ_: hipLaunchKernelGGL(rrq_edge_kernel<NQ_>, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, pq, w1, oe); break;
+#define QOL_EDGE(NQ_)                                                                                      \
+  case NQ_:                                                                                                \
+    hipLaunchKernelGGL(rrq_root_kernel<NQ_>, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, w1, root); \
+    if (mid) (void)hipEventRecord(mid, s);                                                                 \
+    hipLaunchKernelGGL(rrq_solid_kernel<NQ_>, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, pq, root, oe); \
+    break;
   switch (T.nq) {
     QOL_EDGE(4) QOL_EDGE(5) QOL_EDGE(6) QOL_EDGE(7) QOL_EDGE(8) QOL_EDGE(9) QOL_EDGE(10) QOL_EDGE(11)
     QOL_EDGE(12) QOL_EDGE(13) QOL_EDGE(14)
-    default: throw std::runtime_error("RRQ edge kernel: patch orders 2..12 only");
+    default: throw std::runtime_error("RRQ edge kernels: patch orders 2..12 only");
   }
 #undef QOL_EDGE
 }
@@ -893,6 +906,8 @@ __host__ __device__ int rrq_tile(int h, int nq, int ncoef) {
 size_t rrq_target_smem(int h, int nq, int ncoef) {
   return rrq_target_fixed(h, nq) + rrq_target_per(h, nq, ncoef) * rrq_tile(h, nq, ncoef);
 }
+
+constexpr int kTG = 2;  // targets per thread in the moment and weight loops
 
 __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
                                                          const int* __restrict__ lstart, const int* __restrict__ ltot,
@@ -953,33 +968,51 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       for (int c = 0; c < 3; ++c) { o[c] = wr * zx[c]; o[3 + c] = wr * z[c]; }
     }
     __syncthreads();
-    // boundary moments chi (1 complex) and alpha (4 complex) per coefficient
-    for (int it = threadIdx.x; it < nt * ncoef; it += blockDim.x) {
-      const int t = it / ncoef, kk = it % ncoef;
-      double c0r = 0, c0i = 0, a0r = 0, a0i = 0, a1r = 0, a1i = 0, a2r = 0, a2i = 0, a3r = 0, a3i = 0;
-      for (int ib = 0; ib < nb; ++ib) {
-        const double* o = ab + (size_t)t * big + 6 * ib;
-        const double* B = basis + ((size_t)ib * ncoef + kk) * 12;
-        // chi += a . gradH
-        c0r += o[0] * B[0] + o[1] * B[2] + o[2] * B[4];
-        c0i += o[0] * B[1] + o[1] * B[3] + o[2] * B[5];
-        // z (x) V = (-z.V, z x V), z = o[3..5]
-        const double z0 = o[3], z1 = o[4], z2 = o[5];
-        a0r -= z0 * B[6] + z1 * B[8] + z2 * B[10];
-        a0i -= z0 * B[7] + z1 * B[9] + z2 * B[11];
-        a1r += z1 * B[10] - z2 * B[8];
-        a1i += z1 * B[11] - z2 * B[9];
-        a2r += z2 * B[6] - z0 * B[10];
-        a2i += z2 * B[7] - z0 * B[11];
-        a3r += z0 * B[8] - z1 * B[6];
-        a3i += z0 * B[9] - z1 * B[7];
+    // boundary moments chi (1 complex) and alpha (4 complex) per coefficient;
+    // a thread serves one coefficient for kTG targets, so each basis value is
+    // read once per group (basis is component-major: consecutive kk coalesce)
+    {
+      const int ngrp = (nt + kTG - 1) / kTG;
+      for (int it = threadIdx.x; it < ngrp * ncoef; it += blockDim.x) {
+        const int kk = it % ncoef, t0 = kTG * (it / ncoef);
+        double acc[kTG][10];
+#pragma unroll
+        for (int g = 0; g < kTG; ++g)
+#pragma unroll
+          for (int c = 0; c < 10; ++c) acc[g][c] = 0.0;
+#pragma unroll 2
+        for (int ib = 0; ib < nb; ++ib) {
+          const double* Bp = basis + (size_t)ib * 12 * ncoef + kk;
+          double B[12];
+#pragma unroll
+          for (int c = 0; c < 12; ++c) B[c] = Bp[c * ncoef];
+#pragma unroll
+          for (int g = 0; g < kTG; ++g) {
+            if (t0 + g >= nt) break;
+            const double* o = ab + (size_t)(t0 + g) * big + 6 * ib;
+            const double o0 = o[0], o1 = o[1], o2 = o[2], z0 = o[3], z1 = o[4], z2 = o[5];
+            double* A = acc[g];
+            // chi += a . gradH ; z (x) V = (-z.V, z x V)
+            A[0] = fma(o0, B[0], fma(o1, B[2], fma(o2, B[4], A[0])));
+            A[1] = fma(o0, B[1], fma(o1, B[3], fma(o2, B[5], A[1])));
+            A[2] = fma(-z0, B[6], fma(-z1, B[8], fma(-z2, B[10], A[2])));
+            A[3] = fma(-z0, B[7], fma(-z1, B[9], fma(-z2, B[11], A[3])));
+            A[4] = fma(z1, B[10], fma(-z2, B[8], A[4]));
+            A[5] = fma(z1, B[11], fma(-z2, B[9], A[5]));
+            A[6] = fma(z2, B[6], fma(-z0, B[10], A[6]));
+            A[7] = fma(z2, B[7], fma(-z0, B[11], A[7]));
+            A[8] = fma(z0, B[8], fma(-z1, B[6], A[8]));
+            A[9] = fma(z0, B[9], fma(-z1, B[7], A[9]));
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < kTG; ++g) {
+          if (t0 + g >= nt) break;
+          cd* M = mom + ((size_t)(t0 + g) * ncoef + kk) * 5;
+#pragma unroll
+          for (int c = 0; c < 5; ++c) M[c] = cmk(acc[g][2 * c], acc[g][2 * c + 1]);
+        }
       }
-      cd* M = mom + ((size_t)t * ncoef + kk) * 5;
-      M[0] = cmk(c0r, c0i);
-      M[1] = cmk(a0r, a0i);
-      M[2] = cmk(a1r, a1i);
-      M[3] = cmk(a2r, a2i);
-      M[4] = cmk(a3r, a3i);
     }
     __syncthreads();
     // seeds: W~_00 = solid angle, degree-1 Q = Lambda (x) grad H'_1^m with
@@ -1064,14 +1097,26 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       }
       for (int t = threadIdx.x; t < nt; t += blockDim.x) ias_out[plist[t]] = sigma * tq[4 * t];
     }
-    // weights: out(j) = sum_r G(j, r) v(r)
-    for (int it = threadIdx.x; it < nt * 2 * h; it += blockDim.x) {
-      const int t = it / (2 * h), j = it % (2 * h);
-      const double* v = ab + (size_t)t * big;
-      const double* g = G + (size_t)j * H5;
-      double s = 0.0;
-      for (int r = 0; r < H5; ++r) s += g[r] * v[r];
-      wout[(size_t)plist[t] * 2 * h + j] = s;
+    // weights: out(j) = sum_r G(j, r) v(r); a thread serves row j for kTG
+    // targets (G is stored transposed: consecutive j coalesce)
+    {
+      const int m2 = 2 * h, ngrp = (nt + kTG - 1) / kTG;
+      for (int it = threadIdx.x; it < ngrp * m2; it += blockDim.x) {
+        const int j = it % m2, t0 = kTG * (it / m2);
+        double acc[kTG];
+#pragma unroll
+        for (int g = 0; g < kTG; ++g) acc[g] = 0.0;
+#pragma unroll 4
+        for (int r = 0; r < H5; ++r) {
+          const double gv = G[(size_t)r * m2 + j];
+#pragma unroll
+          for (int g = 0; g < kTG; ++g)
+            if (t0 + g < nt) acc[g] = fma(gv, ab[(size_t)(t0 + g) * big + r], acc[g]);
+        }
+#pragma unroll
+        for (int g = 0; g < kTG; ++g)
+          if (t0 + g < nt) wout[(size_t)plist[t0 + g] * m2 + j] = acc[g];
+      }
     }
   }
 }
