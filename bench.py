@@ -62,8 +62,10 @@ def kernel_flops(p, npan, counts, nrrq, nrrq_panels):
                                                    for l in range(4)))),
         # quaternion Gauss-Jordan 32h^3, real Gauss-Jordan 2h^3, B Fn^T 2h^3, C G1 8h^3
         "rrq_solve": ("rrq_panel_solve_kernel", 44.0 * h ** 3 * nrrq_panels),
-        # boundary moments 48 nb ncoef, degree recursion ~200 ncoef, weights GEMV 20 h^2
-        "rrq_targets": ("rrq_target_kernel", float(nrrq) * (48 * nb * ncoef + 200 * ncoef + 20 * h * h)),
+        # boundary moments 48 nb ncoef, degree recursion ~200 ncoef
+        "rrq_moments": ("rrq_target_kernel", float(nrrq) * (48 * nb * ncoef + 200 * ncoef)),
+        # W = G V: 2 (2h)(5h) per RRQ pair
+        "rrq_weights": ("rrq_weights_kernel", float(nrrq) * 20 * h * h),
     }
 
 

@@ -123,9 +123,10 @@ int qol_engine_get_pairs(void* engine, int64_t* offs, int64_t* idx);
  * patch take the exterior (1) / interior (0) limit; others their own side. */
 int qol_engine_eval(void* engine, int64_t exterior, double side_tol, int64_t want_omega);
 int qol_engine_sync(void* engine);
-/* stage times (ms) of the last eval, ms[9]: levels, classify (+ per-level pair
+/* stage times (ms) of the last eval, ms[10]: levels, classify (+ per-level pair
  * lists), direct, rrq setup, rrq solve, rrq projection, rrq root + SSQ,
- * rrq moments + weights, rrq solid angle.  Stages on different streams overlap. */
+ * rrq moments + recursion, rrq solid angle, rrq weights.  Stages on different
+ * streams overlap. */
 int qol_engine_stage_ms(void* engine, double* ms);
 int qol_engine_get_weights(void* engine, double* As, double* Ad, int64_t* label, double* Omega,
                            double* Ias);

@@ -59,7 +59,7 @@ void check_orders(int64_t order, int64_t nquad, int64_t orderff, double distff) 
 }
 
 // engine stage times (ms: levels, classify, direct, rrq setup, rrq solve,
-// projection, root + SSQ, moments+recursion+weights, solid angle) into the
+// projection, root + SSQ, moments+recursion, solid angle, weights) into the
 // reference's timeinfo slots (seconds, accumulated; lap3d_close_mod.f90:714-871):
 // 1 levels, 2 boundary precompute, 3 level direct, 4 SSQ, 5 solid angle
 // (with the projection that orients it), 6 moments and weights, 8 weights solve
@@ -69,7 +69,7 @@ void add_timeinfo(const double* ms, double* timeinfo) {
   timeinfo[2] += 1e-3 * (ms[1] + ms[2]);
   timeinfo[3] += 1e-3 * ms[6];
   timeinfo[4] += 1e-3 * (ms[5] + ms[8]);
-  timeinfo[5] += 1e-3 * ms[7];
+  timeinfo[5] += 1e-3 * (ms[7] + ms[9]);
   timeinfo[7] += 1e-3 * ms[4];
 }
 
@@ -236,7 +236,7 @@ void qol_rrq_mex_(const int64_t* m, const double* tx, const int64_t* n, const do
     e.set_pairs(offs.data(), idx.data());
     e.eval(*exterior != 0 ? 1 : 0, 1e-10, true);
     e.get_weights(As, Ad, nullptr, Omega, IalphaAsvestas);
-    double ms[9];
+    double ms[10];
     e.stage_ms(ms);
     add_timeinfo(ms, timeinfo);
   });
@@ -266,7 +266,7 @@ void qol_getnearquad_lap_rrq_mex_(const int64_t* npan, const int64_t* nterms, co
     e.set_pairs(offs.data(), idx.data());
     e.eval((*iside % 10) == 0 ? 1 : 0, 1e-10, false);
     e.triplets(tcxrow, sparsei, sparsej, sparsevs, sparsevd);
-    double ms[9];
+    double ms[10];
     e.stage_ms(ms);
     add_timeinfo(ms, timeinfo);
   });

@@ -314,7 +314,7 @@ void Engine::eval_levels_only() {
 // projection, root/SSQ and solid angle on s3 (after the set-up); the
 // moment/weights kernel joins s2 and s3, and the main stream joins s1 last.
 // tev[i] brackets stage i: levels, classify+lists, direct, rrq setup, rrq solve,
-// projection, root + SSQ (solid angle from tev[8][0]), moments.
+// projection, root + SSQ (solid angle from tev[8][0]), moments, -, weights.
 void Engine::eval(int exterior, double side_tol, bool want_omega) {
   if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
   if (p > 12) throw std::runtime_error("the RRQ kernels support patch orders up to 12");
@@ -325,7 +325,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
     rec(0, 1, stream);
     HIPCHK(hipMemsetAsync(d_nrrq.ptr, 0, sizeof(int) * npan, stream));
     HIPCHK(hipMemsetAsync(d_lcount.ptr, 0, sizeof(int) * 5 * npan, stream));
-    for (int i = 1; i < 9; ++i) { rec(i, 0, stream); rec(i, 1, stream); }
+    for (int i = 1; i < 10; ++i) { rec(i, 0, stream); rec(i, 1, stream); }
     HIPCHK(hipGetLastError());
     evaluated = true;
     have_omega = want_omega;
@@ -342,6 +342,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   d_w1.alloc(sizeof(double) * 3 * (size_t)nq * nnz);
   d_oe.alloc(sizeof(double) * 3 * (size_t)nnz);
   d_root.alloc(sizeof(double) * 9 * (size_t)nnz);
+  d_vt.alloc(sizeof(double) * 5 * (size_t)h * nnz);
   HIPCHK(hipEventRecord(ev_start, stream));
   // s2: RRQ set-up and solve for every panel
   HIPCHK(hipStreamWaitEvent(s2, ev_start, 0));
@@ -395,9 +396,13 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   hipLaunchKernelGGL(rrq_target_kernel, dim3(npan), dim3(256), lds3, stream, tab->RT, RP, (const int*)d_lpairs.ptr,
                      (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan, (const int*)d_tidx.ptr,
                      (const double*)d_tx.ptr, (const int*)d_nrrq.ptr, (const double*)d_w1.ptr,
-                     (const double*)d_oe.ptr, (double*)d_wout.ptr, want_omega ? (double*)d_omega.ptr : nullptr,
+                     (const double*)d_oe.ptr, (double*)d_vt.ptr, want_omega ? (double*)d_omega.ptr : nullptr,
                      want_omega ? (double*)d_ias.ptr : nullptr);
   rec(7, 1, stream);
+  rec(9, 0, stream);
+  launch_rrq_weights(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan,
+                     (const int*)d_nrrq.ptr, (const double*)d_vt.ptr, (double*)d_wout.ptr, stream);
+  rec(9, 1, stream);
   // join: everything after eval on the main stream sees every weight
   HIPCHK(hipStreamWaitEvent(stream, ev_direct, 0));
   HIPCHK(hipGetLastError());
@@ -413,6 +418,7 @@ void Engine::stage_ms(double* ms) {
     return (double)t;
   };
   for (int i = 0; i < 8; ++i) ms[i] = el(tev[i][0], tev[i][1]);
+  ms[9] = el(tev[9][0], tev[9][1]);
   // the edge stage is two kernels: root + SSQ, then the solid angle (tev[8][0] between)
   if (nnz > 0) {
     ms[6] = el(tev[6][0], tev[8][0]);

@@ -234,7 +234,6 @@ __global__ void __launch_bounds__(256) compact_kernel(const int* __restrict__ of
 // v_mfma_f64_16x16x4 against 4 rows of the level's interpolation matrix
 // (row-major, padded to 16*NT columns).  Accumulators: 16 pairs x 16*NT
 // outputs for S and for D.
-typedef double qd4 __attribute__((ext_vector_type(4)));
 
 template <int NT>
 __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int* __restrict__ lpairs,

@@ -14,6 +14,9 @@ namespace qol {
 #define QOL_MAXNQ 18          // QOL_MAXP + 2
 #define QOL_MAXNCOEF 153      // (QOL_MAXP+1)(QOL_MAXP+2)/2
 
+// result tile of v_mfma_f64_16x16x4 (4 doubles per lane)
+typedef double qd4 __attribute__((ext_vector_type(4)));
+
 struct cd {
   double re, im;
 };

@@ -91,7 +91,9 @@ void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* lto
                      int64_t nnz, hipStream_t s, hipEvent_t mid);
 __global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot,
                                   int npan, const int* tidx, const double* tx, const int* nrrq, const double* w1g,
-                                  const double* oeg, double* wout, double* omega_out, double* ias_out);
+                                  const double* oeg, double* vt, double* omega_out, double* ias_out);
+void launch_rrq_weights(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot, int npan,
+                        const int* nrrq, const double* vt, double* wout, hipStream_t s);
 size_t rrq_setup_smem(int h, int nq, int ncoef);
 size_t rrq_target_smem(int h, int nq, int ncoef);
 

@@ -171,15 +171,22 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
     cross3(xu, xv, n);
     sg[i] = n[0] * ssn[i] + n[1] * ssn[h + i] + n[2] * ssn[2 * h + i];
   }
+  // triangle vertices, one thread each (the patch at (0,0), (1,0), (0,1), or the chart's)
+  __shared__ double srv[9];
+  if (threadIdx.x < 3) {
+    const int j = threadIdx.x;
+    if (ichart == 1) {
+      for (int c = 0; c < 3; ++c) srv[3 * j + c] = rv_chart[9 * (size_t)k + 3 * j + c];
+    } else {
+      double xv[3];
+      geom_eval(scx, h, p - 1, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, xv, nullptr, nullptr);
+      for (int c = 0; c < 3; ++c) srv[3 * j + c] = xv[c];
+    }
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     double rv[9];
-    if (ichart == 1) {
-      for (int i = 0; i < 9; ++i) rv[i] = rv_chart[9 * (size_t)k + i];
-    } else {
-      const double uvv[6] = {0, 0, 1, 0, 0, 1};
-      for (int j = 0; j < 3; ++j) geom_eval(scx, h, p - 1, uvv[2 * j], uvv[2 * j + 1], rv + 3 * j, nullptr, nullptr);
-    }
+    for (int i = 0; i < 9; ++i) rv[i] = srv[i];
     double swsum = 0.0, acc = 0.0, mn[3] = {0, 0, 0};
     for (int i = 0; i < h; ++i) {
       swsum += ssw[i];
@@ -423,19 +430,22 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
       double bq = -1.0, br = -1.0;
       int iq = NR, ir = NR;
 #pragma unroll
-      for (int a = 0; a < RPT; ++a) {
-        const int r = ty + 16 * a;
+      for (int b = 0; b < CPT; ++b) {
+        if (b != kb) continue;  // uniform
 #pragma unroll
-        for (int b = 0; b < CPT; ++b)
-          if (b == kb && r < h) {
+        for (int a = 0; a < RPT; ++a) {
+          const int r = ty + 16 * a;
+          if (r < h) {
             const double* e = Aq[a][b];
             for (int q = 0; q < 4; ++q) colq[buf][4 * r + q] = e[q];
             colr[buf][r] = Ar[a][b];
             const bool fq = (uq[r >> 6] >> (r & 63)) & 1ull, fr = (ur[r >> 6] >> (r & 63)) & 1ull;
             const double vq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3], vr = fabs(Ar[a][b]);
-            if (!fq && vq > bq) { bq = vq; iq = r; }
-            if (!fr && vr > br) { br = vr; ir = r; }
+            const bool tq = !fq & (vq > bq), tr = !fr & (vr > br);
+            bq = tq ? vq : bq; iq = tq ? r : iq;
+            br = tr ? vr : br; ir = tr ? r : ir;
           }
+        }
       }
       cvq[buf][ty] = bq; ciq[buf][ty] = iq;
       cvr[buf][ty] = br; cir[buf][ty] = ir;
@@ -444,28 +454,35 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
     int pq = NR, pr = NR;
     {
       double bq = -1.0, br = -1.0;
+#pragma unroll
       for (int t = 0; t < 16; ++t) {
         const double vq = cvq[buf][t], vr = cvr[buf][t];
         const int iq = ciq[buf][t], ir = cir[buf][t];
-        if (vq > bq || (vq == bq && iq < pq)) { bq = vq; pq = iq; }
-        if (vr > br || (vr == br && ir < pr)) { br = vr; pr = ir; }
+        // bitwise forms: no short-circuit branches
+        const bool tq = (vq > bq) | ((vq == bq) & (iq < pq)), tr = (vr > br) | ((vr == br) & (ir < pr));
+        bq = tq ? vq : bq; pq = tq ? iq : pq;
+        br = tr ? vr : br; pr = tr ? ir : pr;
       }
     }
+    // identical in every lane: keep them in scalar registers
+    pq = __builtin_amdgcn_readfirstlane(pq);
+    pr = __builtin_amdgcn_readfirstlane(pr);
     uq[pq >> 6] |= 1ull << (pq & 63);
     ur[pr >> 6] |= 1ull << (pr & 63);
     if (threadIdx.x == 0) { pivq[kk] = pq; pivr[kk] = pr; }
-    // pivot rows to LDS
+    // pivot rows to LDS (row pq lives in slot a = pq/16 of the threads with ty =
+    // pq%16); all 16*CPT columns are written (the buffers have room for them)
 #pragma unroll
     for (int a = 0; a < RPT; ++a) {
-      const int r = ty + 16 * a;
+      if (a == (pq >> 4) && ty == (pq & 15)) {
 #pragma unroll
-      for (int b = 0; b < CPT; ++b) {
-        const int c = tx + 16 * b;
-        if (c < m) {
-          if (r == pq)
-            for (int q = 0; q < 4; ++q) prq[buf][4 * c + q] = Aq[a][b][q];
-          if (r == pr) prr[buf][c] = Ar[a][b];
-        }
+        for (int b = 0; b < CPT; ++b)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) prq[buf][4 * (tx + 16 * b) + q] = Aq[a][b][q];
+      }
+      if (a == (pr >> 4) && ty == (pr & 15)) {
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) prr[buf][tx + 16 * b] = Ar[a][b];
       }
     }
     __syncthreads();
@@ -478,25 +495,28 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
       iq4[0] = e[0] * in2; iq4[1] = -e[1] * in2; iq4[2] = -e[2] * in2; iq4[3] = -e[3] * in2;
     }
     const double irr = 1.0 / prr[buf][kk];
+    // branch-free update: clamped LDS indices, selects for the pivot rows and
+    // for finished / absent columns
 #pragma unroll
     for (int b = 0; b < CPT; ++b) {
       const int c = tx + 16 * b;
       // left-block columns up to kk are finished (never read again)
-      if (c >= m || (c <= kk && c < h)) continue;
+      const bool colok = c < m && !(c <= kk && c < h);
+      const int cc = c < m ? c : m - 1;
       double nq[4];
-      qmul_fma(iq4, prq[buf] + 4 * c, nq);
-      const double nr = irr * prr[buf][c];
+      qmul_fma(iq4, prq[buf] + 4 * cc, nq);
+      const double nr = irr * prr[buf][cc];
 #pragma unroll
       for (int a = 0; a < RPT; ++a) {
         const int r = ty + 16 * a;
-        if (r >= h) continue;
-        if (r == pq) {
-          for (int q = 0; q < 4; ++q) Aq[a][b][q] = nq[q];
-        } else {
-          qsub_mul(Aq[a][b], colq[buf] + 4 * r, nq);
-        }
-        if (r == pr) Ar[a][b] = nr;
-        else Ar[a][b] = fma(-colr[buf][r], nr, Ar[a][b]);
+        const int rc = r < h ? r : h - 1;
+        const bool isq = r == pq, isr = r == pr;
+        double e[4] = {Aq[a][b][0], Aq[a][b][1], Aq[a][b][2], Aq[a][b][3]};
+        qsub_mul(e, colq[buf] + 4 * rc, nq);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Aq[a][b][q] = colok ? (isq ? nq[q] : e[q]) : Aq[a][b][q];
+        const double er = fma(-colr[buf][rc], nr, Ar[a][b]);
+        Ar[a][b] = colok ? (isr ? nr : er) : Ar[a][b];
       }
     }
   }
@@ -914,7 +934,7 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
                                                          int npan, const int* __restrict__ tidx,
                                                          const double* __restrict__ tx, const int* __restrict__ nrrq,
                                                          const double* __restrict__ w1g, const double* __restrict__ oeg,
-                                                         double* __restrict__ wout, double* __restrict__ omega_out,
+                                                         double* __restrict__ vt, double* __restrict__ omega_out,
                                                          double* __restrict__ ias_out) {
   const int k = blockIdx.x;
   const int nmine = nrrq[k];
@@ -1097,28 +1117,73 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       }
       for (int t = threadIdx.x; t < nt; t += blockDim.x) ias_out[plist[t]] = sigma * tq[4 * t];
     }
-    // weights: out(j) = sum_r G(j, r) v(r); a thread serves row j for kTG
-    // targets (G is stored transposed: consecutive j coalesce)
-    {
-      const int m2 = 2 * h, ngrp = (nt + kTG - 1) / kTG;
-      for (int it = threadIdx.x; it < ngrp * m2; it += blockDim.x) {
-        const int j = it % m2, t0 = kTG * (it / m2);
-        double acc[kTG];
+    // moment vectors to HBM, per panel transposed: vt[r0*5h + i*nrrq + t]
+    for (int it = threadIdx.x; it < nt * H5; it += blockDim.x) {
+      const int t = it % nt, i = it / nt;
+      vt[(size_t)r0 * H5 + (size_t)i * nmine + tile0 + t] = ab[(size_t)t * big + i];
+    }
+  }
+}
+
+// RRQ weights of every RRQ pair of a panel: W (2h x nrrq) = G (2h x 5h) V (5h x nrrq)
+// on the FP64 matrix cores.  One workgroup of kWM waves per panel; wave w owns
+// output rows 16w..16w+15 and walks the panel's targets 16 at a time.  A operand:
+// G (stored transposed, coalesced over rows); B operand: the panel's moment
+// vectors (stored transposed, coalesced over targets).
+constexpr int kWM = 5;  // 16*kWM >= 2h for p <= 8; larger orders loop over row tiles
+
+__global__ void __launch_bounds__(64 * kWM) rrq_weights_kernel(RRQTables T, RRQPanels P,
+                                                               const int* __restrict__ lpairs,
+                                                               const int* __restrict__ lstart,
+                                                               const int* __restrict__ ltot, int npan,
+                                                               const int* __restrict__ nrrq,
+                                                               const double* __restrict__ vt,
+                                                               double* __restrict__ wout) {
+  const int k = blockIdx.x;
+  const int nm = nrrq[k];
+  if (nm == 0) return;
+  const int h = T.h, H5 = 5 * h, m2 = 2 * h;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r0 = lstart[4 * (size_t)npan + k];
+  const int* mine = lpairs + rrq_list_base(ltot) + r0;
+  const double* G = P.G + (size_t)k * m2 * H5;  // G(j, r) at r*2h + j
+  const double* V = vt + (size_t)r0 * H5;       // V(r, t) at r*nm + t
+  for (int mt = w; 16 * mt < m2; mt += kWM) {
+    const int j = 16 * mt + li;
+    const bool jin = j < m2;
+    const int jc = jin ? j : m2 - 1;
+    for (int t0 = 0; t0 < nm; t0 += 16) {
+      const int t = t0 + li;
+      const bool tin = t < nm;
+      const int tc = tin ? t : nm - 1;
+      qd4 acc = qd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 5
+      for (int k0 = 0; k0 < H5; k0 += 4) {
+        const int r = k0 + lk;
+        const bool rin = r < H5;
+        const int rc = rin ? r : H5 - 1;
+        const double a = (jin & rin) ? G[(size_t)rc * m2 + jc] : 0.0;
+        const double b = (tin & rin) ? V[(size_t)rc * nm + tc] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+      // D[4q + lk][li]: output row 16mt + 4q + lk, target t0 + li
+      if (tin) {
+        double* o = wout + (size_t)mine[t] * m2;
 #pragma unroll
-        for (int g = 0; g < kTG; ++g) acc[g] = 0.0;
-#pragma unroll 4
-        for (int r = 0; r < H5; ++r) {
-          const double gv = G[(size_t)r * m2 + j];
-#pragma unroll
-          for (int g = 0; g < kTG; ++g)
-            if (t0 + g < nt) acc[g] = fma(gv, ab[(size_t)(t0 + g) * big + r], acc[g]);
+        for (int q = 0; q < 4; ++q) {
+          const int jj = 16 * mt + 4 * q + lk;
+          if (jj < m2) o[jj] = acc[q];
         }
-#pragma unroll
-        for (int g = 0; g < kTG; ++g)
-          if (t0 + g < nt) wout[(size_t)plist[t0 + g] * m2 + j] = acc[g];
       }
     }
   }
+}
+
+void launch_rrq_weights(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot, int npan,
+                        const int* nrrq, const double* vt, double* wout, hipStream_t s) {
+  hipLaunchKernelGGL(rrq_weights_kernel, dim3(npan), dim3(64 * kWM), 0, s, T, P, lpairs, lstart, ltot, npan, nrrq,
+                     vt, wout);
 }
 
 }  // namespace qol

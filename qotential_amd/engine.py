@@ -93,10 +93,10 @@ class Engine:
         check(lib().qol_engine_sync(self._h), "qol_engine_sync")
 
     def stage_ms(self):
-        ms = np.zeros(9)
+        ms = np.zeros(10)
         check(lib().qol_engine_stage_ms(self._h, ptr(ms)), "qol_engine_stage_ms")
         return dict(levels=ms[0], classify=ms[1], direct=ms[2], rrq_setup=ms[3], rrq_solve=ms[4], rrq_project=ms[5],
-                    rrq_root_ssq=ms[6], rrq_targets=ms[7], rrq_solid_angle=ms[8])
+                    rrq_root_ssq=ms[6], rrq_moments=ms[7], rrq_solid_angle=ms[8], rrq_weights=ms[9])
 
     def weights(self, omega=False):
         n = max(self.nnz, 1)
