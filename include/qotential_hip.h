@@ -132,7 +132,12 @@ int qol_engine_get_weights(void* engine, double* As, double* Ad, int64_t* label,
                            double* Ias);
 /* near-correction apply + fixed-order scatter into u (in: far field) */
 int qol_engine_apply(void* engine, const double* ub, const double* ubn, double* u, double* val);
+/* with ub, ubn already resident (from qol_engine_apply or qol_engine_far): the
+ * corrections (made during eval when the density was resident before it) and the
+ * per-target sum into the resident u */
 int qol_engine_apply_resident(void* engine);
+/* copy back the per-pair corrections val(nnz) and the resident u(nt) (no compute) */
+int qol_engine_get_values(void* engine, double* val, double* u);
 /* far field by direct summation (stellarator_grf_core_mod.f90:560-611) */
 int qol_engine_far(void* engine, const double* ub, const double* ubn, double* u);
 int qol_engine_level_data(void* engine, double* lgeo, double* balls);

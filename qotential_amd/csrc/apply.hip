@@ -10,14 +10,12 @@ namespace qol {
 
 // val(pair) = sum_i (S - S_naive)_i ubn_i - (K - K_naive)_i ub_i, i over the
 // panel's h nodes, left to right as in the reference loop.
-__global__ void __launch_bounds__(256) naive_apply_kernel(int nnz, int h, const int* __restrict__ tidx,
-                                                          const int* __restrict__ ppanel, const double* __restrict__ tx,
-                                                          const double* __restrict__ sx, const double* __restrict__ snx,
-                                                          const double* __restrict__ sw, const double* __restrict__ ub,
-                                                          const double* __restrict__ ubn, const double* __restrict__ wout,
-                                                          double* __restrict__ val) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nnz) return;
+// near-correction value of pair j: sum_i (S - S_naive)_i ubn_i - (K - K_naive)_i ub_i
+__device__ __forceinline__ double pair_correction(int j, int h, const int* __restrict__ tidx,
+                                                  const int* __restrict__ ppanel, const double* __restrict__ tx,
+                                                  const double* __restrict__ sx, const double* __restrict__ snx,
+                                                  const double* __restrict__ sw, const double* __restrict__ ub,
+                                                  const double* __restrict__ ubn, const double* __restrict__ wout) {
   const double pi4inv = 1.0 / (16.0 * atan(1.0));
   const int t = tidx[j], k = ppanel[j];
   const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
@@ -35,7 +33,43 @@ __global__ void __launch_bounds__(256) naive_apply_kernel(int nnz, int h, const 
     }
     acc = acc + (w[i] - an) * ubn[g] - (w[h + i] - dn) * ub[g];
   }
-  val[j] = acc;
+  return acc;
+}
+
+__global__ void __launch_bounds__(256) naive_apply_kernel(int nnz, int h, const int* __restrict__ tidx,
+                                                          const int* __restrict__ ppanel, const double* __restrict__ tx,
+                                                          const double* __restrict__ sx, const double* __restrict__ snx,
+                                                          const double* __restrict__ sw, const double* __restrict__ ub,
+                                                          const double* __restrict__ ubn, const double* __restrict__ wout,
+                                                          double* __restrict__ val) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  val[j] = pair_correction(j, h, tidx, ppanel, tx, sx, snx, sw, ub, ubn, wout);
+}
+
+// the same for the pairs of the per-label lists first..last (lists concatenated
+// in label order, lengths ltot), so each part can run as soon as its weights are in
+__global__ void __launch_bounds__(256) naive_apply_list_kernel(const int* __restrict__ lpairs,
+                                                               const int* __restrict__ ltot, int first, int last,
+                                                               int h, const int* __restrict__ tidx,
+                                                               const int* __restrict__ ppanel,
+                                                               const double* __restrict__ tx,
+                                                               const double* __restrict__ sx,
+                                                               const double* __restrict__ snx,
+                                                               const double* __restrict__ sw,
+                                                               const double* __restrict__ ub,
+                                                               const double* __restrict__ ubn,
+                                                               const double* __restrict__ wout,
+                                                               double* __restrict__ val) {
+  int base = 0, cnt = 0;
+  for (int q = 0; q <= last; ++q) {
+    if (q < first) base += ltot[q];
+    else cnt += ltot[q];
+  }
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int j = lpairs[base + t];
+  val[j] = pair_correction(j, h, tidx, ppanel, tx, sx, snx, sw, ub, ubn, wout);
 }
 
 // COO triplets of getnearquad_lap_rrq: out = (tcxi(k)-1)*h + (j-1)*m + i (1-based)

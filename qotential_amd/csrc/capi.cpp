@@ -133,6 +133,10 @@ int qol_engine_apply_resident(void* h) {
   return guarded([&] { E(h)->apply_device(); });
 }
 
+int qol_engine_get_values(void* h, double* val, double* u) {
+  return guarded([&] { E(h)->get_values(val, u); });
+}
+
 int qol_engine_far(void* h, const double* ub, const double* ubn, double* u) {
   return guarded([&] { E(h)->far(ub, ubn, u); });
 }

@@ -327,6 +327,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
     HIPCHK(hipMemsetAsync(d_lcount.ptr, 0, sizeof(int) * 5 * npan, stream));
     for (int i = 1; i < 10; ++i) { rec(i, 0, stream); rec(i, 1, stream); }
     HIPCHK(hipGetLastError());
+    val_current = false;
     evaluated = true;
     have_omega = want_omega;
     return;
@@ -371,6 +372,9 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   rec(2, 0, s1);
   direct(s1);
   rec(2, 1, s1);
+  // with the density resident, the level-served pairs' corrections follow at once
+  const bool fuse = density_set;
+  if (fuse) launch_correction(0, 3, s1);
   HIPCHK(hipEventRecord(ev_direct, s1));
   // s3: projection, root + SSQ, solid angle (needs the lists and the set-up)
   HIPCHK(hipStreamWaitEvent(s3, ev_lists, 0));
@@ -387,8 +391,8 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                   (double*)d_root.ptr, (double*)d_oe.ptr, nnz, s3, tev[8][0]);
   rec(6, 1, s3);
   HIPCHK(hipEventRecord(ev_edge, s3));
-  // main: moments and weights once the solve and the edges are in
-  HIPCHK(hipStreamWaitEvent(stream, ev_solve, 0));
+  // main: moments once the edges are in (the set-up precedes them on s3),
+  // weights once the solve is in too
   HIPCHK(hipStreamWaitEvent(stream, ev_edge, 0));
   rec(7, 0, stream);
   const size_t lds3 = rrq_target_smem(h, nq, ncoef);
@@ -399,12 +403,15 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                      (const double*)d_oe.ptr, (double*)d_vt.ptr, want_omega ? (double*)d_omega.ptr : nullptr,
                      want_omega ? (double*)d_ias.ptr : nullptr);
   rec(7, 1, stream);
+  HIPCHK(hipStreamWaitEvent(stream, ev_solve, 0));
   rec(9, 0, stream);
   launch_rrq_weights(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan,
                      (const int*)d_nrrq.ptr, (const double*)d_vt.ptr, (double*)d_wout.ptr, stream);
   rec(9, 1, stream);
+  if (fuse) launch_correction(4, 4, stream);
   // join: everything after eval on the main stream sees every weight
   HIPCHK(hipStreamWaitEvent(stream, ev_direct, 0));
+  val_current = fuse;
   HIPCHK(hipGetLastError());
   evaluated = true;
   have_omega = want_omega;
@@ -464,6 +471,8 @@ void Engine::apply(const double* ub, const double* ubn, double* u, double* val, 
   HIPCHK(hipMemcpyAsync(d_ub.ptr, ub, sizeof(double) * nsrc, kin, stream));
   HIPCHK(hipMemcpyAsync(d_ubn.ptr, ubn, sizeof(double) * nsrc, kin, stream));
   if (nt > 0) HIPCHK(hipMemcpyAsync(d_u.ptr, u, sizeof(double) * nt, kin, stream));
+  density_set = true;
+  val_current = false;  // the density may have changed since the last eval
   apply_device();
   if (nt > 0) HIPCHK(hipMemcpyAsync(u, d_u.ptr, sizeof(double) * nt, kout, stream));
   if (val && nnz > 0) HIPCHK(hipMemcpyAsync(val, d_val.ptr, sizeof(double) * nnz, kout, stream));
@@ -487,16 +496,35 @@ void Engine::build_target_lists() {
   tlists_nnz = nnz;
 }
 
+// corrections of the pairs in per-label lists first..last (after their weights)
+void Engine::launch_correction(int first, int last, hipStream_t s) {
+  hipLaunchKernelGGL(naive_apply_list_kernel, dim3(blocks((size_t)nnz, 256)), dim3(256), 0, s,
+                     (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, first, last, h, (const int*)d_tidx.ptr,
+                     (const int*)d_ppanel.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr,
+                     (const double*)d_snx.ptr, (const double*)d_sw.ptr, (const double*)d_ub.ptr,
+                     (const double*)d_ubn.ptr, (const double*)d_wout.ptr, (double*)d_val.ptr);
+}
+
+// near-correction values (unless the last eval already made them from the
+// resident density) and the fixed-order per-target sum into u
 void Engine::apply_device() {
   if (nnz <= 0) return;
-  hipLaunchKernelGGL(naive_apply_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz, h,
-                     (const int*)d_tidx.ptr, (const int*)d_ppanel.ptr, (const double*)d_tx.ptr,
-                     (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,
-                     (const double*)d_ub.ptr, (const double*)d_ubn.ptr, (const double*)d_wout.ptr, (double*)d_val.ptr);
+  if (!val_current)
+    hipLaunchKernelGGL(naive_apply_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz, h,
+                       (const int*)d_tidx.ptr, (const int*)d_ppanel.ptr, (const double*)d_tx.ptr,
+                       (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,
+                       (const double*)d_ub.ptr, (const double*)d_ubn.ptr, (const double*)d_wout.ptr,
+                       (double*)d_val.ptr);
   build_target_lists();
   hipLaunchKernelGGL(target_sum_kernel, dim3(blocks(nt, 256)), dim3(256), 0, stream, nt, (const int*)d_toffs.ptr,
                      (int*)d_tpairs.ptr, (const double*)d_val.ptr, (double*)d_u.ptr);
   HIPCHK(hipGetLastError());
+}
+
+void Engine::get_values(double* val, double* u) {
+  HIPCHK(hipStreamSynchronize(stream));
+  if (val && nnz > 0) HIPCHK(hipMemcpy(val, d_val.ptr, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+  if (u && nt > 0) HIPCHK(hipMemcpy(u, d_u.ptr, sizeof(double) * nt, hipMemcpyDeviceToHost));
 }
 
 void Engine::far(const double* ub, const double* ubn, double* u) {
@@ -505,6 +533,8 @@ void Engine::far(const double* ub, const double* ubn, double* u) {
   d_u.alloc(sizeof(double) * (size_t)(nt > 0 ? nt : 1));
   HIPCHK(hipMemcpyAsync(d_ub.ptr, ub, sizeof(double) * nsrc, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(d_ubn.ptr, ubn, sizeof(double) * nsrc, hipMemcpyHostToDevice, stream));
+  density_set = true;
+  val_current = false;
   if (nt > 0)
     hipLaunchKernelGGL(far_direct_kernel, dim3(blocks(nt, 256)), dim3(256), 0, stream, nt, (const double*)d_tx.ptr,
                        (int)nsrc, (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,

@@ -33,6 +33,8 @@ struct Engine {
   int nt = -1;
   i64 nnz = -1;
   bool levels_built = false, evaluated = false, have_omega = false, rrq_alloc = false;
+  bool density_set = false;   // ub, ubn resident (eval then also makes the corrections)
+  bool val_current = false;   // d_val holds the corrections of the last eval
   i64 tlists_nnz = -1;
   hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
   hipEvent_t tev[10][2] = {};
@@ -65,8 +67,10 @@ struct Engine {
   void get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias);
   void apply(const double* ub, const double* ubn, double* u, double* val, bool device_ptrs);
   void apply_device();
+  void launch_correction(int first, int last, hipStream_t s);
   void build_target_lists();
   void far(const double* ub, const double* ubn, double* u);
+  void get_values(double* val, double* u);
   void level_data(double* lgeo, double* balls);
   void triplets(const i64* trow, i64* si, i64* sj, double* vs, double* vd);
 };

@@ -66,6 +66,10 @@ __global__ void scan_kernel(const int* in, int n, int* out);
 __global__ void naive_apply_kernel(int nnz, int h, const int* tidx, const int* ppanel, const double* tx,
                                    const double* sx, const double* snx, const double* sw, const double* ub,
                                    const double* ubn, const double* wout, double* val);
+__global__ void naive_apply_list_kernel(const int* lpairs, const int* ltot, int first, int last, int h,
+                                        const int* tidx, const int* ppanel, const double* tx, const double* sx,
+                                        const double* snx, const double* sw, const double* ub, const double* ubn,
+                                        const double* wout, double* val);
 __global__ void target_count_kernel(int nnz, const int* tidx, int* tcount);
 __global__ void target_fill_kernel(int nnz, const int* tidx, const int* toffs, int* tcur, int* tpairs);
 __global__ void target_sum_kernel(int nt, const int* toffs, int* tpairs, const double* val, double* u);

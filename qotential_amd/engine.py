@@ -122,6 +122,12 @@ class Engine:
     def apply_resident(self):
         check(lib().qol_engine_apply_resident(self._h), "qol_engine_apply_resident")
 
+    def values(self):
+        """Per-pair corrections and the resident potential, copied back."""
+        val = np.zeros(max(self.nnz, 1)); u = np.zeros(max(self.nt, 1))
+        check(lib().qol_engine_get_values(self._h, ptr(val), ptr(u)), "qol_engine_get_values")
+        return val[:self.nnz], u[:self.nt]
+
     def far(self, ub, ubn):
         u = np.zeros(max(self.nt, 1))
         check(lib().qol_engine_far(self._h, ptr(_f(ub)), ptr(_f(ubn)), ptr(u)), "qol_engine_far")

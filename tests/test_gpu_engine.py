@@ -86,3 +86,19 @@ def test_grf_fixture_order4(eng4, geo4, fixture4, oracle):
     uo = far_o.copy()
     val_o, lab_o = oracle.close_apply(4, geo4, geo4["sx"], offs, idx, ub, ubn, exterior=True, u=uo, nthreads=8)
     assert np.max(np.abs(u - uo)) < 1e-11 * np.abs(ubn).max()
+
+
+def test_fused_corrections_match_apply(eng4, geo4):
+    """With the density resident before eval, eval makes the per-pair corrections
+    itself (level pairs right after the direct kernel, RRQ pairs after the
+    weights); they equal a plain apply bit for bit, and so does the potential."""
+    say("fused corrections vs plain apply")
+    ub, gu = W.harmonic_exp(geo4["sx"])
+    ubn = (geo4["snx"] * gu).sum(1)
+    far = eng4.far(ub, ubn)              # density now resident, u = far field
+    eng4.eval(exterior=True, side_tol=1e300)
+    eng4.apply_resident()                # corrections came from eval
+    val_f, u_f = eng4.values()
+    u_ref, val_ref = eng4.apply(ub, ubn, far)
+    assert np.array_equal(val_f, val_ref)
+    assert np.array_equal(u_f, u_ref)
