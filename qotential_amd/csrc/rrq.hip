@@ -389,8 +389,8 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
   const int h = T.h, H4 = 4 * h, H5 = 5 * h, m = 2 * h;
   constexpr int NR = 16 * RPT, NC = 16 * CPT;
   __shared__ double colq[2][4 * NR], colr[2][NR], prq[2][4 * NC], prr[2][NC];
-  __shared__ double cvq[2][16], cvr[2][16];
-  __shared__ int ciq[2][16], cir[2][16];
+  __shared__ double cvq[2][4], cvr[2][4];
+  __shared__ int ciq[2][4], cir[2][4];
   __shared__ int pivq[NR], pivr[NR];
   extern __shared__ double dyn[];
   double* Bm = dyn;                      // [h][h] B(r,c) at r*h+c
@@ -416,52 +416,61 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
         const double* q = FQ + 3 * ((size_t)r * h + c);
         e[1] = q[0]; e[2] = q[1]; e[3] = q[2];
         Ar[a][b] = MS[r + (size_t)h * c];
-      } else if (r < h && c - h == r) {
-        e[0] = 1.0;
-        Ar[a][b] = 1.0;
       }
     }
   // rows already used as pivots (same in every thread)
   unsigned long long uq[2] = {0ull, 0ull}, ur[2] = {0ull, 0ull};
+  const int wv = threadIdx.x >> 6;
   for (int kk = 0; kk < h; ++kk) {
     const int buf = kk & 1, kt = kk & 15, kb = kk >> 4;
-    // column kk: pivot candidates and multipliers
-    if (tx == kt) {
-      double bq = -1.0, br = -1.0;
-      int iq = NR, ir = NR;
+    // column kk: its owners (tx == kt, four per wave) publish the multipliers and
+    // their best unused row; the wave reduces the four candidates by shuffles
+    double bq = -1.0, br = -1.0;
+    int iq = NR, ir = NR;
+    const bool own = tx == kt;
 #pragma unroll
-      for (int b = 0; b < CPT; ++b) {
-        if (b != kb) continue;  // uniform
+    for (int b = 0; b < CPT; ++b) {
+      if (b != kb) continue;  // uniform
 #pragma unroll
-        for (int a = 0; a < RPT; ++a) {
-          const int r = ty + 16 * a;
-          if (r < h) {
-            const double* e = Aq[a][b];
-            for (int q = 0; q < 4; ++q) colq[buf][4 * r + q] = e[q];
-            colr[buf][r] = Ar[a][b];
-            const bool fq = (uq[r >> 6] >> (r & 63)) & 1ull, fr = (ur[r >> 6] >> (r & 63)) & 1ull;
-            const double vq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3], vr = fabs(Ar[a][b]);
-            const bool tq = !fq & (vq > bq), tr = !fr & (vr > br);
-            bq = tq ? vq : bq; iq = tq ? r : iq;
-            br = tr ? vr : br; ir = tr ? r : ir;
-          }
+      for (int a = 0; a < RPT; ++a) {
+        const int r = ty + 16 * a;
+        if (own & (r < h)) {
+          const double* e = Aq[a][b];
+          for (int q = 0; q < 4; ++q) colq[buf][4 * r + q] = e[q];
+          colr[buf][r] = Ar[a][b];
         }
+        const bool ok = own & (r < h);
+        const bool fq = (uq[r >> 6] >> (r & 63)) & 1ull, fr = (ur[r >> 6] >> (r & 63)) & 1ull;
+        const double* e = Aq[a][b];
+        const double vq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3], vr = fabs(Ar[a][b]);
+        const bool tq = ok & !fq & (vq > bq), tr = ok & !fr & (vr > br);
+        bq = tq ? vq : bq; iq = tq ? r : iq;
+        br = tr ? vr : br; ir = tr ? r : ir;
       }
-      cvq[buf][ty] = bq; ciq[buf][ty] = iq;
-      cvr[buf][ty] = br; cir[buf][ty] = ir;
+    }
+#pragma unroll
+    for (int off = 16; off <= 32; off <<= 1) {
+      const double oq = __shfl_xor(bq, off, 64), orr = __shfl_xor(br, off, 64);
+      const int oiq = __shfl_xor(iq, off, 64), oir = __shfl_xor(ir, off, 64);
+      const bool tq = (oq > bq) | ((oq == bq) & (oiq < iq)), tr = (orr > br) | ((orr == br) & (oir < ir));
+      bq = tq ? oq : bq; iq = tq ? oiq : iq;
+      br = tr ? orr : br; ir = tr ? oir : ir;
+    }
+    if ((threadIdx.x & 63) == kt) {
+      cvq[buf][wv] = bq; ciq[buf][wv] = iq;
+      cvr[buf][wv] = br; cir[buf][wv] = ir;
     }
     __syncthreads();
     int pq = NR, pr = NR;
     {
-      double bq = -1.0, br = -1.0;
+      double vbq = -1.0, vbr = -1.0;
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
+      for (int t = 0; t < 4; ++t) {
         const double vq = cvq[buf][t], vr = cvr[buf][t];
-        const int iq = ciq[buf][t], ir = cir[buf][t];
-        // bitwise forms: no short-circuit branches
-        const bool tq = (vq > bq) | ((vq == bq) & (iq < pq)), tr = (vr > br) | ((vr == br) & (ir < pr));
-        bq = tq ? vq : bq; pq = tq ? iq : pq;
-        br = tr ? vr : br; pr = tr ? ir : pr;
+        const int i1 = ciq[buf][t], i2 = cir[buf][t];
+        const bool tq = (vq > vbq) | ((vq == vbq) & (i1 < pq)), tr = (vr > vbr) | ((vr == vbr) & (i2 < pr));
+        vbq = tq ? vq : vbq; pq = tq ? i1 : pq;
+        vbr = tr ? vr : vbr; pr = tr ? i2 : pr;
       }
     }
     // identical in every lane: keep them in scalar registers
@@ -472,17 +481,22 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
     if (threadIdx.x == 0) { pivq[kk] = pq; pivr[kk] = pr; }
     // pivot rows to LDS (row pq lives in slot a = pq/16 of the threads with ty =
     // pq%16); all 16*CPT columns are written (the buffers have room for them)
+    // The right block is kept in pivot order: its column h+kk is the identity
+    // column of the row pivoted at step kk, materialised now (1 in the pivot row,
+    // 0 elsewhere, which is what every other row still holds there).
 #pragma unroll
     for (int a = 0; a < RPT; ++a) {
       if (a == (pq >> 4) && ty == (pq & 15)) {
 #pragma unroll
-        for (int b = 0; b < CPT; ++b)
+        for (int b = 0; b < CPT; ++b) {
+          const bool unit = tx + 16 * b == h + kk;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) prq[buf][4 * (tx + 16 * b) + q] = Aq[a][b][q];
+          for (int q = 0; q < 4; ++q) prq[buf][4 * (tx + 16 * b) + q] = unit ? (q == 0 ? 1.0 : 0.0) : Aq[a][b][q];
+        }
       }
       if (a == (pr >> 4) && ty == (pr & 15)) {
 #pragma unroll
-        for (int b = 0; b < CPT; ++b) prr[buf][tx + 16 * b] = Ar[a][b];
+        for (int b = 0; b < CPT; ++b) prr[buf][tx + 16 * b] = tx + 16 * b == h + kk ? 1.0 : Ar[a][b];
       }
     }
     __syncthreads();
@@ -495,28 +509,50 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
       iq4[0] = e[0] * in2; iq4[1] = -e[1] * in2; iq4[2] = -e[2] * in2; iq4[3] = -e[3] * in2;
     }
     const double irr = 1.0 / prr[buf][kk];
-    // branch-free update: clamped LDS indices, selects for the pivot rows and
-    // for finished / absent columns
+    // every row: A_r -= A_rk * (pivot^-1 * pivot row); the pivot row itself is
+    // overwritten below.  Live columns are the window (kk, h+kk]: finished left
+    // columns and right columns of rows not yet pivoted (still unit columns,
+    // untouched by this step) get a zero normalised row, which leaves them
+    // unchanged; chunks of 16 columns outside the window are skipped outright.
 #pragma unroll
     for (int b = 0; b < CPT; ++b) {
+      if (16 * b + 15 <= kk || 16 * b > h + kk) continue;  // uniform
       const int c = tx + 16 * b;
-      // left-block columns up to kk are finished (never read again)
-      const bool colok = c < m && !(c <= kk && c < h);
+      const bool colok = (c > kk) & (c <= h + kk) & (c < m);
       const int cc = c < m ? c : m - 1;
       double nq[4];
       qmul_fma(iq4, prq[buf] + 4 * cc, nq);
-      const double nr = irr * prr[buf][cc];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) nq[q] = colok ? nq[q] : 0.0;
+      const double nr = colok ? irr * prr[buf][cc] : 0.0;
 #pragma unroll
       for (int a = 0; a < RPT; ++a) {
         const int r = ty + 16 * a;
         const int rc = r < h ? r : h - 1;
-        const bool isq = r == pq, isr = r == pr;
-        double e[4] = {Aq[a][b][0], Aq[a][b][1], Aq[a][b][2], Aq[a][b][3]};
-        qsub_mul(e, colq[buf] + 4 * rc, nq);
+        qsub_mul(Aq[a][b], colq[buf] + 4 * rc, nq);
+        Ar[a][b] = fma(-colr[buf][rc], nr, Ar[a][b]);
+      }
+    }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) Aq[a][b][q] = colok ? (isq ? nq[q] : e[q]) : Aq[a][b][q];
-        const double er = fma(-colr[buf][rc], nr, Ar[a][b]);
-        Ar[a][b] = colok ? (isr ? nr : er) : Ar[a][b];
+    for (int a = 0; a < RPT; ++a) {
+      if (a == (pq >> 4) && ty == (pq & 15)) {
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+          const int c = tx + 16 * b, cc = c < m ? c : m - 1;
+          double nq[4];
+          qmul_fma(iq4, prq[buf] + 4 * cc, nq);
+          // columns right of the window stay zero (unit columns of later pivots)
+          const bool keep = c <= h + kk;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) Aq[a][b][q] = keep ? nq[q] : 0.0;
+        }
+      }
+      if (a == (pr >> 4) && ty == (pr & 15)) {
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+          const int c = tx + 16 * b, cc = c < m ? c : m - 1;
+          Ar[a][b] = c <= h + kk ? irr * prr[buf][cc] : 0.0;
+        }
       }
     }
   }
@@ -531,10 +567,11 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
     for (int b = 0; b < CPT; ++b) {
       const int r = ty + 16 * a, c = tx + 16 * b;
       if (r < h && c >= h && c < m) {
-        const int i = c - h;
+        // row piv[k] holds row k of the inverse, column h+j' holds its column piv[j']
+        const int jq = pivq[c - h], jr = pivr[c - h];
         const int kq = posq[r], kr = posr[r];
-        for (int q = 0; q < 4; ++q) G1[(size_t)i * H4 + q * h + kq] = Aq[a][b][q];
-        Bm[(size_t)kr * h + i] = Ar[a][b];
+        for (int q = 0; q < 4; ++q) G1[(size_t)jq * H4 + q * h + kq] = Aq[a][b][q];
+        Bm[(size_t)kr * h + jr] = Ar[a][b];
       }
     }
   __syncthreads();
