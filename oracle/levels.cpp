@@ -38,7 +38,7 @@ void build_patch_levels(i64 korder, i64 npols, const double* sx, const double* r
     for (i64 i = 0; i < n; ++i) {
       for (int c = 0; c < 9; ++c) {
         double s = 0.0;
-        for (i64 k = 0; k < npols; ++k) s += RL.interpmat[i + n * k] * srcT[k + npols * c];
+        for (i64 k = 0; k < npols; ++k) s = std::fma(RL.interpmat[i + n * k], srcT[k + npols * c], s);
         geo[c] = s;
       }
       L.sx[3 * i] = geo[0]; L.sx[3 * i + 1] = geo[1]; L.sx[3 * i + 2] = geo[2];

@@ -10,18 +10,23 @@ namespace qol {
 
 // val(pair) = sum_i (S - S_naive)_i ubn_i - (K - K_naive)_i ub_i, i over the
 // panel's h nodes, left to right as in the reference loop.
-// near-correction value of pair j: sum_i (S - S_naive)_i ubn_i - (K - K_naive)_i ub_i
-__device__ __forceinline__ double pair_correction(int j, int h, const int* __restrict__ tidx,
-                                                  const int* __restrict__ ppanel, const double* __restrict__ tx,
-                                                  const double* __restrict__ sx, const double* __restrict__ snx,
-                                                  const double* __restrict__ sw, const double* __restrict__ ub,
-                                                  const double* __restrict__ ubn, const double* __restrict__ wout) {
+// near-correction value of pair j, part p of 4 (nodes p, p+4, ...):
+// sum_i (S - S_naive)_i ubn_i - (K - K_naive)_i ub_i
+__device__ __forceinline__ double pair_correction_part(int j, int part, int h, const int* __restrict__ tidx,
+                                                       const int* __restrict__ ppanel,
+                                                       const double* __restrict__ tx,
+                                                       const double* __restrict__ sx,
+                                                       const double* __restrict__ snx,
+                                                       const double* __restrict__ sw,
+                                                       const double* __restrict__ ub,
+                                                       const double* __restrict__ ubn,
+                                                       const double* __restrict__ wout) {
   const double pi4inv = 1.0 / (16.0 * atan(1.0));
   const int t = tidx[j], k = ppanel[j];
   const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
   const double* w = wout + (size_t)j * 2 * h;
   double acc = 0.0;
-  for (int i = 0; i < h; ++i) {
+  for (int i = part; i < h; i += 4) {
     const size_t g = (size_t)k * h + i;
     const double dx = x0 - sx[3 * g], dy = x1 - sx[3 * g + 1], dz = x2 - sx[3 * g + 2];
     const double rr = dx * dx + dy * dy + dz * dz;
@@ -42,13 +47,19 @@ __global__ void __launch_bounds__(256) naive_apply_kernel(int nnz, int h, const 
                                                           const double* __restrict__ sw, const double* __restrict__ ub,
                                                           const double* __restrict__ ubn, const double* __restrict__ wout,
                                                           double* __restrict__ val) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nnz) return;
-  val[j] = pair_correction(j, h, tidx, ppanel, tx, sx, snx, sw, ub, ubn, wout);
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = gt >> 2, part = gt & 3;
+  const bool live = j < nnz;
+  const double acc0 = live ? pair_correction_part(j, part, h, tidx, ppanel, tx, sx, snx, sw, ub, ubn, wout) : 0.0;
+  double acc = acc0 + __shfl_xor(acc0, 1, 64);
+  acc = acc + __shfl_xor(acc, 2, 64);
+  if (live && part == 0) val[j] = acc;
 }
 
-// the same for the pairs of the per-label lists first..last (lists concatenated
-// in label order, lengths ltot), so each part can run as soon as its weights are in
+// The same for the pairs of the per-label lists first..last (lists concatenated
+// in label order, lengths ltot), so each part can run as soon as its weights are
+// in.  Four lanes per pair, each taking every fourth node, so a wave reads 16
+// weight rows contiguously; the four partial sums are combined in a fixed order.
 __global__ void __launch_bounds__(256) naive_apply_list_kernel(const int* __restrict__ lpairs,
                                                                const int* __restrict__ ltot, int first, int last,
                                                                int h, const int* __restrict__ tidx,
@@ -66,10 +77,15 @@ __global__ void __launch_bounds__(256) naive_apply_list_kernel(const int* __rest
     if (q < first) base += ltot[q];
     else cnt += ltot[q];
   }
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
-  const int j = lpairs[base + t];
-  val[j] = pair_correction(j, h, tidx, ppanel, tx, sx, snx, sw, ub, ubn, wout);
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = gt >> 2, part = gt & 3;
+  const bool live = t < cnt;
+  const int j = live ? lpairs[base + t] : 0;
+  double acc = live ? pair_correction_part(j, part, h, tidx, ppanel, tx, sx, snx, sw, ub, ubn, wout) : 0.0;
+  // (p0 + p1) + (p2 + p3) in every lane of the group
+  acc = acc + __shfl_xor(acc, 1, 64);
+  acc = acc + __shfl_xor(acc, 2, 64);
+  if (live && part == 0) val[j] = acc;
 }
 
 // COO triplets of getnearquad_lap_rrq: out = (tcxi(k)-1)*h + (j-1)*m + i (1-based)

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -125,9 +126,16 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
   if (p < 2 || p > QOL_MAXP) throw std::runtime_error("patch order out of range for the device kernels");
   if (npan <= 0) throw std::runtime_error("npan must be positive");
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+  // QOL_ONE_STREAM=1 runs every stage on the main stream (kernel times without
+  // overlap, for profiling); by default the stages overlap on four streams
+  const char* one = std::getenv("QOL_ONE_STREAM");
+  if (one && one[0] == '1') {
+    s1 = s2 = s3 = stream;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+  }
   for (auto& st : tev)
     for (auto& e : st) HIPCHK(hipEventCreate(&e));
   for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_solve, &ev_edge})
@@ -159,9 +167,9 @@ Engine::~Engine() {
   for (auto& st : tev)
     for (auto& e : st) (void)hipEventDestroy(e);
   for (hipEvent_t e : {ev_start, ev_lists, ev_direct, ev_setup, ev_solve, ev_edge}) (void)hipEventDestroy(e);
-  if (s1) (void)hipStreamDestroy(s1);
-  if (s2) (void)hipStreamDestroy(s2);
-  if (s3) (void)hipStreamDestroy(s3);
+  if (s1 && s1 != stream) (void)hipStreamDestroy(s1);
+  if (s2 && s2 != stream) (void)hipStreamDestroy(s2);
+  if (s3 && s3 != stream) (void)hipStreamDestroy(s3);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -287,11 +295,12 @@ void Engine::alloc_rrq() {
   d_Msl.alloc(sizeof(double) * (size_t)h * h * np_);
   d_Fn.alloc(sizeof(double) * (size_t)h * h * np_);
   d_G.alloc(sizeof(double) * 10 * (size_t)h * h * np_);
+  d_Binv.alloc(sizeof(double) * (size_t)h * h * np_);
   if (!rrq_solve_g1_in_lds(h)) d_scr.alloc(sizeof(double) * 4 * (size_t)h * h * np_);
   RP.frame = (double*)d_frame.ptr; RP.edge = (double*)d_edge.ptr; RP.bX = (double*)d_bX.ptr;
   RP.cx = (double*)d_cx.ptr; RP.basis = (double*)d_basis.ptr; RP.Fq = (double*)d_Fq.ptr;
   RP.Msl = (double*)d_Msl.ptr; RP.Fn = (double*)d_Fn.ptr;
-  RP.G = (double*)d_G.ptr; RP.scr = rrq_solve_g1_in_lds(h) ? nullptr : (double*)d_scr.ptr;
+  RP.G = (double*)d_G.ptr; RP.Binv = (double*)d_Binv.ptr; RP.scr = rrq_solve_g1_in_lds(h) ? nullptr : (double*)d_scr.ptr;
   rrq_alloc = true;
 }
 
@@ -498,7 +507,7 @@ void Engine::build_target_lists() {
 
 // corrections of the pairs in per-label lists first..last (after their weights)
 void Engine::launch_correction(int first, int last, hipStream_t s) {
-  hipLaunchKernelGGL(naive_apply_list_kernel, dim3(blocks((size_t)nnz, 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(naive_apply_list_kernel, dim3(blocks(4 * (size_t)nnz, 256)), dim3(256), 0, s,
                      (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, first, last, h, (const int*)d_tidx.ptr,
                      (const int*)d_ppanel.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr,
                      (const double*)d_snx.ptr, (const double*)d_sw.ptr, (const double*)d_ub.ptr,
@@ -510,7 +519,7 @@ void Engine::launch_correction(int first, int last, hipStream_t s) {
 void Engine::apply_device() {
   if (nnz <= 0) return;
   if (!val_current)
-    hipLaunchKernelGGL(naive_apply_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz, h,
+    hipLaunchKernelGGL(naive_apply_kernel, dim3(blocks(4 * (size_t)nnz, 256)), dim3(256), 0, stream, (int)nnz, h,
                        (const int*)d_tidx.ptr, (const int*)d_ppanel.ptr, (const double*)d_tx.ptr,
                        (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,
                        (const double*)d_ub.ptr, (const double*)d_ubn.ptr, (const double*)d_wout.ptr,

@@ -12,9 +12,10 @@
 namespace qol {
 
 // kLP panels per workgroup, so each interpolation-matrix entry is loaded once
-// for all of them.  Level nodes are interpolated in the reference's summation
-// order (sequential over the h panel nodes, no FMA) so that the ball centres and
-// radii -- and hence every classification -- match the host oracle bit for bit.
+// for all of them.  Level nodes are interpolated as sequential fused multiply-adds
+// over the h panel nodes, the same operations in the same order as the host
+// oracle, so the ball centres and radii -- and hence every classification --
+// match it bit for bit.
 // Output: lgeo[panel][7][ntot] (x,y,z,nx,ny,nz,w), balls[panel][nsubtot][4].
 constexpr int kLP = 4;
 
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
       for (int pp = 0; pp < kLP; ++pp)
 #pragma unroll
         for (int c = 0; c < 9; ++c) geo[pp][c] = 0.0;
-      // sequential in q for every accumulator (the oracle's order); the next
+      // sequential fma in q for every accumulator (the oracle's order); the next
       // interpolation entry is fetched one step ahead and the 36 LDS values of
       // a step are read together
       double vnext = IM[i];
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
 #pragma unroll
         for (int pp = 0; pp < kLP; ++pp)
 #pragma unroll
-          for (int c = 0; c < 9; ++c) geo[pp][c] = geo[pp][c] + v * sv[pp * 9 + c];
+          for (int c = 0; c < 9; ++c) geo[pp][c] = fma(v, sv[pp * 9 + c], geo[pp][c]);
       }
 #pragma unroll
       for (int pp = 0; pp < kLP; ++pp) {

@@ -40,7 +40,8 @@ struct RRQPanels {
   double* Fq;      // [npan][h][h][3] pure quaternions grad F_k(y_i)
   double* Msl;     // [npan][h][h] Msl^T (column-major)
   double* Fn;      // [npan][h][h] F_k at the nodes, F(i,k)
-  double* G;       // [npan][2h][5h] weights operator
+  double* G;       // [npan][5h][2h] weights operator (transposed)
+  double* Binv;    // [npan][h][h] (Msl^T)^{-1}, row-major
   double* scr;     // [npan][4h^2] G1 when it does not fit in LDS, else null
 };
 

@@ -8,6 +8,7 @@
 //            (SSQ weights, solid angle, omega moments: absent LQ/QA modules,
 //             re-derived -- see the oracle header for the identities used)
 // FP64 throughout.
+#include <cstdlib>
 #include <stdexcept>
 
 #include "qol_kernels.h"
@@ -381,6 +382,36 @@ size_t rrq_solve_smem(int h) {
   return sizeof(double) * (size_t)((rrq_solve_g1_in_lds(h) ? 4 * h * h : 0) + 2 * h * h);
 }
 
+// Shared tail of the solve kernels: C = B Fn^T, then G (stored transposed,
+// G(row, c) at c*2h + row): rows j<h  As = (C rhs + B W) * fa, rows h..2h-1
+// Ad = G1 rhs * fac.  Bt = B^T (Bt[r*h + j] = B(j, r)) and Ct = C^T in LDS, G1 in
+// LDS or scratch.  Threads run over consecutive j, so the G stores coalesce and
+// the LDS reads of Bt / Ct are conflict-free (G1 and Fn values are broadcasts).
+__device__ void solve_tail(int h, const double* Bt, double* Ct, const double* G1, const double* __restrict__ FN,
+                           double* __restrict__ G, double fa, double fac) {
+  const int H4 = 4 * h, H5 = 5 * h, m = 2 * h;
+  // Ct[i*h + j] = C(j, i) = sum_r B(j, r) Fn(i, r)
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+    const int j = e % h, i = e / h;
+    double s = 0.0;
+    for (int r = 0; r < h; ++r) s = fma(Bt[(size_t)r * h + j], FN[i + (size_t)h * r], s);
+    Ct[(size_t)i * h + j] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
+    const int j = e % h, c = e / h;
+    double* Gc = G + (size_t)c * m;
+    Gc[h + j] = c < H4 ? G1[(size_t)j * H4 + c] * fac : 0.0;
+    double s = 0.0;
+    if (c < H4) {
+      for (int i = 0; i < h; ++i) s = fma(Ct[(size_t)i * h + j], G1[(size_t)i * H4 + c], s);
+    } else {
+      s = Bt[(size_t)(c - H4) * h + j];
+    }
+    Gc[j] = s * fa;
+  }
+}
+
 template <int RPT, int CPT>
 __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPanels P, int npan,
                                                               const int* __restrict__ nrrq) {
@@ -571,32 +602,305 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
         const int jq = pivq[c - h], jr = pivr[c - h];
         const int kq = posq[r], kr = posr[r];
         for (int q = 0; q < 4; ++q) G1[(size_t)jq * H4 + q * h + kq] = Aq[a][b][q];
-        Bm[(size_t)kr * h + jr] = Ar[a][b];
+        Bm[(size_t)jr * h + kr] = Ar[a][b];  // transposed: Bt[r*h + j] = B(j, r)
       }
     }
   __syncthreads();
-  // C = B Fn^T
-  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
-    const int j = e / h, i = e % h;
-    double s = 0.0;
-    for (int r = 0; r < h; ++r) s = fma(Bm[(size_t)j * h + r], FN[i + (size_t)h * r], s);
-    Cm[(size_t)j * h + i] = s;
+  solve_tail(h, Bm, Cm, G1, FN, G, fac / alpha, fac);
+}
+
+// Entry (a, b) of the left-multiplication matrix of quaternion q (q*x = L(q) x):
+// L[a][b] = +/- q[a ^ b], negative at (0,1) (0,2) (0,3) (1,2) (2,3) (3,1).
+__device__ __forceinline__ double lmat(const double* q, int a, int b) {
+  const unsigned neg = 0x0Eu | (0x4u << 4) | (0x8u << 8) | (0x2u << 12);
+  const double v = q[a ^ b];
+  return ((neg >> (a * 4 + b)) & 1u) ? -v : v;
+}
+
+__device__ __forceinline__ double qd4_get(const qd4& d, int i) {
+  return i == 0 ? d[0] : (i == 1 ? d[1] : (i == 2 ? d[2] : d[3]));
+}
+
+// The same weights operator with the quaternion elimination on the FP64 matrix
+// cores.  In real form the quaternion rank-1 update A_rc -= m_r n_c is
+// A(4h x 2h) -= L(m)(4h x 4) N(4 x 2h): one v_mfma_f64_16x16x4 per 16x16 tile.
+// A is held as accumulator tiles (row 4r + component, column c): wave w owns the
+// row tiles w, w+4, ...; lane l holds rows 4i + l/16 (i = 0..3, i.e. quaternion
+// 4*tile + i, component l/16) of column l%16.  The real Msl^T elimination stays
+// on the 16 x 16 thread grid as in rrq_panel_solve_kernel; both share the step
+// loop (implicit partial pivoting, pivot-ordered right block, column window).
+template <int RT, int CT>
+__global__ void __launch_bounds__(256) rrq_panel_solve_mfma_kernel(RRQTables T, RRQPanels P, int npan,
+                                                                   const int* __restrict__ nrrq) {
+  const int k = blockIdx.x;
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
+  const int h = T.h, H4 = 4 * h, m = 2 * h;
+  constexpr int RTW = (RT + 3) / 4, NQR = 4 * RT, NC = 16 * CT;
+  __shared__ double colq[2][4 * NQR], prq[2][4 * NC];
+  __shared__ double cvq[2][4];
+  __shared__ int ciq[2][4];
+  __shared__ int pivq[NQR], posq[NQR];
+  extern __shared__ double dyn[];
+  double* Bm = dyn;
+  double* Cm = Bm + (size_t)h * h;
+  double* G1 = P.scr ? P.scr + (size_t)k * 4 * h * h : Cm + (size_t)h * h;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const double* FQ = P.Fq + (size_t)k * 3 * h * h;
+  const double* FN = P.Fn + (size_t)k * h * h;
+  double* G = P.G + (size_t)k * 2 * h * 5 * h;
+  const double alpha = P.frame[(size_t)k * 16 + 12];
+  const double fac = 1.0 / (16.0 * atan(1.0));
+  qd4 D[RTW][CT];
+#pragma unroll
+  for (int t = 0; t < RTW; ++t)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rt = wv + 4 * t, r = 4 * rt + i, c = 16 * ct + li;
+        D[t][ct][i] = (rt < RT && r < h && c < h && lk > 0) ? FQ[3 * ((size_t)r * h + c) + lk - 1] : 0.0;
+      }
+  // B = (Msl^T)^{-1} from its own kernel, to LDS transposed (Bt[r*h + j] = B(j, r))
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) Bm[(size_t)(e % h) * h + e / h] = P.Binv[(size_t)k * h * h + e];
+  unsigned long long uq[2] = {0ull, 0ull};
+  for (int kk = 0; kk < h; ++kk) {
+    const int buf = kk & 1, kt = kk & 15, kb = kk >> 4;
+    // ---- column kk: multipliers and pivot candidates
+    {
+      double bq = -1.0;
+      int iq = NQR;
+#pragma unroll
+      for (int t = 0; t < RTW; ++t) {
+        const int rt = wv + 4 * t;
+        if (rt >= RT) continue;  // uniform
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          if (ct != kb) continue;  // uniform
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const double v = D[t][ct][i];
+            double s2 = v * v;
+            s2 += __shfl_xor(s2, 16, 64);
+            s2 += __shfl_xor(s2, 32, 64);
+            const int r = 4 * rt + i;
+            const bool here = (li == kt) & (r < h);
+            if (here) colq[buf][4 * r + lk] = v;
+            const bool used = (uq[r >> 6] >> (r & 63)) & 1ull;
+            const bool tq = here & !used & (s2 > bq);
+            bq = tq ? s2 : bq;
+            iq = tq ? r : iq;
+          }
+        }
+      }
+      if (lane == kt) { cvq[buf][wv] = bq; ciq[buf][wv] = iq; }
+    }
+    __syncthreads();
+    int pq = NQR;
+    {
+      double vbq = -1.0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double vq = cvq[buf][t];
+        const int i1 = ciq[buf][t];
+        const bool tq = (vq > vbq) | ((vq == vbq) & (i1 < pq));
+        vbq = tq ? vq : vbq; pq = tq ? i1 : pq;
+      }
+    }
+    pq = __builtin_amdgcn_readfirstlane(pq);
+    uq[pq >> 6] |= 1ull << (pq & 63);
+    if (threadIdx.x == 0) pivq[kk] = pq;
+    const int rtp = pq >> 2, ip = pq & 3, wp = rtp & 3, tp = rtp >> 2;
+    // pivot row to LDS; the right block is in pivot order, its column h+kk is the
+    // unit column of the row pivoted now (materialised here)
+    if (wv == wp) {
+#pragma unroll
+      for (int t = 0; t < RTW; ++t) {
+        if (t != tp) continue;  // uniform
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const int c = 16 * ct + li;
+          const double v = qd4_get(D[t][ct], ip);
+          prq[buf][4 * c + lk] = c == h + kk ? (lk == 0 ? 1.0 : 0.0) : v;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- update
+    double iq4[4];
+    {
+      const double* e = prq[buf] + 4 * kk;
+      const double n2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
+      const double in2 = 1.0 / n2;
+      iq4[0] = e[0] * in2; iq4[1] = -e[1] * in2; iq4[2] = -e[2] * in2; iq4[3] = -e[3] * in2;
+    }
+    // B operand: component lk of the normalised pivot row at column 16ct + li
+    // (zero outside the live window (kk, h+kk])
+    double Bop[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const int c = 16 * ct + li;
+      const int cc = c < m ? c : m - 1;
+      const double* x = prq[buf] + 4 * cc;
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v = fma(lmat(iq4, lk, j), x[j], v);
+      const bool colok = (c > kk) & (c <= h + kk) & (c < m);
+      Bop[ct] = colok ? v : 0.0;
+    }
+    // A operand: -L(m_r)[comp][lk] for row 4r + comp = 16 rt + li; the pivot row
+    // gets zero (it is overwritten below)
+#pragma unroll
+    for (int t = 0; t < RTW; ++t) {
+      const int rt = wv + 4 * t;
+      if (rt >= RT) continue;  // uniform
+      const int r = 4 * rt + (li >> 2), comp = li & 3;
+      const double mv = lmat(colq[buf] + 4 * r, comp, lk);
+      const double aop = ((r < h) & (r != pq)) ? -mv : 0.0;
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        if (16 * ct + 15 <= kk || 16 * ct > h + kk) continue;  // uniform: tile outside the window
+        D[t][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, Bop[ct], D[t][ct], 0, 0, 0);
+      }
+    }
+    if (wv == wp) {
+#pragma unroll
+      for (int t = 0; t < RTW; ++t) {
+        if (t != tp) continue;  // uniform
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) D[t][ct][i] = i == ip ? Bop[ct] : D[t][ct][i];
+      }
+    }
   }
   __syncthreads();
-  const double fa = fac / alpha;
-  for (int e = threadIdx.x; e < h * H5; e += blockDim.x) {
-    const int j = e / H5, c = e % H5;
-    // Ad rows (G stored transposed, G(row, c) at c*2h + row)
-    G[(size_t)c * m + h + j] = c < H4 ? G1[(size_t)j * H4 + c] * fac : 0.0;
-    // As rows
-    double s = 0.0;
-    if (c < H4) {
-      for (int i = 0; i < h; ++i) s = fma(Cm[(size_t)j * h + i], G1[(size_t)i * H4 + c], s);
-    } else {
-      s = Bm[(size_t)j * h + (c - H4)];
-    }
-    G[(size_t)c * m + j] = s * fa;
+  for (int kk = threadIdx.x; kk < h; kk += blockDim.x) posq[pivq[kk]] = kk;
+  __syncthreads();
+  // row piv[k] holds row k of the inverse; column h+j' holds its column piv[j']
+#pragma unroll
+  for (int t = 0; t < RTW; ++t) {
+    const int rt = wv + 4 * t;
+    if (rt >= RT) continue;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 4 * rt + i, c = 16 * ct + li;
+        if (r < h && c >= h && c < m) G1[(size_t)pivq[c - h] * H4 + lk * h + posq[r]] = D[t][ct][i];
+      }
   }
+  __syncthreads();
+  solve_tail(h, Bm, Cm, G1, FN, G, fac / alpha, fac);
+}
+
+// B = (Msl^T)^{-1} (row-major, B(r,c) at r*h + c) by the same windowed
+// Gauss-Jordan on the 16 x 16 thread grid; small enough to run many panels per CU.
+template <int RPT, int CPT>
+__global__ void __launch_bounds__(256) rrq_msl_inverse_kernel(RRQTables T, RRQPanels P, int npan,
+                                                              const int* __restrict__ nrrq) {
+  const int k = blockIdx.x;
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
+  const int h = T.h, m = 2 * h;
+  constexpr int NR = 16 * RPT, NC = 16 * CPT;
+  __shared__ double colr[2][NR], prr[2][NC], cvr[2][4];
+  __shared__ int cir[2][4], pivr[NR], posr[NR];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const double* MS = P.Msl + (size_t)k * h * h;
+  double Ar[RPT][CPT];
+#pragma unroll
+  for (int a = 0; a < RPT; ++a)
+#pragma unroll
+    for (int b = 0; b < CPT; ++b) {
+      const int r = ty + 16 * a, c = tx + 16 * b;
+      Ar[a][b] = (r < h && c < h) ? MS[r + (size_t)h * c] : 0.0;
+    }
+  unsigned long long ur[2] = {0ull, 0ull};
+  for (int kk = 0; kk < h; ++kk) {
+    const int buf = kk & 1, kt = kk & 15, kb = kk >> 4;
+    double br = -1.0;
+    int ir = NR;
+    const bool own = tx == kt;
+#pragma unroll
+    for (int b = 0; b < CPT; ++b) {
+      if (b != kb) continue;  // uniform
+#pragma unroll
+      for (int a = 0; a < RPT; ++a) {
+        const int r = ty + 16 * a;
+        if (own & (r < h)) colr[buf][r] = Ar[a][b];
+        const bool fr = (ur[r >> 6] >> (r & 63)) & 1ull;
+        const double vr = fabs(Ar[a][b]);
+        const bool tr = own & (r < h) & !fr & (vr > br);
+        br = tr ? vr : br;
+        ir = tr ? r : ir;
+      }
+    }
+#pragma unroll
+    for (int off = 16; off <= 32; off <<= 1) {
+      const double orr = __shfl_xor(br, off, 64);
+      const int oir = __shfl_xor(ir, off, 64);
+      const bool tr = (orr > br) | ((orr == br) & (oir < ir));
+      br = tr ? orr : br;
+      ir = tr ? oir : ir;
+    }
+    if (lane == kt) { cvr[buf][wv] = br; cir[buf][wv] = ir; }
+    __syncthreads();
+    int pr = NR;
+    {
+      double vbr = -1.0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double vr = cvr[buf][t];
+        const int i2 = cir[buf][t];
+        const bool tr = (vr > vbr) | ((vr == vbr) & (i2 < pr));
+        vbr = tr ? vr : vbr; pr = tr ? i2 : pr;
+      }
+    }
+    pr = __builtin_amdgcn_readfirstlane(pr);
+    ur[pr >> 6] |= 1ull << (pr & 63);
+    if (threadIdx.x == 0) pivr[kk] = pr;
+#pragma unroll
+    for (int a = 0; a < RPT; ++a)
+      if (a == (pr >> 4) && ty == (pr & 15)) {
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) prr[buf][tx + 16 * b] = tx + 16 * b == h + kk ? 1.0 : Ar[a][b];
+      }
+    __syncthreads();
+    const double irr = 1.0 / prr[buf][kk];
+#pragma unroll
+    for (int b = 0; b < CPT; ++b) {
+      if (16 * b + 15 <= kk || 16 * b > h + kk) continue;  // uniform
+      const int c = tx + 16 * b;
+      const bool colok = (c > kk) & (c <= h + kk) & (c < m);
+      const int cc = c < m ? c : m - 1;
+      const double nr = colok ? irr * prr[buf][cc] : 0.0;
+#pragma unroll
+      for (int a = 0; a < RPT; ++a) {
+        const int r = ty + 16 * a;
+        const int rc = r < h ? r : h - 1;
+        Ar[a][b] = fma(-colr[buf][rc], nr, Ar[a][b]);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < RPT; ++a)
+      if (a == (pr >> 4) && ty == (pr & 15)) {
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+          const int c = tx + 16 * b, cc = c < m ? c : m - 1;
+          Ar[a][b] = c <= h + kk ? irr * prr[buf][cc] : 0.0;
+        }
+      }
+  }
+  __syncthreads();
+  for (int kk = threadIdx.x; kk < h; kk += blockDim.x) posr[pivr[kk]] = kk;
+  __syncthreads();
+  double* Bk = P.Binv + (size_t)k * h * h;
+#pragma unroll
+  for (int a = 0; a < RPT; ++a)
+#pragma unroll
+    for (int b = 0; b < CPT; ++b) {
+      const int r = ty + 16 * a, c = tx + 16 * b;
+      if (r < h && c >= h && c < m) Bk[(size_t)posr[r] * h + pivr[c - h]] = Ar[a][b];
+    }
 }
 
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s) {
@@ -604,6 +908,26 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
   const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
   const size_t lds = rrq_solve_smem(h);
   if (!rrq_solve_g1_in_lds(h) && !P.scr) throw std::runtime_error("RRQ solve: scratch for G1 not allocated");
+  // matrix-core elimination for patch orders up to 8 (A fits as 4-wave accumulator
+  // tiles); QOL_SOLVE_VALU=1 selects the vector-unit elimination instead (profiling)
+  const char* valu = std::getenv("QOL_SOLVE_VALU");
+  if (!(valu && valu[0] == '1')) switch (T.p) {
+#define QOL_SOLVE_MFMA(P_, RT_, CT_, RPT_)                                                                  \
+  case P_:                                                                                                \
+    hipLaunchKernelGGL((rrq_msl_inverse_kernel<RPT_, CT_>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);  \
+    hipLaunchKernelGGL((rrq_panel_solve_mfma_kernel<RT_, CT_>), dim3(npan), dim3(256), lds, s, T, P, npan,   \
+                       nrrq);                                                                             \
+    return;
+    QOL_SOLVE_MFMA(2, 1, 1, 1)
+    QOL_SOLVE_MFMA(3, 2, 1, 1)
+    QOL_SOLVE_MFMA(4, 3, 2, 1)
+    QOL_SOLVE_MFMA(5, 4, 2, 1)
+    QOL_SOLVE_MFMA(6, 6, 3, 2)
+    QOL_SOLVE_MFMA(7, 7, 4, 2)
+    QOL_SOLVE_MFMA(8, 9, 5, 3)
+    default: break;
+  }
+#undef QOL_SOLVE_MFMA
   if (rpt <= 3 && cpt <= 5) {
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void*)rrq_panel_solve_kernel<3, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
