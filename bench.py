@@ -55,7 +55,7 @@ def kernel_flops(p, npan, counts, nrrq, nrrq_panels):
     nb = 3 * nq
     ntot = int(sum(info["nl"]))
     return {
-        # 9 products + 9 sums per (level node, panel node), sequential (no FMA)
+        # 9 sequential FMAs per (level node, panel node)
         "levels": ("level_geom_kernel", 18.0 * h * ntot * npan),
         # per level node: 19 for the S/D entry (incl. one rsqrt), 4h for the two contractions
         "direct": ("direct_mfma_kernel", float(sum(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h)
@@ -155,14 +155,14 @@ def main():
     print(f"[bench] {a.steps} steps: {ms_step:.2f} ms/step, {value:.4g} pairs/s", flush=True)
 
     if a.calib_copy_mib > 0:
-        # plain copy of known size for the HBM-traffic counter calibration (not timed)
-        import torch
-        n = a.calib_copy_mib * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"[bench] calibration copy: {a.calib_copy_mib} MiB read, {a.calib_copy_mib} MiB written", flush=True)
+        # plain copy kernel of known size for the HBM-traffic counter calibration (not timed)
+        from qotential_amd._lib import lib, check
+        import ctypes as C
+        cms = C.c_double(0.0)
+        check(lib().qol_hbm_copy(C.c_int64(a.calib_copy_mib << 20), C.byref(cms)), "qol_hbm_copy")
+        gbs = 2 * (a.calib_copy_mib << 20) / (cms.value * 1e-3) / 1e9
+        print(f"[bench] calibration copy: {a.calib_copy_mib} MiB read, {a.calib_copy_mib} MiB written "
+              f"({cms.value:.3f} ms, {gbs:.0f} GB/s)", flush=True)
 
     # GRF identity on every target
     u, _ = eng.apply(ub, ubn, far)

@@ -145,6 +145,12 @@ int qol_engine_triplets(void* engine, const int64_t* trow, int64_t* si, int64_t*
                         double* vd);
 int qol_engine_stream(void* engine, void** stream);
 
+/* No reference counterpart: one plain device-to-device copy of nbytes (a
+ * multiple of 16) through hbm_copy_kernel, on buffers it allocates and frees,
+ * synchronous; *ms = its event time.  A workload of known HBM traffic
+ * (nbytes read + nbytes written) that scales the byte counters of a profiler. */
+int qol_hbm_copy(int64_t nbytes, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,4 +195,13 @@ __global__ void __launch_bounds__(256) far_direct_kernel(int nt, const double* _
   if (t < nt) u[t] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// plain copy of n 16-byte words (each read once, written once): the reference
+// workload for scaling the HBM byte counters
+__global__ void __launch_bounds__(256) hbm_copy_kernel(size_t n, const double2* __restrict__ src,
+                                                       double2* __restrict__ dst) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace qol

@@ -13,6 +13,7 @@
 // (qol_engine_*) keeps geometry, targets and weights resident across calls.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -151,6 +152,32 @@ int qol_engine_triplets(void* h, const int64_t* trow, int64_t* si, int64_t* sj, 
 
 int qol_engine_stream(void* h, void** stream) {
   return guarded([&] { *stream = (void*)E(h)->stream; });
+}
+
+int qol_hbm_copy(int64_t nbytes, double* ms) {
+  return guarded([&] {
+    if (nbytes <= 0 || nbytes % 16) throw std::runtime_error("qol_hbm_copy: nbytes must be a positive multiple of 16");
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0, e1;
+    HIPCHK(hipMalloc(&src, (size_t)nbytes));
+    HIPCHK(hipMalloc(&dst, (size_t)nbytes));
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const size_t n = (size_t)nbytes / 16;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    HIPCHK(hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(hbm_copy_kernel, dim3(grid), dim3(256), 0, 0, n, (const double2*)src, (double2*)dst);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    HIPCHK(hipFree(src));
+    HIPCHK(hipFree(dst));
+  });
 }
 
 int qol_order_info(int64_t p, int64_t* info) {

@@ -26,13 +26,6 @@ namespace qol {
 void set_status(int64_t s);
 void set_error(const std::string& s);
 
-#define HIPCHK(x)                                                                              \
-  do {                                                                                         \
-    hipError_t e_ = (x);                                                                       \
-    if (e_ != hipSuccess)                                                                      \
-      throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
-                               std::to_string(__LINE__) + ")");                                \
-  } while (0)
 
 DevBuf::~DevBuf() {
   if (ptr) (void)hipFree(ptr);

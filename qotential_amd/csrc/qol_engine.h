@@ -3,9 +3,19 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <stdexcept>
+#include <string>
 
 #include "qol_host.h"
 #include "qol_kernels.h"
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess)                                                                      \
+      throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
+                               std::to_string(__LINE__) + ")");                                \
+  } while (0)
 
 namespace qol {
 

@@ -74,6 +74,7 @@ __global__ void naive_apply_list_kernel(const int* lpairs, const int* ltot, int 
 __global__ void target_count_kernel(int nnz, const int* tidx, int* tcount);
 __global__ void target_fill_kernel(int nnz, const int* tidx, const int* toffs, int* tcur, int* tpairs);
 __global__ void target_sum_kernel(int nt, const int* toffs, int* tpairs, const double* val, double* u);
+__global__ void hbm_copy_kernel(size_t n, const double2* src, double2* dst);
 __global__ void far_direct_kernel(int nt, const double* tx, int ns, const double* sx, const double* snx,
                                   const double* sw, const double* ub, const double* ubn, double* u);
 __global__ void triplet_kernel(int nnz, int h, const int* tidx, const int* ppanel, const int* trow,

@@ -908,10 +908,12 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
   const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
   const size_t lds = rrq_solve_smem(h);
   if (!rrq_solve_g1_in_lds(h) && !P.scr) throw std::runtime_error("RRQ solve: scratch for G1 not allocated");
-  // matrix-core elimination for patch orders up to 8 (A fits as 4-wave accumulator
-  // tiles); QOL_SOLVE_VALU=1 selects the vector-unit elimination instead (profiling)
-  const char* valu = std::getenv("QOL_SOLVE_VALU");
-  if (!(valu && valu[0] == '1')) switch (T.p) {
+  // the fused vector-unit elimination by default; QOL_SOLVE_MFMA=1 selects, for
+  // patch orders up to 8, the matrix-core quaternion elimination (A as 4-wave
+  // accumulator tiles, B from a separate real inverse), which is slower on the
+  // order-8 workload (5.3 vs 4.9 ms for 14,250 panels)
+  const char* mf = std::getenv("QOL_SOLVE_MFMA");
+  if (mf && mf[0] == '1') switch (T.p) {
 #define QOL_SOLVE_MFMA(P_, RT_, CT_, RPT_)                                                                  \
   case P_:                                                                                                \
     hipLaunchKernelGGL((rrq_msl_inverse_kernel<RPT_, CT_>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);  \

@@ -50,3 +50,37 @@ def test_offsurface_p8_vs_oracle_and_grf(oracle):
     rel = np.abs(res).max() / np.abs(uex).max()
     print("GRF rel err (off-surface, both sides)", rel, "pairs", nnz, "labels", np.bincount(w["label"]), flush=True)
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("p", [6, 8])
+def test_solve_variants_and_single_stream(p, monkeypatch):
+    """The matrix-core quaternion solve (QOL_SOLVE_MFMA=1) agrees with the default
+    vector-unit solve to the conditioning of the system, and the one-stream
+    schedule (QOL_ONE_STREAM=1) gives the four-stream result bit for bit."""
+    print(f"solve variants and one-stream schedule, VMEC surface 12x30, p={p}", flush=True)
+    geo = W.geometry(12, 30, p, modes=W.VMEC_BENCH)
+    tx, tri, dist = W.targets(geo, 2000, -6, -1, sides=0, seed=5)
+
+    def run():
+        e = Engine(geo)
+        e.set_targets(tx)
+        e.nearlist()
+        e.eval(exterior=True, side_tol=1e-10)
+        return e.weights()
+
+    base = run()
+    monkeypatch.setenv("QOL_ONE_STREAM", "1")
+    one = run()
+    monkeypatch.delenv("QOL_ONE_STREAM")
+    for key in ("label", "As", "Ad"):
+        assert np.array_equal(one[key], base[key]), key
+    monkeypatch.setenv("QOL_SOLVE_MFMA", "1")
+    mf = run()
+    monkeypatch.delenv("QOL_SOLVE_MFMA")
+    assert np.array_equal(mf["label"], base["label"])
+    rq = base["label"] == 5
+    assert rq.any()
+    for key in ("As", "Ad"):
+        assert np.array_equal(mf[key][~rq], base[key][~rq]), key
+        err = np.abs(mf[key][rq] - base[key][rq]) / np.abs(base[key][rq]).max(axis=1, keepdims=True)
+        assert err.max() < 2e-7, (key, err.max())
