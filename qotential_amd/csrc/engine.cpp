@@ -98,6 +98,10 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
   LA.suboff = (const int*)d_suboff.ptr;
   LA.subtyp = (const int*)d_subtyp.ptr;
   LA.dist2 = 1.4 * 1.4;
+  const std::vector<int> chunks = level_chunks(LA, suboff.data());
+  upload(d_chunks, chunks.data(), chunks.size(), s);
+  LA.chunks = (const int*)d_chunks.ptr;
+  LA.nchunk = (int)chunks.size() / 5;
   upload(d_tgl, T.tgl.data(), T.tgl.size(), s);
   upload(d_wgl, T.wgl.data(), T.wgl.size(), s);
   upload(d_Leg, T.Legmat.data(), T.Legmat.size(), s);

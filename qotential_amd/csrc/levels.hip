@@ -6,6 +6,7 @@
 //            lap3dsdlpmat_levels_r64  patch_refine_mod.f90:516-615
 //            lap3dsdlpmat_r64 (direct kernel; zero at coincident points)
 #include <stdexcept>
+#include <vector>
 
 #include "qol_kernels.h"
 
@@ -16,11 +17,32 @@ namespace qol {
 // over the h panel nodes, the same operations in the same order as the host
 // oracle, so the ball centres and radii -- and hence every classification --
 // match it bit for bit.
+// Work goes by chunks of whole sub-patches of one level (at most kChunk nodes,
+// one node per thread; the host builds the chunk table): the chunk's nodes are
+// written to HBM with streaming stores and kept in LDS, and the balls of its
+// sub-patches are summed from LDS in node order, so nothing is read back.
 // Output: lgeo[panel][7][ntot] (x,y,z,nx,ny,nz,w), balls[panel][nsubtot][4].
 constexpr int kLP = 4;
+constexpr int kChunk = 256;
 
-size_t level_geom_smem(int h) { return sizeof(double) * 9 * (size_t)h * kLP; }
+size_t level_geom_smem(int h) { return sizeof(double) * (9 * (size_t)h * kLP + (size_t)kLP * 4 * kChunk); }
 int level_geom_blocks(int npan) { return (npan + kLP - 1) / kLP; }
+
+std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host) {
+  std::vector<int> ch;
+  for (int l = 0; l < 4; ++l) {
+    const int* so = suboff_host + A.sub_base[l] + l;  // nsub + 1 offsets of level l
+    int s0 = 0;
+    while (s0 < A.lev_nsub[l]) {
+      int s1 = s0 + 1;
+      if (so[s1] - so[s0] > kChunk) throw std::runtime_error("level_chunks: a sub-patch exceeds the chunk size");
+      while (s1 < A.lev_nsub[l] && so[s1 + 1] - so[s0] <= kChunk) ++s1;
+      ch.insert(ch.end(), {l, so[s0], so[s1], s0, s1});
+      s0 = s1;
+    }
+  }
+  return ch;
+}
 
 __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, const double* __restrict__ sx,
                                                          const double* __restrict__ rts,
@@ -29,8 +51,10 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
   extern __shared__ double smem[];
   const int h = A.h, k0 = blockIdx.x * kLP;
   const int np_ = min(kLP, npan - k0);
+  const size_t nt_ = A.ntot;
   constexpr int NV = kLP * 9;
-  double* srcT = smem;  // [h][kLP][9]: all values of panel node q together
+  double* srcT = smem;                          // [h][kLP][9]: all values of panel node q together
+  double* cn = smem + (size_t)NV * h;           // [kLP][4][kChunk]: x, y, z, w of the chunk's nodes
   for (int e = threadIdx.x; e < kLP * h; e += blockDim.x) {
     const int pp = e / h, i = e % h;
     double* S = srcT + (size_t)i * NV + pp * 9;
@@ -46,10 +70,13 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
     }
   }
   __syncthreads();
-  for (int l = 0; l < 4; ++l) {
+  for (int ch = 0; ch < A.nchunk; ++ch) {
+    const int* C = A.chunks + 5 * ch;
+    const int l = C[0], c0 = C[1], c1 = C[2], sb0 = C[3], sb1 = C[4];
     const int n = A.lev_n[l], off = A.lev_off[l];
     const double* IM = A.interp + (size_t)h * off;  // column-major n x h
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int i = c0 + (int)threadIdx.x;
+    if (i < c1) {
       double geo[kLP][9];
 #pragma unroll
       for (int pp = 0; pp < kLP; ++pp)
@@ -71,6 +98,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
 #pragma unroll
           for (int c = 0; c < 9; ++c) geo[pp][c] = fma(v, sv[pp * 9 + c], geo[pp][c]);
       }
+      const int li = i - c0;
 #pragma unroll
       for (int pp = 0; pp < kLP; ++pp) {
         if (pp >= np_) break;
@@ -79,47 +107,62 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
         const double n1 = gp[8] * gp[3] - gp[6] * gp[5];
         const double n2 = gp[6] * gp[4] - gp[7] * gp[3];
         const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-        double* G = lgeo + (size_t)(k0 + pp) * 7 * A.ntot + off + i;
-        G[0] = gp[0];
-        G[A.ntot] = gp[1];
-        G[2 * (size_t)A.ntot] = gp[2];
-        G[3 * (size_t)A.ntot] = n0 / sp;
-        G[4 * (size_t)A.ntot] = n1 / sp;
-        G[5 * (size_t)A.ntot] = n2 / sp;
-        G[6 * (size_t)A.ntot] = sp * A.wt[off + i];
+        const double w = sp * A.wt[off + i];
+        double* G = lgeo + (size_t)(k0 + pp) * 7 * nt_ + off + i;
+        __builtin_nontemporal_store(gp[0], G);
+        __builtin_nontemporal_store(gp[1], G + nt_);
+        __builtin_nontemporal_store(gp[2], G + 2 * nt_);
+        __builtin_nontemporal_store(n0 / sp, G + 3 * nt_);
+        __builtin_nontemporal_store(n1 / sp, G + 4 * nt_);
+        __builtin_nontemporal_store(n2 / sp, G + 5 * nt_);
+        __builtin_nontemporal_store(w, G + 6 * nt_);
+        double* L = cn + (size_t)pp * 4 * kChunk + li;
+        L[0] = gp[0];
+        L[kChunk] = gp[1];
+        L[2 * kChunk] = gp[2];
+        L[3 * kChunk] = w;
       }
     }
-  }
-  __syncthreads();
-  // sub-patch balls from the level nodes just written (this workgroup's panels)
-  const int ns_all = A.nsubtot;
-  for (int e = threadIdx.x; e < np_ * ns_all; e += blockDim.x) {
-    const int pp = e / ns_all, s_all = e % ns_all;
-    int l = 0;
-    while (l < 3 && s_all >= A.sub_base[l + 1]) ++l;
-    const int s = s_all - A.sub_base[l], off = A.lev_off[l];
-    const double* G = lgeo + (size_t)(k0 + pp) * 7 * A.ntot + off;
-    const size_t nt_ = A.ntot;
-    const int i0 = A.suboff[A.sub_base[l] + l + s], i1 = A.suboff[A.sub_base[l] + l + s + 1];
-    double swsum = 0.0;
-    for (int i = i0; i < i1; ++i) swsum += G[6 * nt_ + i];
-    double qp[3];
-    for (int c = 0; c < 3; ++c) {
-      double acc = 0.0;
-      for (int i = i0; i < i1; ++i) acc += G[c * nt_ + i] * G[6 * nt_ + i];
-      qp[c] = acc / swsum;
+    __syncthreads();
+    // balls of the chunk's sub-patches, summed in node order from LDS
+    const int nsc = sb1 - sb0;
+    for (int e = threadIdx.x; e < np_ * nsc; e += blockDim.x) {
+      const int pp = e / nsc, s = sb0 + e % nsc;
+      const int s_all = A.sub_base[l] + s;
+      const int i0 = A.suboff[A.sub_base[l] + l + s] - c0, i1 = A.suboff[A.sub_base[l] + l + s + 1] - c0;
+      const double* L = cn + (size_t)pp * 4 * kChunk;
+      // four independent sums, each in node order (the oracle's), in one pass
+      double swsum = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      for (int j = i0; j < i1; ++j) {
+        const double w = L[3 * kChunk + j];
+        swsum += w;
+        a0 += L[j] * w;
+        a1 += L[kChunk + j] * w;
+        a2 += L[2 * kChunk + j] * w;
+      }
+      const double qp[3] = {a0 / swsum, a1 / swsum, a2 / swsum};
+      // a maximum is exact in any order: two interleaved partial maxima
+      double rm0 = 0.0, rm1 = 0.0;
+      int j = i0;
+      for (; j + 1 < i1; j += 2) {
+        const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
+        const double e0 = L[j + 1] - qp[0], e1 = L[kChunk + j + 1] - qp[1], e2 = L[2 * kChunk + j + 1] - qp[2];
+        rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
+        rm1 = fmax(rm1, e0 * e0 + e1 * e1 + e2 * e2);
+      }
+      if (j < i1) {
+        const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
+        rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
+      }
+      const double rmax2 = fmax(rm0, rm1);
+      const double r2 = A.subtyp[s_all] == 1 ? A.dist2 * fmax(rmax2, 0.5 * swsum) : A.dist2 * fmax(rmax2, swsum);
+      double* B = balls + (size_t)(k0 + pp) * 4 * A.nsubtot + 4 * s_all;
+      B[0] = qp[0];
+      B[1] = qp[1];
+      B[2] = qp[2];
+      B[3] = r2;
     }
-    double rmax2 = 0.0;
-    for (int i = i0; i < i1; ++i) {
-      const double d0 = G[i] - qp[0], d1 = G[nt_ + i] - qp[1], d2 = G[2 * nt_ + i] - qp[2];
-      rmax2 = fmax(rmax2, d0 * d0 + d1 * d1 + d2 * d2);
-    }
-    const double r2 = A.subtyp[s_all] == 1 ? A.dist2 * fmax(rmax2, 0.5 * swsum) : A.dist2 * fmax(rmax2, swsum);
-    double* B = balls + (size_t)(k0 + pp) * 4 * A.nsubtot + 4 * s_all;
-    B[0] = qp[0];
-    B[1] = qp[1];
-    B[2] = qp[2];
-    B[3] = r2;
+    __syncthreads();  // the chunk buffer is rewritten by the next chunk
   }
 }
 

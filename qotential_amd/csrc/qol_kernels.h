@@ -1,5 +1,6 @@
 // Kernel interfaces shared by the HIP translation units and the host engine.
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -18,6 +19,8 @@ struct LevelArgs {
   const int* suboff;     // per level (nsub+1) node offsets, concatenated
   const int* subtyp;     // nsubtot
   double dist2;
+  const int* chunks;     // nchunk x {level, first node, end node, first sub-patch, end sub-patch}
+  int nchunk;            // (node and sub-patch indices within the level)
 };
 
 // per-order RRQ tables on the device
@@ -48,6 +51,7 @@ struct RRQPanels {
 __global__ void level_geom_kernel(LevelArgs A, int npan, const double* sx, const double* rts, const double* rps,
                                   double* lgeo, double* balls);
 size_t level_geom_smem(int h);
+std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host);
 int level_geom_blocks(int npan);
 __global__ void classify_kernel(LevelArgs A, const int* offs, const int* tidx, const double* tx,
                                 const double* balls, int8_t* label, int* ppanel, int* nrrq, int* lcount);

@@ -377,6 +377,12 @@ __device__ __forceinline__ void qsub_mul(double* e, const double* a, const doubl
   e[3] = fma(-a[0], b[3], fma(-a[1], b[2], fma(a[2], b[1], fma(-a[3], b[0], e[3]))));
 }
 
+// lowest row not yet pivoted (bit clear in u[0..1]); the fallback pivot when
+// NaN entries leave the search without a candidate
+__device__ __forceinline__ int first_unused(const unsigned long long* u) {
+  return u[0] != ~0ull ? __builtin_ctzll(~u[0]) : 64 + __builtin_ctzll(~u[1]);
+}
+
 bool rrq_solve_g1_in_lds(int h) { return (size_t)6 * h * h * sizeof(double) <= 120 * 1024; }
 size_t rrq_solve_smem(int h) {
   return sizeof(double) * (size_t)((rrq_solve_g1_in_lds(h) ? 4 * h * h : 0) + 2 * h * h);
@@ -507,6 +513,8 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_kernel(RRQTables T, RRQPa
     // identical in every lane: keep them in scalar registers
     pq = __builtin_amdgcn_readfirstlane(pq);
     pr = __builtin_amdgcn_readfirstlane(pr);
+    if (pq >= h) pq = first_unused(uq);
+    if (pr >= h) pr = first_unused(ur);
     uq[pq >> 6] |= 1ull << (pq & 63);
     ur[pr >> 6] |= 1ull << (pr & 63);
     if (threadIdx.x == 0) { pivq[kk] = pq; pivr[kk] = pr; }
@@ -707,6 +715,7 @@ __global__ void __launch_bounds__(256) rrq_panel_solve_mfma_kernel(RRQTables T, 
       }
     }
     pq = __builtin_amdgcn_readfirstlane(pq);
+    if (pq >= h) pq = first_unused(uq);
     uq[pq >> 6] |= 1ull << (pq & 63);
     if (threadIdx.x == 0) pivq[kk] = pq;
     const int rtp = pq >> 2, ip = pq & 3, wp = rtp & 3, tp = rtp >> 2;
@@ -856,6 +865,7 @@ __global__ void __launch_bounds__(256) rrq_msl_inverse_kernel(RRQTables T, RRQPa
       }
     }
     pr = __builtin_amdgcn_readfirstlane(pr);
+    if (pr >= h) pr = first_unused(ur);
     ur[pr >> 6] |= 1ull << (pr & 63);
     if (threadIdx.x == 0) pivr[kk] = pr;
 #pragma unroll
@@ -903,6 +913,198 @@ __global__ void __launch_bounds__(256) rrq_msl_inverse_kernel(RRQTables T, RRQPa
     }
 }
 
+// One wave per panel, the whole elimination in registers: X = Fq^{-1} for the
+// h x h pure-quaternion matrix Fq, h <= 8*NA.  Lane (ly, lx) of an 8 x 8 lane
+// grid owns rows ly + 8a and column slots lx + 8b (a, b < NA).  Slot s holds left
+// column s until step s eliminates it and right column h+s afterwards (the unit
+// column of the row pivoted at step s, zero in every other row until then), so h
+// slots carry the live window (kk, h+kk] at every step.  Pivoting is implicit
+// (no row moves): the unused row with the largest |A_r,kk| wins, ties to the
+// lower row.  The pivot column and the pivot row pass through LDS inside the
+// wave, so a step needs no workgroup barrier.  Afterwards row piv[k] holds row k
+// of X with slot s holding column piv[s].
+// Output: the Ad rows of the weights operator, G(h+j, q*h+k) = (X_kj)_q * fac.
+template <int NA>
+__global__ void __launch_bounds__(64) rrq_quat_inverse_kernel(RRQTables T, RRQPanels P, int npan,
+                                                              const int* __restrict__ nrrq) {
+  const int k = blockIdx.x;
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
+  const int h = T.h, m = 2 * h;
+  constexpr int N8 = 8 * NA;
+  __shared__ double colq[2][4 * N8], rowq[2][4 * N8];
+  __shared__ int piv[N8], pos[N8];
+  const int lane = threadIdx.x, ly = lane >> 3, lx = lane & 7;
+  const double* FQ = P.Fq + (size_t)k * 3 * h * h;
+  double A[NA][NA][4];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NA; ++b) {
+      const int r = ly + 8 * a, c = lx + 8 * b;
+      double* e = A[a][b];
+      e[0] = e[1] = e[2] = e[3] = 0.0;
+      if (r < h && c < h) {
+        const double* q = FQ + 3 * ((size_t)r * h + c);
+        e[1] = q[0]; e[2] = q[1]; e[3] = q[2];
+      }
+    }
+  unsigned long long used = 0ull;  // rows already pivoted (N8 <= 64)
+  for (int kk = 0; kk < h; ++kk) {
+    const int buf = kk & 1, kt = kk & 7, kb = kk >> 3;
+    double* cq = colq[buf];
+    double* rq = rowq[buf];
+    const bool own = lx == kt;  // lanes holding column kk (in slot kb)
+    double bv = -1.0;
+    int bi = N8;
+#pragma unroll
+    for (int b = 0; b < NA; ++b) {
+      if (b != kb) continue;  // uniform
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int r = ly + 8 * a;
+        const double* e = A[a][b];
+        if (own) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cq[4 * r + q] = e[q];
+        }
+        const bool fr = (used >> r) & 1ull;
+        const double v = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
+        const bool t = own & (r < h) & !fr & (v > bv);
+        bv = t ? v : bv;
+        bi = t ? r : bi;
+      }
+    }
+    // best of the 8 lanes of column kk (lane ids kt + 8 ly)
+#pragma unroll
+    for (int off = 8; off <= 32; off <<= 1) {
+      const double ov = __shfl_xor(bv, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      const bool t = (ov > bv) | ((ov == bv) & (oi < bi));
+      bv = t ? ov : bv;
+      bi = t ? oi : bi;
+    }
+    int p = __builtin_amdgcn_readlane(bi, kt);
+    // no candidate only with NaN entries: take the lowest unused row, so every
+    // row is pivoted once and every index stays in range
+    if (p >= h) p = __builtin_ctzll(~used);
+    used |= 1ull << p;
+    if (lane == 0) piv[kk] = p;
+    const int pa = p >> 3, pl = p & 7;
+    // pivot row by slot; slot kk carries the unit entry of right column h+kk
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      if (a != pa) continue;  // uniform
+      if (ly == pl) {
+#pragma unroll
+        for (int b = 0; b < NA; ++b) {
+          const int c = lx + 8 * b;
+          const bool unit = c == kk;
+          rq[4 * c] = unit ? 1.0 : A[a][b][0];
+#pragma unroll
+          for (int q = 1; q < 4; ++q) rq[4 * c + q] = unit ? 0.0 : A[a][b][q];
+        }
+      }
+    }
+    // slot kk turns into right column h+kk, zero before the update
+#pragma unroll
+    for (int b = 0; b < NA; ++b) {
+      if (b != kb) continue;  // uniform
+      if (own) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) A[a][b][0] = A[a][b][1] = A[a][b][2] = A[a][b][3] = 0.0;
+      }
+    }
+    __syncthreads();  // one wave: orders the LDS stores above before the loads below
+    double pinv[4];
+    {
+      const double* e = cq + 4 * p;
+      const double in2 = 1.0 / (e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3]);
+      pinv[0] = e[0] * in2; pinv[1] = -e[1] * in2; pinv[2] = -e[2] * in2; pinv[3] = -e[3] * in2;
+    }
+    // every row: A_r -= A_r,kk * (pivot^-1 * pivot row); the pivot row itself
+    // becomes the normalised row
+#pragma unroll
+    for (int b = 0; b < NA; ++b) {
+      double nq[4];
+      qmul_fma(pinv, rq + 4 * (lx + 8 * b), nq);
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        qsub_mul(A[a][b], cq + 4 * (ly + 8 * a), nq);
+        if (a == pa) {  // uniform
+#pragma unroll
+          for (int q = 0; q < 4; ++q) A[a][b][q] = ly == pl ? nq[q] : A[a][b][q];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < h; i += 64) pos[piv[i]] = i;
+  __syncthreads();
+  const double fac = 1.0 / (16.0 * atan(1.0));
+  double* G = P.G + (size_t)k * m * 5 * h;
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < NA; ++b) {
+      const int r = ly + 8 * a, c = lx + 8 * b;
+      if (r < h && c < h) {
+        const int kx = pos[r], j = piv[c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) G[(size_t)(q * h + kx) * m + h + j] = A[a][b][q] * fac;
+      }
+    }
+}
+
+// The rest of the weights operator from its Ad rows (written by
+// rrq_quat_inverse_kernel) and B = (Msl^T)^{-1} (from rrq_msl_inverse_kernel):
+// C = B Fn^T, As rows G(j, c) = (1/alpha) sum_i C(j,i) G(h+i, c) for c < 4h,
+// G(j, 4h+r) = B(j, r) fac/alpha, and G(h+j, 4h+r) = 0.  A thread makes the four
+// entries c = q*h + kq (q = 0..3) of one row j.
+__global__ void __launch_bounds__(256) rrq_solve_as_kernel(RRQTables T, RRQPanels P, int npan,
+                                                           const int* __restrict__ nrrq) {
+  const int k = blockIdx.x;
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
+  const int h = T.h, H4 = 4 * h, m = 2 * h;
+  extern __shared__ double dyn[];
+  double* Bt = dyn;                 // Bt[r*h + j] = B(j, r)
+  double* Ct = dyn + (size_t)h * h; // Ct[i*h + j] = C(j, i)
+  const double* Bk = P.Binv + (size_t)k * h * h;
+  const double* FN = P.Fn + (size_t)k * h * h;
+  double* G = P.G + (size_t)k * m * 5 * h;
+  const double alpha = P.frame[(size_t)k * 16 + 12];
+  const double fac = 1.0 / (16.0 * atan(1.0)), ia = 1.0 / alpha;
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) Bt[(size_t)(e % h) * h + e / h] = Bk[e];
+  __syncthreads();
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+    const int j = e % h, i = e / h;
+    double s = 0.0;
+    for (int r = 0; r < h; ++r) s = fma(Bt[(size_t)r * h + j], FN[i + (size_t)h * r], s);
+    Ct[(size_t)i * h + j] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+    const int j = e % h, kq = e / h;
+    const double* g0 = G + (size_t)kq * m + h;
+    const size_t st = (size_t)h * m;  // from c to c + h
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int i = 0; i < h; ++i) {
+      const double cv = Ct[(size_t)i * h + j];
+      s0 = fma(cv, g0[i], s0);
+      s1 = fma(cv, g0[st + i], s1);
+      s2 = fma(cv, g0[2 * st + i], s2);
+      s3 = fma(cv, g0[3 * st + i], s3);
+    }
+    double* o = G + (size_t)kq * m + j;
+    o[0] = s0 * ia;
+    o[st] = s1 * ia;
+    o[2 * st] = s2 * ia;
+    o[3 * st] = s3 * ia;
+    double* w = G + (size_t)(H4 + kq) * m;
+    w[j] = Bt[(size_t)kq * h + j] * fac * ia;
+    w[h + j] = 0.0;
+  }
+}
+
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s) {
   const int h = T.h;
   const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
@@ -930,6 +1132,28 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
     default: break;
   }
 #undef QOL_SOLVE_MFMA
+  // patch orders up to 8 (h <= 40): the real inverse, the one-wave quaternion
+  // inverse, then the As rows; QOL_SOLVE_FUSED=1 keeps the single fused kernel
+  const char* fu = std::getenv("QOL_SOLVE_FUSED");
+  if (h <= 40 && !(fu && fu[0] == '1')) {
+    const int na = (h + 7) / 8;
+    if (rpt <= 1 && cpt <= 2)
+      hipLaunchKernelGGL((rrq_msl_inverse_kernel<1, 2>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
+    else if (rpt <= 2 && cpt <= 4)
+      hipLaunchKernelGGL((rrq_msl_inverse_kernel<2, 4>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
+    else
+      hipLaunchKernelGGL((rrq_msl_inverse_kernel<3, 5>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
+    switch (na) {
+      case 1: hipLaunchKernelGGL((rrq_quat_inverse_kernel<1>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
+      case 2: hipLaunchKernelGGL((rrq_quat_inverse_kernel<2>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
+      case 3: hipLaunchKernelGGL((rrq_quat_inverse_kernel<3>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
+      case 4: hipLaunchKernelGGL((rrq_quat_inverse_kernel<4>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
+      default: hipLaunchKernelGGL((rrq_quat_inverse_kernel<5>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
+    }
+    hipLaunchKernelGGL(rrq_solve_as_kernel, dim3(npan), dim3(256), sizeof(double) * 2 * (size_t)h * h, s, T, P, npan,
+                       nrrq);
+    return;
+  }
   if (rpt <= 3 && cpt <= 5) {
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void*)rrq_panel_solve_kernel<3, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -54,9 +54,10 @@ def test_offsurface_p8_vs_oracle_and_grf(oracle):
 
 @pytest.mark.parametrize("p", [6, 8])
 def test_solve_variants_and_single_stream(p, monkeypatch):
-    """The matrix-core quaternion solve (QOL_SOLVE_MFMA=1) agrees with the default
-    vector-unit solve to the conditioning of the system, and the one-stream
-    schedule (QOL_ONE_STREAM=1) gives the four-stream result bit for bit."""
+    """The fused vector-unit solve (QOL_SOLVE_FUSED=1) and the matrix-core
+    quaternion solve (QOL_SOLVE_MFMA=1) agree with the default one-wave solve to
+    the conditioning of the system, and the one-stream schedule (QOL_ONE_STREAM=1)
+    gives the four-stream result bit for bit."""
     print(f"solve variants and one-stream schedule, VMEC surface 12x30, p={p}", flush=True)
     geo = W.geometry(12, 30, p, modes=W.VMEC_BENCH)
     tx, tri, dist = W.targets(geo, 2000, -6, -1, sides=0, seed=5)
@@ -74,13 +75,14 @@ def test_solve_variants_and_single_stream(p, monkeypatch):
     monkeypatch.delenv("QOL_ONE_STREAM")
     for key in ("label", "As", "Ad"):
         assert np.array_equal(one[key], base[key]), key
-    monkeypatch.setenv("QOL_SOLVE_MFMA", "1")
-    mf = run()
-    monkeypatch.delenv("QOL_SOLVE_MFMA")
-    assert np.array_equal(mf["label"], base["label"])
     rq = base["label"] == 5
     assert rq.any()
-    for key in ("As", "Ad"):
-        assert np.array_equal(mf[key][~rq], base[key][~rq]), key
-        err = np.abs(mf[key][rq] - base[key][rq]) / np.abs(base[key][rq]).max(axis=1, keepdims=True)
-        assert err.max() < 2e-7, (key, err.max())
+    for var in ("QOL_SOLVE_MFMA", "QOL_SOLVE_FUSED"):
+        monkeypatch.setenv(var, "1")
+        alt = run()
+        monkeypatch.delenv(var)
+        assert np.array_equal(alt["label"], base["label"]), var
+        for key in ("As", "Ad"):
+            assert np.array_equal(alt[key][~rq], base[key][~rq]), (var, key)
+            err = np.abs(alt[key][rq] - base[key][rq]) / np.abs(base[key][rq]).max(axis=1, keepdims=True)
+            assert err.max() < 2e-7, (var, key, err.max())
