@@ -113,6 +113,9 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
   RT.Legmat = (const double*)d_Leg.ptr;
   RT.umatr = (const double*)d_umatr.ptr;
   RT.uvn = (const double*)d_uvn.ptr;
+  const std::vector<double> kt = koorn_eval_table((int)T.p);
+  upload(d_ktab, kt.data(), kt.size(), s);
+  RT.ktab = (const double*)d_ktab.ptr;
   HIPCHK(hipStreamSynchronize(s));
 }
 

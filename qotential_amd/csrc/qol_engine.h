@@ -33,7 +33,7 @@ struct DeviceTables {
   int h = 0, ntot = 0, nsubtot = 0;
   LevelArgs LA{};
   RRQTables RT{};
-  DevBuf d_interp, d_imp, d_wt, d_suboff, d_subtyp, d_chunks, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn;
+  DevBuf d_interp, d_imp, d_wt, d_suboff, d_subtyp, d_chunks, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn, d_ktab;
   DeviceTables(i64 p, hipStream_t s);
 };
 

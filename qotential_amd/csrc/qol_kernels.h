@@ -31,7 +31,9 @@ struct RRQTables {
   const double* Legmat;  // nq x nq (coef = Legmat * vals)
   const double* umatr;   // h x h
   const double* uvn;     // 2 x h VR nodes
+  const double* ktab;    // Koornwinder evaluation constants (koorn_eval_table(p))
 };
+std::vector<double> koorn_eval_table(int N);
 
 // per-panel RRQ data on the device (one slot per panel)
 struct RRQPanels {

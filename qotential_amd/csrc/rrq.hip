@@ -17,8 +17,13 @@ namespace qol {
 
 // ---------------------------------------------------------------------------
 // geometry polynomial x(u,v) = sum_k K_k(u,v) cx[c][k] and its uv derivatives,
-// accumulated while the Koornwinder basis is generated (no h-sized arrays)
-__device__ void geom_eval(const double* cx, int h, int korder, double u, double v, double* x, double* xu, double* xv) {
+// accumulated while the Koornwinder basis is generated (no h-sized arrays).
+// The recurrence constants come from the per-order table kt (koorn_eval_table:
+// no divisions or square roots here): kt[2am], kt[2am+1] scale the Legendre
+// step to am, and term t (loop order am, bd) has its normalisation and Jacobi
+// coefficients at kt[2N + 4t .. 2N + 4t + 3].
+__device__ void geom_eval(const double* cx, const double* __restrict__ kt, int h, int korder, double u, double v,
+                          double* x, double* xu, double* xv) {
   const int N = korder + 1;
   const double a = 2.0 * u + v - 1.0, b = 1.0 - v, bb = b * b;
   for (int c = 0; c < 3; ++c) { x[c] = 0.0; if (xu) { xu[c] = 0.0; xv[c] = 0.0; } }
@@ -27,37 +32,33 @@ __device__ void geom_eval(const double* cx, int h, int korder, double u, double 
   // for each am, Jacobi P_bd^(2am+1,0)(xx) by its three-term recurrence in bd.
   // Basis index of (degree d = am + bd, bd) is d(d+1)/2 + bd.
   double Q = 1.0, Qu = 0.0, Qv = 0.0, Qm = 0.0, Qum = 0.0, Qvm = 0.0;
+  const double* tt = kt + 2 * N;
   for (int am = 0; am < N; ++am) {
     if (am == 1) {
       Qm = Q; Qum = Qu; Qvm = Qv;
       Q = a; Qu = 2.0; Qv = 1.0;
     } else if (am >= 2) {
-      const int m = am - 1;
-      const double c1 = 2.0 * m + 1.0, c2 = (double)m, inv = 1.0 / (double)(m + 1);
-      const double Qn = (c1 * a * Q - c2 * bb * Qm) * inv;
-      const double Qun = (c1 * (2.0 * Q + a * Qu) - c2 * bb * Qum) * inv;
-      const double Qvn = (c1 * (Q + a * Qv) - c2 * (bb * Qvm - 2.0 * b * Qm)) * inv;
+      const double e1 = kt[2 * am], e2 = kt[2 * am + 1];
+      const double Qn = e1 * a * Q - e2 * bb * Qm;
+      const double Qun = e1 * (2.0 * Q + a * Qu) - e2 * bb * Qum;
+      const double Qvn = e1 * (Q + a * Qv) - e2 * (bb * Qvm - 2.0 * b * Qm);
       Qm = Q; Qum = Qu; Qvm = Qv;
       Q = Qn; Qu = Qun; Qv = Qvn;
     }
     const double al = 2.0 * am + 1.0;
     double p = 1.0, dp = 0.0, pm = 1.0, dpm = 0.0;
-    for (int bd = 0; am + bd < N; ++bd) {
+    for (int bd = 0; am + bd < N; ++bd, tt += 4) {
       if (bd == 1) {
         p = 0.5 * ((al + 2.0) * xx + al);
         dp = al + 2.0;
       } else if (bd >= 2) {
-        const double dk = (double)(bd - 1);
-        const double a1 = 2.0 * (dk + 1.0) * (dk + al + 1.0) * (2.0 * dk + al);
-        const double a2 = (2.0 * dk + al + 1.0) * al * al;
-        const double a3 = (2.0 * dk + al) * (2.0 * dk + al + 1.0) * (2.0 * dk + al + 2.0);
-        const double a4 = 2.0 * (dk + al) * dk * (2.0 * dk + al + 2.0);
-        const double pn = ((a2 + a3 * xx) * p - a4 * pm) / a1;
-        const double dpn = ((a2 + a3 * xx) * dp + 2.0 * a3 * p - a4 * dpm) / a1;
+        const double w = tt[1] + tt[2] * xx;
+        const double pn = w * p - tt[3] * pm;
+        const double dpn = w * dp + 2.0 * tt[2] * p - tt[3] * dpm;
         pm = p; p = pn; dpm = dp; dp = dpn;
       }
       const int d = am + bd, idx = d * (d + 1) / 2 + bd;
-      const double sc = sqrt((2.0 * am + 1.0) * 2.0 * (double)(d + 1));
+      const double sc = tt[0];
       const double f = sc * Q * p;
       for (int c = 0; c < 3; ++c) x[c] += f * cx[c * h + idx];
       if (xu) {
@@ -66,6 +67,33 @@ __device__ void geom_eval(const double* cx, int h, int korder, double u, double 
       }
     }
   }
+}
+
+std::vector<double> koorn_eval_table(int N) {
+  std::vector<double> t(2 * (size_t)N, 0.0);
+  for (int am = 2; am < N; ++am) {
+    const int m = am - 1;
+    const double inv = 1.0 / (double)(m + 1);
+    t[2 * am] = (2.0 * m + 1.0) * inv;
+    t[2 * am + 1] = (double)m * inv;
+  }
+  for (int am = 0; am < N; ++am) {
+    const double al = 2.0 * am + 1.0;
+    for (int bd = 0; am + bd < N; ++bd) {
+      const int d = am + bd;
+      double ca = 0.0, cb = 0.0, cc = 0.0;
+      if (bd >= 2) {
+        const double dk = (double)(bd - 1);
+        const double a1 = 2.0 * (dk + 1.0) * (dk + al + 1.0) * (2.0 * dk + al);
+        const double a2 = (2.0 * dk + al + 1.0) * al * al;
+        const double a3 = (2.0 * dk + al) * (2.0 * dk + al + 1.0) * (2.0 * dk + al + 2.0);
+        const double a4 = 2.0 * (dk + al) * dk * (2.0 * dk + al + 2.0);
+        ca = a2 / a1; cb = a3 / a1; cc = a4 / a1;
+      }
+      t.insert(t.end(), {std::sqrt((2.0 * am + 1.0) * 2.0 * (double)(d + 1)), ca, cb, cc});
+    }
+  }
+  return t;
 }
 
 __device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
@@ -168,7 +196,7 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
   // orientation: sigma = sign(sum (x_u x x_v) . snx) over the nodes
   for (int i = threadIdx.x; i < h; i += blockDim.x) {
     double x[3], xu[3], xv[3], n[3];
-    geom_eval(scx, h, p - 1, T.uvn[2 * i], T.uvn[2 * i + 1], x, xu, xv);
+    geom_eval(scx, T.ktab, h, p - 1, T.uvn[2 * i], T.uvn[2 * i + 1], x, xu, xv);
     cross3(xu, xv, n);
     sg[i] = n[0] * ssn[i] + n[1] * ssn[h + i] + n[2] * ssn[2 * h + i];
   }
@@ -180,7 +208,7 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
       for (int c = 0; c < 3; ++c) srv[3 * j + c] = rv_chart[9 * (size_t)k + 3 * j + c];
     } else {
       double xv[3];
-      geom_eval(scx, h, p - 1, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, xv, nullptr, nullptr);
+      geom_eval(scx, T.ktab, h, p - 1, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0, xv, nullptr, nullptr);
       for (int c = 0; c < 3; ++c) srv[3 * j + c] = xv[c];
     }
   }
@@ -244,7 +272,7 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
       const double EA[3][2] = {{0, 1}, {0, 0}, {1, 0}}, EB[3][2] = {{0, 0}, {1, 0}, {0, 1}};
       const double s01 = 0.5 * (T.tgl[i] + 1.0);
       const double u = EA[e][0] + (EB[e][0] - EA[e][0]) * s01, v = EA[e][1] + (EB[e][1] - EA[e][1]) * s01;
-      geom_eval(scx, h, p - 1, u, v, x, nullptr, nullptr);
+      geom_eval(scx, T.ktab, h, p - 1, u, v, x, nullptr, nullptr);
     }
     to_frame(fr, x, bx + 6 * ib);
   }
@@ -913,141 +941,182 @@ __global__ void __launch_bounds__(256) rrq_msl_inverse_kernel(RRQTables T, RRQPa
     }
 }
 
-// One wave per panel, the whole elimination in registers: X = Fq^{-1} for the
-// h x h pure-quaternion matrix Fq, h <= 8*NA.  Lane (ly, lx) of an 8 x 8 lane
-// grid owns rows ly + 8a and column slots lx + 8b (a, b < NA).  Slot s holds left
-// column s until step s eliminates it and right column h+s afterwards (the unit
-// column of the row pivoted at step s, zero in every other row until then), so h
-// slots carry the live window (kk, h+kk] at every step.  Pivoting is implicit
-// (no row moves): the unused row with the largest |A_r,kk| wins, ties to the
-// lower row.  The pivot column and the pivot row pass through LDS inside the
-// wave, so a step needs no workgroup barrier.  Afterwards row piv[k] holds row k
-// of X with slot s holding column piv[s].
+// X = Fq^{-1} for the h x h pure-quaternion matrix Fq by Gauss-Jordan, the
+// matrix in registers of NW waves (one panel per workgroup).  Lane (ly, lx) of
+// an RL x CL lane grid (RL*CL <= 64; lanes beyond it idle) owns rows ly + RL*a
+// (a < RA) and column slots lx + CL*b for the wave's CB/NW slot groups b;
+// RL*RA >= h and CL*CB >= h, the grid chosen per order to waste few registers.
+// Slot s holds left column s until step s eliminates it and right column h+s
+// afterwards (the unit column of the row pivoted at step s, zero in every other
+// row until then), so h slots carry the live window (kk, h+kk] at every step.
+// Pivoting is implicit (no row moves): the unused row with the largest |A_r,kk|
+// wins, ties to the lower row.  Column kk lies in one wave: it finds the pivot
+// and publishes the column with a zero in the pivot row (so the rank-1 update,
+// the same in every lane, leaves that row alone; it is scaled by the inverse
+// pivot afterwards).  Each wave already holds its part of the pivot row, so one
+// workgroup barrier per step suffices (LDS double-buffered by step parity).
+// Afterwards row piv[k] holds row k of X with slot s holding column piv[s].
 // Output: the Ad rows of the weights operator, G(h+j, q*h+k) = (X_kj)_q * fac.
-template <int NA>
-__global__ void __launch_bounds__(64) rrq_quat_inverse_kernel(RRQTables T, RRQPanels P, int npan,
-                                                              const int* __restrict__ nrrq) {
+template <int RL, int RA, int CL, int CB, int NW>
+__global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, RRQPanels P, int npan,
+                                                                   const int* __restrict__ nrrq) {
+  static_assert(RL * CL <= 64 && RL * RA <= 64 && CB % NW == 0, "lane grid");
+  constexpr int CW = CB / NW;  // slot groups per wave
   const int k = blockIdx.x;
   if (nrrq && nrrq[k] == 0) return;  // null: every panel
   const int h = T.h, m = 2 * h;
-  constexpr int N8 = 8 * NA;
-  __shared__ double colq[2][4 * N8], rowq[2][4 * N8];
-  __shared__ int piv[N8], pos[N8];
-  const int lane = threadIdx.x, ly = lane >> 3, lx = lane & 7;
+  constexpr int NRW = RL * RA, NSL = CL * CB;
+  __shared__ double colq[2][4 * NRW], rowq[2][4 * NSL], pivel[2][4];
+  __shared__ int pivs[2], piv[NRW], pos[NRW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool act = lane < RL * CL;
+  const int ly = lane / CL, lx = lane % CL;
   const double* FQ = P.Fq + (size_t)k * 3 * h * h;
-  double A[NA][NA][4];
+  double A[RA][CW][4];
 #pragma unroll
-  for (int a = 0; a < NA; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
-    for (int b = 0; b < NA; ++b) {
-      const int r = ly + 8 * a, c = lx + 8 * b;
+    for (int b = 0; b < CW; ++b) {
+      const int r = ly + RL * a, c = lx + CL * (wv * CW + b);
       double* e = A[a][b];
       e[0] = e[1] = e[2] = e[3] = 0.0;
-      if (r < h && c < h) {
+      if (act & (r < h) & (c < h)) {
         const double* q = FQ + 3 * ((size_t)r * h + c);
         e[1] = q[0]; e[2] = q[1]; e[3] = q[2];
       }
     }
-  unsigned long long used = 0ull;  // rows already pivoted (N8 <= 64)
+  unsigned long long used = 0ull;  // rows already pivoted (h <= 64)
   for (int kk = 0; kk < h; ++kk) {
-    const int buf = kk & 1, kt = kk & 7, kb = kk >> 3;
+    const int buf = kk & 1, kt = kk % CL, kbg = kk / CL;
+    const int ow = kbg / CW, kb = kbg % CW;  // wave and slot group holding column kk
     double* cq = colq[buf];
     double* rq = rowq[buf];
-    const bool own = lx == kt;  // lanes holding column kk (in slot kb)
-    double bv = -1.0;
-    int bi = N8;
+    if (wv == ow) {  // uniform
+      const bool own = act & (lx == kt);
+      double bv = -1.0;
+      int bi = NRW;
 #pragma unroll
-    for (int b = 0; b < NA; ++b) {
-      if (b != kb) continue;  // uniform
+      for (int b = 0; b < CW; ++b) {
+        if (b != kb) continue;  // uniform
 #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const int r = ly + 8 * a;
-        const double* e = A[a][b];
-        if (own) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) cq[4 * r + q] = e[q];
+        for (int a = 0; a < RA; ++a) {
+          const int r = ly + RL * a;
+          const double* e = A[a][b];
+          const double v = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
+          const bool t = own & (r < h) & !((used >> (r & 63)) & 1ull) & (v > bv);
+          bv = t ? v : bv;
+          bi = t ? r : bi;
         }
-        const bool fr = (used >> r) & 1ull;
-        const double v = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
-        const bool t = own & (r < h) & !fr & (v > bv);
-        bv = t ? v : bv;
-        bi = t ? r : bi;
+      }
+      // best of the RL owner lanes (lane ids kt + CL*t)
+      int p = NRW;
+      {
+        double pv = -1.0;
+#pragma unroll
+        for (int t = 0; t < RL; ++t) {
+          const double v = __shfl(bv, kt + CL * t, 64);
+          const int i = __shfl(bi, kt + CL * t, 64);
+          const bool tk = (v > pv) | ((v == pv) & (i < p));
+          pv = tk ? v : pv;
+          p = tk ? i : p;
+        }
+      }
+      p = __builtin_amdgcn_readfirstlane(p);
+      // no candidate only with NaN entries: take the lowest unused row, so every
+      // row is pivoted once and every index stays in range
+      if (p >= h) p = __builtin_ctzll(~used);
+      const int pa = p / RL;
+      const bool prow = act & (ly == p % RL);
+      if (lane == 0) pivs[buf] = p;
+      // column kk by row (zero in the pivot row), the pivot itself, and slot kk
+      // turned into right column h+kk: zero, 1 in the pivot row
+#pragma unroll
+      for (int b = 0; b < CW; ++b) {
+        if (b != kb) continue;  // uniform
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+          double* e = A[a][b];
+          if (own) {
+            const int r = ly + RL * a;
+            const bool isp = (a == pa) & prow;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cq[4 * r + q] = isp ? 0.0 : e[q];
+            if (isp) {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) pivel[buf][q] = e[q];
+            }
+            e[0] = isp ? 1.0 : 0.0;
+            e[1] = e[2] = e[3] = 0.0;
+          }
+        }
       }
     }
-    // best of the 8 lanes of column kk (lane ids kt + 8 ly)
-#pragma unroll
-    for (int off = 8; off <= 32; off <<= 1) {
-      const double ov = __shfl_xor(bv, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      const bool t = (ov > bv) | ((ov == bv) & (oi < bi));
-      bv = t ? ov : bv;
-      bi = t ? oi : bi;
-    }
-    int p = __builtin_amdgcn_readlane(bi, kt);
-    // no candidate only with NaN entries: take the lowest unused row, so every
-    // row is pivoted once and every index stays in range
-    if (p >= h) p = __builtin_ctzll(~used);
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(pivs[buf]);
     used |= 1ull << p;
-    if (lane == 0) piv[kk] = p;
-    const int pa = p >> 3, pl = p & 7;
-    // pivot row by slot; slot kk carries the unit entry of right column h+kk
+    if (threadIdx.x == 0) piv[kk] = p;
+    const int pa = p / RL;
+    const bool prow = act & (ly == p % RL);
+    // this wave's part of the pivot row, by slot (wave-local exchange)
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
+    for (int a = 0; a < RA; ++a) {
       if (a != pa) continue;  // uniform
-      if (ly == pl) {
+      if (prow) {
 #pragma unroll
-        for (int b = 0; b < NA; ++b) {
-          const int c = lx + 8 * b;
-          const bool unit = c == kk;
-          rq[4 * c] = unit ? 1.0 : A[a][b][0];
+        for (int b = 0; b < CW; ++b) {
+          const int c = lx + CL * (wv * CW + b);
 #pragma unroll
-          for (int q = 1; q < 4; ++q) rq[4 * c + q] = unit ? 0.0 : A[a][b][q];
+          for (int q = 0; q < 4; ++q) rq[4 * c + q] = A[a][b][q];
         }
       }
     }
-    // slot kk turns into right column h+kk, zero before the update
-#pragma unroll
-    for (int b = 0; b < NA; ++b) {
-      if (b != kb) continue;  // uniform
-      if (own) {
-#pragma unroll
-        for (int a = 0; a < NA; ++a) A[a][b][0] = A[a][b][1] = A[a][b][2] = A[a][b][3] = 0.0;
-      }
-    }
-    __syncthreads();  // one wave: orders the LDS stores above before the loads below
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double pinv[4];
     {
-      const double* e = cq + 4 * p;
+      const double* e = pivel[buf];
       const double in2 = 1.0 / (e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3]);
       pinv[0] = e[0] * in2; pinv[1] = -e[1] * in2; pinv[2] = -e[2] * in2; pinv[3] = -e[3] * in2;
     }
-    // every row: A_r -= A_r,kk * (pivot^-1 * pivot row); the pivot row itself
-    // becomes the normalised row
+    // every other row: A_r -= A_r,kk * (pivot^-1 * pivot row); the normalised
+    // row entries of this wave's slots first, then one column value per row
+    double nq[CW][4];
 #pragma unroll
-    for (int b = 0; b < NA; ++b) {
-      double nq[4];
-      qmul_fma(pinv, rq + 4 * (lx + 8 * b), nq);
+    for (int b = 0; b < CW; ++b) qmul_fma(pinv, rq + 4 * (lx + CL * (wv * CW + b)), nq[b]);
 #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        qsub_mul(A[a][b], cq + 4 * (ly + 8 * a), nq);
-        if (a == pa) {  // uniform
+    for (int a = 0; a < RA; ++a) {
+      const double* c = cq + 4 * (ly + RL * a);
+      const double cv[4] = {c[0], c[1], c[2], c[3]};
 #pragma unroll
-          for (int q = 0; q < 4; ++q) A[a][b][q] = ly == pl ? nq[q] : A[a][b][q];
+      for (int b = 0; b < CW; ++b) qsub_mul(A[a][b], cv, nq[b]);
+    }
+    // the pivot row: pivot^-1 * row
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      if (a != pa) continue;  // uniform
+      if (prow) {
+#pragma unroll
+        for (int b = 0; b < CW; ++b) {
+          double nq[4];
+          qmul_fma(pinv, A[a][b], nq);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) A[a][b][q] = nq[q];
         }
       }
     }
   }
   __syncthreads();
-  for (int i = lane; i < h; i += 64) pos[piv[i]] = i;
+  for (int i = threadIdx.x; i < h; i += 64 * NW) pos[piv[i]] = i;
   __syncthreads();
   const double fac = 1.0 / (16.0 * atan(1.0));
   double* G = P.G + (size_t)k * m * 5 * h;
 #pragma unroll
-  for (int a = 0; a < NA; ++a)
+  for (int a = 0; a < RA; ++a)
 #pragma unroll
-    for (int b = 0; b < NA; ++b) {
-      const int r = ly + 8 * a, c = lx + 8 * b;
-      if (r < h && c < h) {
+    for (int b = 0; b < CW; ++b) {
+      const int r = ly + RL * a, c = lx + CL * (wv * CW + b);
+      if (act & (r < h) & (c < h)) {
         const int kx = pos[r], j = piv[c];
 #pragma unroll
         for (int q = 0; q < 4; ++q) G[(size_t)(q * h + kx) * m + h + j] = A[a][b][q] * fac;
@@ -1055,53 +1124,196 @@ __global__ void __launch_bounds__(64) rrq_quat_inverse_kernel(RRQTables T, RRQPa
     }
 }
 
+// B = (Msl^T)^{-1} by Gauss-Jordan in the registers of one wave per panel, the
+// real counterpart of rrq_quat_inverse_kernel (same lane grid, slot reuse,
+// implicit pivoting with ties to the lower row, zeroed pivot-row column entry).
+// The system matrix is A(r, c) = Msl[r + h*c]; output B row-major, B(kx, j) at
+// Binv[kx*h + j].
+template <int RL, int RA, int CL, int CB>
+__global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPanels P, int npan,
+                                                              const int* __restrict__ nrrq) {
+  static_assert(RL * CL <= 64 && RL * RA <= 64, "lane grid");
+  const int k = blockIdx.x;
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
+  const int h = T.h;
+  constexpr int NRW = RL * RA, NSL = CL * CB;
+  __shared__ double colr[2][NRW], rowr[2][NSL], pivel[2];
+  __shared__ int piv[NRW], pos[NRW];
+  const int lane = threadIdx.x;
+  const bool act = lane < RL * CL;
+  const int ly = lane / CL, lx = lane % CL;
+  const double* MS = P.Msl + (size_t)k * h * h;
+  double A[RA][CB];
+#pragma unroll
+  for (int a = 0; a < RA; ++a)
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      const int r = ly + RL * a, c = lx + CL * b;
+      A[a][b] = (act & (r < h) & (c < h)) ? MS[r + (size_t)h * c] : 0.0;
+    }
+  unsigned long long used = 0ull;
+  for (int kk = 0; kk < h; ++kk) {
+    const int buf = kk & 1, kt = kk % CL, kb = kk / CL;
+    const bool own = act & (lx == kt);
+    double bv = -1.0;
+    int bi = NRW;
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      if (b != kb) continue;  // uniform
+#pragma unroll
+      for (int a = 0; a < RA; ++a) {
+        const int r = ly + RL * a;
+        const double v = fabs(A[a][b]);
+        const bool t = own & (r < h) & !((used >> (r & 63)) & 1ull) & (v > bv);
+        bv = t ? v : bv;
+        bi = t ? r : bi;
+      }
+    }
+    int p = NRW;
+    {
+      double pv = -1.0;
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const double v = __shfl(bv, kt + CL * t, 64);
+        const int i = __shfl(bi, kt + CL * t, 64);
+        const bool tk = (v > pv) | ((v == pv) & (i < p));
+        pv = tk ? v : pv;
+        p = tk ? i : p;
+      }
+    }
+    p = __builtin_amdgcn_readfirstlane(p);
+    if (p >= h) p = __builtin_ctzll(~used);  // NaN entries only
+    used |= 1ull << p;
+    if (lane == 0) piv[kk] = p;
+    const int pa = p / RL;
+    const bool prow = act & (ly == p % RL);
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      if (b != kb) continue;  // uniform
+#pragma unroll
+      for (int a = 0; a < RA; ++a) {
+        if (own) {
+          const bool isp = (a == pa) & prow;
+          colr[buf][ly + RL * a] = isp ? 0.0 : A[a][b];
+          if (isp) pivel[buf] = A[a][b];
+          A[a][b] = isp ? 1.0 : 0.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      if (a != pa) continue;  // uniform
+      if (prow) {
+#pragma unroll
+        for (int b = 0; b < CB; ++b) rowr[buf][lx + CL * b] = A[a][b];
+      }
+    }
+    __syncthreads();  // one wave: orders the LDS stores before the loads
+    const double ip = 1.0 / pivel[buf];
+    double nr[CB];
+#pragma unroll
+    for (int b = 0; b < CB; ++b) nr[b] = ip * rowr[buf][lx + CL * b];
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      const double c = colr[buf][ly + RL * a];
+#pragma unroll
+      for (int b = 0; b < CB; ++b) A[a][b] = fma(-c, nr[b], A[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      if (a != pa) continue;  // uniform
+      if (prow) {
+#pragma unroll
+        for (int b = 0; b < CB; ++b) A[a][b] *= ip;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < h; i += 64) pos[piv[i]] = i;
+  __syncthreads();
+  double* Bk = P.Binv + (size_t)k * h * h;
+#pragma unroll
+  for (int a = 0; a < RA; ++a)
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      const int r = ly + RL * a, c = lx + CL * b;
+      if (act & (r < h) & (c < h)) Bk[(size_t)pos[r] * h + piv[c]] = A[a][b];
+    }
+}
+
 // The rest of the weights operator from its Ad rows (written by
-// rrq_quat_inverse_kernel) and B = (Msl^T)^{-1} (from rrq_msl_inverse_kernel):
-// C = B Fn^T, As rows G(j, c) = (1/alpha) sum_i C(j,i) G(h+i, c) for c < 4h,
-// G(j, 4h+r) = B(j, r) fac/alpha, and G(h+j, 4h+r) = 0.  A thread makes the four
-// entries c = q*h + kq (q = 0..3) of one row j.
+// rrq_quat_inverse_kernel) and B (from rrq_real_inverse_kernel): C = B Fn^T,
+// As rows G(j, c) = (1/alpha) sum_i C(j,i) G(h+i, c) for c < 4h,
+// G(j, 4h+r) = B(j, r) fac/alpha, and G(h+j, 4h+r) = 0.  Every operand is staged
+// in LDS (Fn first, then the Ad block in its place); a thread makes a tile of
+// 4 columns c x 6 rows j.
+constexpr int kAsC = 4, kAsJ = 6;
+size_t rrq_solve_as_smem(int h) { return sizeof(double) * (size_t)6 * h * h; }
+
 __global__ void __launch_bounds__(256) rrq_solve_as_kernel(RRQTables T, RRQPanels P, int npan,
                                                            const int* __restrict__ nrrq) {
   const int k = blockIdx.x;
   if (nrrq && nrrq[k] == 0) return;  // null: every panel
   const int h = T.h, H4 = 4 * h, m = 2 * h;
   extern __shared__ double dyn[];
-  double* Bt = dyn;                 // Bt[r*h + j] = B(j, r)
-  double* Ct = dyn + (size_t)h * h; // Ct[i*h + j] = C(j, i)
+  double* Bt = dyn;                      // Bt[r*h + j] = B(j, r)
+  double* Ct = Bt + (size_t)h * h;       // Ct[i*h + j] = C(j, i)
+  double* Ad = Ct + (size_t)h * h;       // Ad[i*4h + c] = G(h+i, c); Fn before it
+  double* Fs = Ad;                       // Fs[i + h*r] = Fn(i, r)
   const double* Bk = P.Binv + (size_t)k * h * h;
   const double* FN = P.Fn + (size_t)k * h * h;
   double* G = P.G + (size_t)k * m * 5 * h;
   const double alpha = P.frame[(size_t)k * 16 + 12];
   const double fac = 1.0 / (16.0 * atan(1.0)), ia = 1.0 / alpha;
-  for (int e = threadIdx.x; e < h * h; e += blockDim.x) Bt[(size_t)(e % h) * h + e / h] = Bk[e];
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+    Bt[(size_t)(e % h) * h + e / h] = Bk[e];
+    Fs[e] = FN[e];
+  }
   __syncthreads();
   for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
     const int j = e % h, i = e / h;
     double s = 0.0;
-    for (int r = 0; r < h; ++r) s = fma(Bt[(size_t)r * h + j], FN[i + (size_t)h * r], s);
+    for (int r = 0; r < h; ++r) s = fma(Bt[(size_t)r * h + j], Fs[i + (size_t)h * r], s);
     Ct[(size_t)i * h + j] = s;
   }
-  __syncthreads();
+  // W block and the zero block under it
   for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
-    const int j = e % h, kq = e / h;
-    const double* g0 = G + (size_t)kq * m + h;
-    const size_t st = (size_t)h * m;  // from c to c + h
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (int i = 0; i < h; ++i) {
-      const double cv = Ct[(size_t)i * h + j];
-      s0 = fma(cv, g0[i], s0);
-      s1 = fma(cv, g0[st + i], s1);
-      s2 = fma(cv, g0[2 * st + i], s2);
-      s3 = fma(cv, g0[3 * st + i], s3);
-    }
-    double* o = G + (size_t)kq * m + j;
-    o[0] = s0 * ia;
-    o[st] = s1 * ia;
-    o[2 * st] = s2 * ia;
-    o[3 * st] = s3 * ia;
-    double* w = G + (size_t)(H4 + kq) * m;
-    w[j] = Bt[(size_t)kq * h + j] * fac * ia;
+    const int j = e % h, r = e / h;
+    double* w = G + (size_t)(H4 + r) * m;
+    w[j] = Bt[(size_t)r * h + j] * fac * ia;
     w[h + j] = 0.0;
+  }
+  __syncthreads();  // Fn no longer needed: the Ad block goes in its place
+  for (int e = threadIdx.x; e < H4 * h; e += blockDim.x) {
+    const int c = e / h, i = e % h;
+    Ad[(size_t)i * H4 + c] = G[(size_t)c * m + h + i];
+  }
+  __syncthreads();
+  const int njg = (h + kAsJ - 1) / kAsJ, ncg = H4 / kAsC;  // 4h is a multiple of 4
+  for (int it = threadIdx.x; it < njg * ncg; it += blockDim.x) {
+    const int cg = it % ncg, jg = it / ncg;
+    const int j0 = jg * kAsJ, c0 = cg * kAsC;
+    double acc[kAsC][kAsJ];
+#pragma unroll
+    for (int u = 0; u < kAsC; ++u)
+#pragma unroll
+      for (int v = 0; v < kAsJ; ++v) acc[u][v] = 0.0;
+    for (int i = 0; i < h; ++i) {
+      double cv[kAsJ], av[kAsC];
+#pragma unroll
+      for (int v = 0; v < kAsJ; ++v) cv[v] = j0 + v < h ? Ct[(size_t)i * h + j0 + v] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kAsC; ++u) av[u] = Ad[(size_t)i * H4 + c0 + u];
+#pragma unroll
+      for (int u = 0; u < kAsC; ++u)
+#pragma unroll
+        for (int v = 0; v < kAsJ; ++v) acc[u][v] = fma(cv[v], av[u], acc[u][v]);
+    }
+#pragma unroll
+    for (int u = 0; u < kAsC; ++u)
+#pragma unroll
+      for (int v = 0; v < kAsJ; ++v)
+        if (j0 + v < h) G[(size_t)(c0 + u) * m + j0 + v] = acc[u][v] * ia;
   }
 }
 
@@ -1132,26 +1344,27 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
     default: break;
   }
 #undef QOL_SOLVE_MFMA
-  // patch orders up to 8 (h <= 40): the real inverse, the one-wave quaternion
-  // inverse, then the As rows; QOL_SOLVE_FUSED=1 keeps the single fused kernel
+  // patch orders up to 8: the real and the quaternion inverse with the matrix in
+  // registers, then the As rows; QOL_SOLVE_FUSED=1 keeps the single fused kernel
   const char* fu = std::getenv("QOL_SOLVE_FUSED");
-  if (h <= 40 && !(fu && fu[0] == '1')) {
-    const int na = (h + 7) / 8;
-    if (rpt <= 1 && cpt <= 2)
-      hipLaunchKernelGGL((rrq_msl_inverse_kernel<1, 2>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
-    else if (rpt <= 2 && cpt <= 4)
-      hipLaunchKernelGGL((rrq_msl_inverse_kernel<2, 4>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
-    else
-      hipLaunchKernelGGL((rrq_msl_inverse_kernel<3, 5>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);
-    switch (na) {
-      case 1: hipLaunchKernelGGL((rrq_quat_inverse_kernel<1>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
-      case 2: hipLaunchKernelGGL((rrq_quat_inverse_kernel<2>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
-      case 3: hipLaunchKernelGGL((rrq_quat_inverse_kernel<3>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
-      case 4: hipLaunchKernelGGL((rrq_quat_inverse_kernel<4>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
-      default: hipLaunchKernelGGL((rrq_quat_inverse_kernel<5>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); break;
-    }
-    hipLaunchKernelGGL(rrq_solve_as_kernel, dim3(npan), dim3(256), sizeof(double) * 2 * (size_t)h * h, s, T, P, npan,
-                       nrrq);
+  if (h <= 36 && !(fu && fu[0] == '1')) {
+    // lane grid per order (rows RL x RA, slots CL x CB)
+#define QOL_QINV(RL_, RA_, CL_, CB_, NW_)                                                                \
+  hipLaunchKernelGGL((rrq_real_inverse_kernel<RL_, RA_, CL_, CB_>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); \
+  hipLaunchKernelGGL((rrq_quat_inverse_kernel<RL_, RA_, CL_, CB_, NW_>), dim3(npan), dim3(64 * NW_), 0, s, T, P, \
+                     npan, nrrq)
+    if (h <= 3) { QOL_QINV(3, 1, 3, 1, 1); }
+    else if (h <= 6) { QOL_QINV(6, 1, 6, 1, 1); }
+    else if (h <= 10) { QOL_QINV(5, 2, 10, 1, 1); }
+    else if (h <= 15) { QOL_QINV(3, 5, 15, 1, 1); }
+    else if (h <= 21) { QOL_QINV(3, 7, 21, 1, 1); }
+    else if (h <= 28) { QOL_QINV(4, 7, 16, 2, 2); }
+    else { QOL_QINV(6, 6, 10, 4, 2); }
+#undef QOL_QINV
+    const size_t las = rrq_solve_as_smem(h);
+    if (las > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)rrq_solve_as_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)las);
+    hipLaunchKernelGGL(rrq_solve_as_kernel, dim3(npan), dim3(256), las, s, T, P, npan, nrrq);
     return;
   }
   if (rpt <= 3 && cpt <= 5) {
@@ -1351,7 +1564,8 @@ __device__ double solid_angle_edge(const double* cf, const double* tgl, const do
 // Gauss-Newton projection of x (original coordinates) onto the patch, started
 // at the nearest node (ssx: the panel's nodes, x(3,h)); returns the signed
 // distance along the uv normal and that normal
-__device__ double project(const double* cx, int h, int korder, const double* ssx, const double* uvn,
+__device__ double project(const double* cx, const double* __restrict__ kt, int h, int korder,
+                          const double* ssx, const double* uvn,
                           const double* x, double* nstar) {
   double best = 1e300, u = 1.0 / 3.0, v = 1.0 / 3.0;
   for (int i = 0; i < h; ++i) {
@@ -1361,7 +1575,7 @@ __device__ double project(const double* cx, int h, int korder, const double* ssx
   }
   double y[3], yu[3], yv[3];
   for (int it = 0; it < 40; ++it) {
-    geom_eval(cx, h, korder, u, v, y, yu, yv);
+    geom_eval(cx, kt, h, korder, u, v, y, yu, yv);
     const double r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
     const double a11 = yu[0] * yu[0] + yu[1] * yu[1] + yu[2] * yu[2];
     const double a12 = yu[0] * yv[0] + yu[1] * yv[1] + yu[2] * yv[2];
@@ -1382,7 +1596,7 @@ __device__ double project(const double* cx, int h, int korder, const double* ssx
     u = un; v = vn;
     if (step < 1e-15) break;
   }
-  geom_eval(cx, h, korder, u, v, y, yu, yv);
+  geom_eval(cx, kt, h, korder, u, v, y, yu, yv);
   double nn[3];
   cross3(yu, yv, nn);
   const double l = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
@@ -1418,7 +1632,7 @@ __global__ void __launch_bounds__(256) rrq_project_kernel(RRQTables T, RRQPanels
   const size_t g = (size_t)tidx[pair];
   const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
   double ns[3];
-  const double d = project(P.cx + (size_t)k * 3 * h, h, T.p - 1, sx + (size_t)k * 3 * h, T.uvn, x, ns);
+  const double d = project(P.cx + (size_t)k * 3 * h, T.ktab, h, T.p - 1, sx + (size_t)k * 3 * h, T.uvn, x, ns);
   double side;
   if (fabs(d) > side_tol * fr[14]) side = d > 0.0 ? 1.0 : -1.0;
   else side = (exterior ? 1.0 : -1.0) * fr[13];
