@@ -73,8 +73,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
   for (int ch = 0; ch < A.nchunk; ++ch) {
     const int* C = A.chunks + 5 * ch;
     const int l = C[0], c0 = C[1], c1 = C[2], sb0 = C[3], sb1 = C[4];
-    const int n = A.lev_n[l], off = A.lev_off[l];
-    const double* IM = A.interp + (size_t)h * off;  // column-major n x h
+    const int off = A.lev_off[l];
     const int i = c0 + (int)threadIdx.x;
     if (i < c1) {
       double geo[kLP][9];
@@ -82,21 +81,28 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
       for (int pp = 0; pp < kLP; ++pp)
 #pragma unroll
         for (int c = 0; c < 9; ++c) geo[pp][c] = 0.0;
-      // sequential fma in q for every accumulator (the oracle's order); the next
-      // interpolation entry is fetched one step ahead and the 36 LDS values of
-      // a step are read together
-      double vnext = IM[i];
-      for (int q = 0; q < h; ++q) {
-        const double v = vnext;
-        if (q + 1 < h) vnext = IM[i + (size_t)n * (q + 1)];
-        const double* sq = srcT + (size_t)q * NV;
-        double sv[NV];
+      // sequential fma in q for every accumulator (the oracle's order).  The
+      // node's interpolation row comes from the row-major copy (zero-padded to
+      // whole 16-entry blocks) 8 entries at a time, the loads of a block issued
+      // together, and the 36 LDS values of a step are read together.
+      const double2* row = (const double2*)(A.imp + (size_t)(off + i) * A.pad);
+      for (int q0 = 0; q0 < h; q0 += 8) {
+        double2 iv[4];
 #pragma unroll
-        for (int c = 0; c < NV; ++c) sv[c] = sq[c];
+        for (int u = 0; u < 4; ++u) iv[u] = row[q0 / 2 + u];
 #pragma unroll
-        for (int pp = 0; pp < kLP; ++pp)
+        for (int u = 0; u < 8; ++u) {
+          if (q0 + u >= h) break;
+          const double v = (u & 1) ? iv[u >> 1].y : iv[u >> 1].x;
+          const double* sq = srcT + (size_t)(q0 + u) * NV;
+          double sv[NV];
 #pragma unroll
-          for (int c = 0; c < 9; ++c) geo[pp][c] = fma(v, sv[pp * 9 + c], geo[pp][c]);
+          for (int c = 0; c < NV; ++c) sv[c] = sq[c];
+#pragma unroll
+          for (int pp = 0; pp < kLP; ++pp)
+#pragma unroll
+            for (int c = 0; c < 9; ++c) geo[pp][c] = fma(v, sv[pp * 9 + c], geo[pp][c]);
+        }
       }
       const int li = i - c0;
 #pragma unroll

@@ -1108,21 +1108,96 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
   }
   __syncthreads();
   for (int i = threadIdx.x; i < h; i += 64 * NW) pos[piv[i]] = i;
-  __syncthreads();
+  // the rest of the weights operator G (stored transposed, G(row, c) at c*2h + row):
+  //   Ad rows  G(h+j, q*h+k) = (X_kj)_q fac
+  //   As rows  G(j, q*h+k)   = (fac/alpha) sum_i C(j,i) (X_ki)_q,  C = B Fn^T
+  //   W block  G(j, 4h+r)    = (fac/alpha) B(j,r),  G(h+j, 4h+r) = 0
+  // with B = (Msl^T)^{-1} from rrq_real_inverse_kernel.  LDS: C, then Bt and Fn
+  // (for C) or one component of X at a time.
+  extern __shared__ double dyn[];
+  double* Ct = dyn;                    // Ct[i*h + j] = C(j, i)
+  double* Bt = dyn + (size_t)h * h;    // Bt[r*h + j] = B(j, r)
+  double* Fs = Bt + (size_t)h * h;     // Fs[i + h*r] = Fn(i, r)
+  double* Xq = Bt;                     // Xq[i*h + k] = (X_ki)_q
   const double fac = 1.0 / (16.0 * atan(1.0));
+  const double fa = fac / P.frame[(size_t)k * 16 + 12];
   double* G = P.G + (size_t)k * m * 5 * h;
-#pragma unroll
-  for (int a = 0; a < RA; ++a)
-#pragma unroll
-    for (int b = 0; b < CW; ++b) {
-      const int r = ly + RL * a, c = lx + CL * (wv * CW + b);
-      if (act & (r < h) & (c < h)) {
-        const int kx = pos[r], j = piv[c];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) G[(size_t)(q * h + kx) * m + h + j] = A[a][b][q] * fac;
-      }
+  {
+    const double* Bk = P.Binv + (size_t)k * h * h;
+    const double* FN = P.Fn + (size_t)k * h * h;
+    for (int e = threadIdx.x; e < h * h; e += 64 * NW) {
+      Bt[(size_t)(e % h) * h + e / h] = Bk[e];
+      Fs[e] = FN[e];
     }
+  }
+  __syncthreads();
+  {
+    const int nig = (h + 3) / 4;
+    for (int it = threadIdx.x; it < h * nig; it += 64 * NW) {
+      const int j = it % h, i0 = 4 * (it / h);
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int r = 0; r < h; ++r) {
+        const double bv = Bt[(size_t)r * h + j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = fma(bv, Fs[(i0 + u < h ? i0 + u : h - 1) + (size_t)h * r], acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i0 + u < h) Ct[(size_t)(i0 + u) * h + j] = acc[u];
+    }
+    for (int e = threadIdx.x; e < h * h; e += 64 * NW) {
+      const int j = e % h, r = e / h;
+      double* w = G + (size_t)(4 * h + r) * m;
+      w[j] = Bt[(size_t)r * h + j] * fa;
+      w[h + j] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    __syncthreads();  // Bt/Fn (q = 0) or the previous component no longer read
+#pragma unroll
+    for (int a = 0; a < RA; ++a)
+#pragma unroll
+      for (int b = 0; b < CW; ++b) {
+        const int r = ly + RL * a, c = lx + CL * (wv * CW + b);
+        if (act & (r < h) & (c < h)) {
+          const int kx = pos[r], j = piv[c];
+          Xq[(size_t)j * h + kx] = A[a][b][q];
+          G[(size_t)(q * h + kx) * m + h + j] = A[a][b][q] * fac;
+        }
+      }
+    __syncthreads();
+    // As rows of component q: tiles of 3 rows j x 4 columns k
+    const int njg = (h + 2) / 3, nkg = (h + 3) / 4;
+    for (int it = threadIdx.x; it < njg * nkg; it += 64 * NW) {
+      const int j0 = 3 * (it % njg), k0 = 4 * (it / njg);
+      double acc[3][4];
+#pragma unroll
+      for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[v][u] = 0.0;
+      for (int i = 0; i < h; ++i) {
+        double cv[3], xv[4];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) cv[v] = Ct[(size_t)i * h + (j0 + v < h ? j0 + v : h - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = Xq[(size_t)i * h + (k0 + u < h ? k0 + u : h - 1)];
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[v][u] = fma(cv[v], xv[u], acc[v][u]);
+      }
+#pragma unroll
+      for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if ((j0 + v < h) & (k0 + u < h)) G[(size_t)(q * h + k0 + u) * m + j0 + v] = acc[v][u] * fa;
+    }
+  }
 }
+
+// dynamic LDS of rrq_quat_inverse_kernel: C plus Bt and Fn (or one component of X)
+size_t rrq_quat_inverse_smem(int h) { return sizeof(double) * (size_t)3 * h * h; }
 
 // B = (Msl^T)^{-1} by Gauss-Jordan in the registers of one wave per panel, the
 // real counterpart of rrq_quat_inverse_kernel (same lane grid, slot reuse,
@@ -1241,82 +1316,6 @@ __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPa
     }
 }
 
-// The rest of the weights operator from its Ad rows (written by
-// rrq_quat_inverse_kernel) and B (from rrq_real_inverse_kernel): C = B Fn^T,
-// As rows G(j, c) = (1/alpha) sum_i C(j,i) G(h+i, c) for c < 4h,
-// G(j, 4h+r) = B(j, r) fac/alpha, and G(h+j, 4h+r) = 0.  Every operand is staged
-// in LDS (Fn first, then the Ad block in its place); a thread makes a tile of
-// 4 columns c x 6 rows j.
-constexpr int kAsC = 4, kAsJ = 6;
-size_t rrq_solve_as_smem(int h) { return sizeof(double) * (size_t)6 * h * h; }
-
-__global__ void __launch_bounds__(256) rrq_solve_as_kernel(RRQTables T, RRQPanels P, int npan,
-                                                           const int* __restrict__ nrrq) {
-  const int k = blockIdx.x;
-  if (nrrq && nrrq[k] == 0) return;  // null: every panel
-  const int h = T.h, H4 = 4 * h, m = 2 * h;
-  extern __shared__ double dyn[];
-  double* Bt = dyn;                      // Bt[r*h + j] = B(j, r)
-  double* Ct = Bt + (size_t)h * h;       // Ct[i*h + j] = C(j, i)
-  double* Ad = Ct + (size_t)h * h;       // Ad[i*4h + c] = G(h+i, c); Fn before it
-  double* Fs = Ad;                       // Fs[i + h*r] = Fn(i, r)
-  const double* Bk = P.Binv + (size_t)k * h * h;
-  const double* FN = P.Fn + (size_t)k * h * h;
-  double* G = P.G + (size_t)k * m * 5 * h;
-  const double alpha = P.frame[(size_t)k * 16 + 12];
-  const double fac = 1.0 / (16.0 * atan(1.0)), ia = 1.0 / alpha;
-  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
-    Bt[(size_t)(e % h) * h + e / h] = Bk[e];
-    Fs[e] = FN[e];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
-    const int j = e % h, i = e / h;
-    double s = 0.0;
-    for (int r = 0; r < h; ++r) s = fma(Bt[(size_t)r * h + j], Fs[i + (size_t)h * r], s);
-    Ct[(size_t)i * h + j] = s;
-  }
-  // W block and the zero block under it
-  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
-    const int j = e % h, r = e / h;
-    double* w = G + (size_t)(H4 + r) * m;
-    w[j] = Bt[(size_t)r * h + j] * fac * ia;
-    w[h + j] = 0.0;
-  }
-  __syncthreads();  // Fn no longer needed: the Ad block goes in its place
-  for (int e = threadIdx.x; e < H4 * h; e += blockDim.x) {
-    const int c = e / h, i = e % h;
-    Ad[(size_t)i * H4 + c] = G[(size_t)c * m + h + i];
-  }
-  __syncthreads();
-  const int njg = (h + kAsJ - 1) / kAsJ, ncg = H4 / kAsC;  // 4h is a multiple of 4
-  for (int it = threadIdx.x; it < njg * ncg; it += blockDim.x) {
-    const int cg = it % ncg, jg = it / ncg;
-    const int j0 = jg * kAsJ, c0 = cg * kAsC;
-    double acc[kAsC][kAsJ];
-#pragma unroll
-    for (int u = 0; u < kAsC; ++u)
-#pragma unroll
-      for (int v = 0; v < kAsJ; ++v) acc[u][v] = 0.0;
-    for (int i = 0; i < h; ++i) {
-      double cv[kAsJ], av[kAsC];
-#pragma unroll
-      for (int v = 0; v < kAsJ; ++v) cv[v] = j0 + v < h ? Ct[(size_t)i * h + j0 + v] : 0.0;
-#pragma unroll
-      for (int u = 0; u < kAsC; ++u) av[u] = Ad[(size_t)i * H4 + c0 + u];
-#pragma unroll
-      for (int u = 0; u < kAsC; ++u)
-#pragma unroll
-        for (int v = 0; v < kAsJ; ++v) acc[u][v] = fma(cv[v], av[u], acc[u][v]);
-    }
-#pragma unroll
-    for (int u = 0; u < kAsC; ++u)
-#pragma unroll
-      for (int v = 0; v < kAsJ; ++v)
-        if (j0 + v < h) G[(size_t)(c0 + u) * m + j0 + v] = acc[u][v] * ia;
-  }
-}
-
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s) {
   const int h = T.h;
   const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
@@ -1345,14 +1344,15 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
   }
 #undef QOL_SOLVE_MFMA
   // patch orders up to 8: the real and the quaternion inverse with the matrix in
-  // registers, then the As rows; QOL_SOLVE_FUSED=1 keeps the single fused kernel
+  // registers (the latter also forms G); QOL_SOLVE_FUSED=1 keeps the single fused
+  // kernel
   const char* fu = std::getenv("QOL_SOLVE_FUSED");
   if (h <= 36 && !(fu && fu[0] == '1')) {
     // lane grid per order (rows RL x RA, slots CL x CB)
 #define QOL_QINV(RL_, RA_, CL_, CB_, NW_)                                                                \
   hipLaunchKernelGGL((rrq_real_inverse_kernel<RL_, RA_, CL_, CB_>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); \
-  hipLaunchKernelGGL((rrq_quat_inverse_kernel<RL_, RA_, CL_, CB_, NW_>), dim3(npan), dim3(64 * NW_), 0, s, T, P, \
-                     npan, nrrq)
+  hipLaunchKernelGGL((rrq_quat_inverse_kernel<RL_, RA_, CL_, CB_, NW_>), dim3(npan), dim3(64 * NW_),             \
+                     rrq_quat_inverse_smem(h), s, T, P, npan, nrrq)
     if (h <= 3) { QOL_QINV(3, 1, 3, 1, 1); }
     else if (h <= 6) { QOL_QINV(6, 1, 6, 1, 1); }
     else if (h <= 10) { QOL_QINV(5, 2, 10, 1, 1); }
@@ -1361,10 +1361,6 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
     else if (h <= 28) { QOL_QINV(4, 7, 16, 2, 2); }
     else { QOL_QINV(6, 6, 10, 4, 2); }
 #undef QOL_QINV
-    const size_t las = rrq_solve_as_smem(h);
-    if (las > 64 * 1024)
-      (void)hipFuncSetAttribute((const void*)rrq_solve_as_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)las);
-    hipLaunchKernelGGL(rrq_solve_as_kernel, dim3(npan), dim3(256), las, s, T, P, npan, nrrq);
     return;
   }
   if (rpt <= 3 && cpt <= 5) {
