@@ -25,7 +25,7 @@ namespace qol {
 constexpr int kLP = 4;
 constexpr int kChunk = 256;
 
-size_t level_geom_smem(int h) { return sizeof(double) * (9 * (size_t)h * kLP + (size_t)kLP * 4 * kChunk); }
+size_t level_geom_smem(int h) { return sizeof(double) * (size_t)kLP * 4 * kChunk; }
 int level_geom_blocks(int npan) { return (npan + kLP - 1) / kLP; }
 
 std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host) {
@@ -52,28 +52,12 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
   const int h = A.h, k0 = blockIdx.x * kLP;
   const int np_ = min(kLP, npan - k0);
   const size_t nt_ = A.ntot;
-  constexpr int NV = kLP * 9;
-  double* srcT = smem;                          // [h][kLP][9]: all values of panel node q together
-  double* cn = smem + (size_t)NV * h;           // [kLP][4][kChunk]: x, y, z, w of the chunk's nodes
-  for (int e = threadIdx.x; e < kLP * h; e += blockDim.x) {
-    const int pp = e / h, i = e % h;
-    double* S = srcT + (size_t)i * NV + pp * 9;
-    if (pp < np_) {
-      const size_t g = (size_t)(k0 + pp) * h + i;
-      for (int c = 0; c < 3; ++c) {
-        S[c] = sx[3 * g + c];
-        S[3 + c] = rts[3 * g + c];
-        S[6 + c] = rps[3 * g + c];
-      }
-    } else {
-      for (int c = 0; c < 9; ++c) S[c] = 0.0;
-    }
-  }
-  __syncthreads();
+  double* cn = smem;  // [kLP][4][kChunk]: x, y, z, w of the chunk's nodes
   for (int ch = 0; ch < A.nchunk; ++ch) {
     const int* C = A.chunks + 5 * ch;
     const int l = C[0], c0 = C[1], c1 = C[2], sb0 = C[3], sb1 = C[4];
-    const int off = A.lev_off[l];
+    const int n = A.lev_n[l], off = A.lev_off[l];
+    const double* IM = A.interp + (size_t)h * off;  // column-major n x h
     const int i = c0 + (int)threadIdx.x;
     if (i < c1) {
       double geo[kLP][9];
@@ -81,27 +65,24 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
       for (int pp = 0; pp < kLP; ++pp)
 #pragma unroll
         for (int c = 0; c < 9; ++c) geo[pp][c] = 0.0;
-      // sequential fma in q for every accumulator (the oracle's order).  The
-      // node's interpolation row comes from the row-major copy (zero-padded to
-      // whole 16-entry blocks) 8 entries at a time, the loads of a block issued
-      // together, and the 36 LDS values of a step are read together.
-      const double2* row = (const double2*)(A.imp + (size_t)(off + i) * A.pad);
-      for (int q0 = 0; q0 < h; q0 += 8) {
-        double2 iv[4];
+      // sequential fma in q for every accumulator (the oracle's order); the next
+      // interpolation entry is fetched one step ahead.  The panel values of a
+      // step are the same for the whole workgroup: read with uniform addresses
+      // they come through the scalar cache and feed the FMAs from scalar
+      // registers (no LDS traffic).
+      double vnext = IM[i];
+      for (int q = 0; q < h; ++q) {
+        const double v = vnext;
+        if (q + 1 < h) vnext = IM[i + (size_t)n * (q + 1)];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) iv[u] = row[q0 / 2 + u];
+        for (int pp = 0; pp < kLP; ++pp) {
+          const size_t g = (size_t)(k0 + (pp < np_ ? pp : 0)) * h + q;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          if (q0 + u >= h) break;
-          const double v = (u & 1) ? iv[u >> 1].y : iv[u >> 1].x;
-          const double* sq = srcT + (size_t)(q0 + u) * NV;
-          double sv[NV];
-#pragma unroll
-          for (int c = 0; c < NV; ++c) sv[c] = sq[c];
-#pragma unroll
-          for (int pp = 0; pp < kLP; ++pp)
-#pragma unroll
-            for (int c = 0; c < 9; ++c) geo[pp][c] = fma(v, sv[pp * 9 + c], geo[pp][c]);
+          for (int c = 0; c < 3; ++c) {
+            geo[pp][c] = fma(v, sx[3 * g + c], geo[pp][c]);
+            geo[pp][3 + c] = fma(v, rts[3 * g + c], geo[pp][3 + c]);
+            geo[pp][6 + c] = fma(v, rps[3 * g + c], geo[pp][6 + c]);
+          }
         }
       }
       const int li = i - c0;
