@@ -1797,12 +1797,21 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
         for (int g = 0; g < kTG; ++g)
 #pragma unroll
           for (int c = 0; c < 10; ++c) acc[g][c] = 0.0;
-#pragma unroll 2
-        for (int ib = 0; ib < nb; ++ib) {
-          const double* Bp = basis + (size_t)ib * 12 * ncoef + kk;
-          double B[12];
+        // the basis of kIB nodes is loaded together (more loads in flight)
+        constexpr int kIB = 3;
+        for (int ib0 = 0; ib0 < nb; ib0 += kIB) {
+          double Bv[kIB][12];
 #pragma unroll
-          for (int c = 0; c < 12; ++c) B[c] = Bp[c * ncoef];
+          for (int u = 0; u < kIB; ++u) {
+            const double* Bp = basis + (size_t)min(ib0 + u, nb - 1) * 12 * ncoef + kk;
+#pragma unroll
+            for (int c = 0; c < 12; ++c) Bv[u][c] = Bp[c * ncoef];
+          }
+#pragma unroll
+          for (int u = 0; u < kIB; ++u) {
+          const int ib = ib0 + u;
+          if (ib >= nb) break;
+          const double* B = Bv[u];
 #pragma unroll
           for (int g = 0; g < kTG; ++g) {
             if (t0 + g >= nt) break;
@@ -1820,6 +1829,7 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
             A[7] = fma(z2, B[7], fma(-z0, B[11], A[7]));
             A[8] = fma(z0, B[8], fma(-z1, B[6], A[8]));
             A[9] = fma(z0, B[9], fma(-z1, B[7], A[9]));
+          }
           }
         }
 #pragma unroll
