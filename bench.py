@@ -60,13 +60,37 @@ def kernel_flops(p, npan, counts, nrrq, nrrq_panels):
         # per level node: 19 for the S/D entry (incl. one rsqrt), 4h for the two contractions
         "direct": ("direct_mfma_kernel", float(sum(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h)
                                                    for l in range(4)))),
-        # quaternion Gauss-Jordan 32h^3, real Gauss-Jordan 2h^3, B Fn^T 2h^3, C G1 8h^3
-        "rrq_solve": ("rrq_panel_solve_kernel", 44.0 * h ** 3 * nrrq_panels),
+        # real Gauss-Jordan 2h^3 (rrq_real_inverse_kernel); quaternion Gauss-Jordan
+        # 32h^3, C = B Fn^T 2h^3 and the As rows C G1 8h^3 (rrq_quat_inverse_kernel)
+        "rrq_solve": ("rrq_real_inverse_kernel+rrq_quat_inverse_kernel", 44.0 * h ** 3 * nrrq_panels),
         # boundary moments 48 nb ncoef, degree recursion ~200 ncoef
         "rrq_moments": ("rrq_target_kernel", float(nrrq) * (48 * nb * ncoef + 200 * ncoef)),
         # W = G V: 2 (2h)(5h) per RRQ pair
         "rrq_weights": ("rrq_weights_kernel", float(nrrq) * 20 * h * h),
     }
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_p8_48x120.json")
+
+
+def measured_traffic(kernel, a):
+    """HBM bytes per launch of `kernel` ('+'-joined names are summed) from the
+    FETCH_SIZE / WRITE_SIZE counter runs of this workload (scaled by a plain copy
+    of known size), or None when no such record exists for this configuration."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except OSError:
+        return None
+    if rec.get("workload") != [a.mp, a.np, a.order, a.targets]:
+        return None
+    total = 0.0
+    for name in kernel.split("+"):
+        k = rec["kernels"].get(name)
+        if k is None:
+            return None
+        total += k["bytes"]
+    return total
 
 
 def cpu_baseline(geo, tx, offs, idx, ub, ubn, sample_pairs):
@@ -187,7 +211,7 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     achieved = kernels[dom]["tflops"]
     roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(dom, a),
             "flops_per_launch": kernels[dom]["gflop"] * 1e9, "ms_per_launch": kernels[dom]["ms"]}
 
     base = None
