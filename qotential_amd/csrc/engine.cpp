@@ -138,7 +138,7 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
   }
   for (auto& st : tev)
     for (auto& e : st) HIPCHK(hipEventCreate(&e));
-  for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_solve, &ev_edge})
+  for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge})
     HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   tab = std::make_unique<DeviceTables>(p, stream);
   h = tab->h;
@@ -166,7 +166,7 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
 Engine::~Engine() {
   for (auto& st : tev)
     for (auto& e : st) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {ev_start, ev_lists, ev_direct, ev_setup, ev_solve, ev_edge}) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ev_start, ev_lists, ev_direct, ev_setup, ev_system, ev_solve, ev_edge}) (void)hipEventDestroy(e);
   if (s1 && s1 != stream) (void)hipStreamDestroy(s1);
   if (s2 && s2 != stream) (void)hipStreamDestroy(s2);
   if (s3 && s3 != stream) (void)hipStreamDestroy(s3);
@@ -317,11 +317,12 @@ void Engine::eval_levels_only() {
 }
 
 // Stages and streams (all at the default priority).  The per-panel RRQ set-up
-// and solve need only the surface, so they run for every panel on s2 from the
-// start, beside the level geometry and classification on the main stream.
-// After the pair lists: the direct levels on s1 (matrix cores) and the per-pair
-// projection, root/SSQ and solid angle on s3 (after the set-up); the
-// moment/weights kernel joins s2 and s3, and the main stream joins s1 last.
+// (frame kernel, then system kernel) and solve need only the surface, so they
+// run for every panel on s2 from the start, beside the level geometry and
+// classification on the main stream.  After the pair lists: the direct levels on
+// s1 (matrix cores) and the per-pair projection, root/SSQ and solid angle on s3
+// (after the frame kernel); the moments join s3 and the system kernel, the
+// weights the solve, and the main stream joins s1 last.
 // tev[i] brackets stage i: levels, classify+lists, direct, rrq setup, rrq solve,
 // projection, root + SSQ (solid angle from tev[8][0]), moments, -, weights.
 void Engine::eval(int exterior, double side_tol, bool want_omega) {
@@ -357,13 +358,18 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   // s2: RRQ set-up and solve for every panel
   HIPCHK(hipStreamWaitEvent(s2, ev_start, 0));
   rec(3, 0, s2);
-  const size_t lds1 = rrq_setup_smem(h, nq, ncoef);
-  set_lds((const void*)rrq_panel_setup_kernel, lds1);
-  hipLaunchKernelGGL(rrq_panel_setup_kernel, dim3(npan), dim3(256), lds1, s2, tab->RT, RP, npan, (const int*)nullptr,
+  const size_t lds1 = rrq_frame_smem(h, nq);
+  set_lds((const void*)rrq_panel_frame_kernel, lds1);
+  hipLaunchKernelGGL(rrq_panel_frame_kernel, dim3(npan), dim3(256), lds1, s2, tab->RT, RP, npan, (const int*)nullptr,
                      (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr, ichart,
                      (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
-  rec(3, 1, s2);
   HIPCHK(hipEventRecord(ev_setup, s2));
+  const size_t lds2 = rrq_system_smem(h, nq, ncoef);
+  set_lds((const void*)rrq_panel_system_kernel, lds2);
+  hipLaunchKernelGGL(rrq_panel_system_kernel, dim3(npan), dim3(256), lds2, s2, tab->RT, RP, npan, (const int*)nullptr,
+                     (const double*)d_sx.ptr, (const double*)d_snx.ptr);
+  rec(3, 1, s2);
+  HIPCHK(hipEventRecord(ev_system, s2));
   rec(4, 0, s2);
   launch_rrq_panel_solve(tab->RT, RP, npan, nullptr, s2);
   rec(4, 1, s2);
@@ -400,9 +406,10 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                   (double*)d_root.ptr, (double*)d_oe.ptr, nnz, s3, tev[8][0]);
   rec(6, 1, s3);
   HIPCHK(hipEventRecord(ev_edge, s3));
-  // main: moments once the edges are in (the set-up precedes them on s3),
-  // weights once the solve is in too
+  // main: moments once the edges and the basis data are in, weights once the
+  // solve is in too
   HIPCHK(hipStreamWaitEvent(stream, ev_edge, 0));
+  HIPCHK(hipStreamWaitEvent(stream, ev_system, 0));
   rec(7, 0, stream);
   const size_t lds3 = rrq_target_smem(h, nq, ncoef);
   set_lds((const void*)rrq_target_kernel, lds3);

@@ -48,7 +48,7 @@ struct Engine {
   i64 tlists_nnz = -1;
   hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
   hipEvent_t tev[10][2] = {};
-  hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_solve = nullptr,
+  hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_system = nullptr, ev_solve = nullptr,
             ev_edge = nullptr;
   std::unique_ptr<DeviceTables> tab;
   DevBuf d_sx, d_snx, d_sw, d_rts, d_rps, d_sxbd, d_rv;

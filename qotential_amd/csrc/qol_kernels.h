@@ -89,9 +89,11 @@ __global__ void triplet_kernel(int nnz, int h, const int* tidx, const int* ppane
                                double* vd);
 
 // RRQ
-__global__ void rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq, const double* sx,
+__global__ void rrq_panel_frame_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq, const double* sx,
                                        const double* snx, const double* sw, int ichart, const double* sxbd_chart,
                                        const double* rv_chart);
+__global__ void rrq_panel_system_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq, const double* sx,
+                                        const double* snx);
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s);
 bool rrq_solve_g1_in_lds(int h);
 __global__ void rrq_project_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
@@ -106,7 +108,8 @@ __global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* lpairs, c
                                   const double* oeg, double* vt, double* omega_out, double* ias_out);
 void launch_rrq_weights(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot, int npan,
                         const int* nrrq, const double* vt, double* wout, hipStream_t s);
-size_t rrq_setup_smem(int h, int nq, int ncoef);
+size_t rrq_frame_smem(int h, int nq);
+size_t rrq_system_smem(int h, int nq, int ncoef);
 size_t rrq_target_smem(int h, int nq, int ncoef);
 
 }  // namespace qol

@@ -144,18 +144,22 @@ __device__ __forceinline__ double imrot(int m, cd z) {
 }
 
 // ---------------------------------------------------------------------------
-size_t rrq_setup_smem(int h, int nq, int ncoef) {
+size_t rrq_frame_smem(int h, int nq) {
   const int nb = 3 * nq;
-  size_t a = (size_t)(7 * h + 3 * h + 16 + 9 * nq + 6 * nb + h) * sizeof(double);
+  return (size_t)(7 * h + 3 * h + 16 + 9 * nq + 6 * nb + h) * sizeof(double);
+}
+size_t rrq_system_smem(int h, int nq, int ncoef) {
+  const int nb = 3 * nq;
+  size_t a = (size_t)(6 * h + 16 + 6 * nb) * sizeof(double);
   size_t b = (size_t)nb * ncoef * sizeof(cd);
   size_t c = (size_t)64 * ncoef * sizeof(cd);
   return a + (b > c ? b : c);
 }
 
 // One workgroup per panel (with RRQ pairs, or every panel when nrrq is null):
-// frame, boundary, basis data at the boundary nodes, and the quaternion matrix
-// and Msl^T at the panel nodes.
-__global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPanels P, int npan,
+// Koornwinder coefficients, orientation, frame, boundary nodes and the edges'
+// Legendre coefficients -- what the per-pair projection and edge kernels need.
+__global__ void __launch_bounds__(256) rrq_panel_frame_kernel(RRQTables T, RRQPanels P, int npan,
                                                               const int* __restrict__ nrrq,
                                                               const double* __restrict__ sx,
                                                               const double* __restrict__ snx,
@@ -294,6 +298,33 @@ __global__ void __launch_bounds__(256) rrq_panel_setup_kernel(RRQTables T, RRQPa
     }
     for (int c = 0; c < 6; ++c) P.bX[((size_t)k * nb + ib) * 6 + c] = bx[6 * ib + c];
   }
+  __syncthreads();
+}
+
+// One workgroup per panel: basis data at the boundary nodes, and the quaternion
+// matrix, Msl^T and Fn at the panel nodes, from the frame and boundary nodes
+// written by rrq_panel_frame_kernel.
+__global__ void __launch_bounds__(256) rrq_panel_system_kernel(RRQTables T, RRQPanels P, int npan,
+                                                               const int* __restrict__ nrrq,
+                                                               const double* __restrict__ sx,
+                                                               const double* __restrict__ snx) {
+  const int k = blockIdx.x;
+  if (nrrq && nrrq[k] == 0) return;  // null: every panel
+  extern __shared__ double sm[];
+  const int h = T.h, nq = T.nq, p = T.p, nb = 3 * nq, ncoef = T.ncoef;
+  double* ssx = sm;             // [3][h]
+  double* ssn = ssx + 3 * h;    // [3][h]
+  double* fr = ssn + 3 * h;     // 16
+  double* bx = fr + 16;         // [nb][6]
+  cd* Hs = (cd*)(bx + 6 * nb);  // [nb or 64][ncoef]
+  const size_t gk = (size_t)k * h;
+  for (int i = threadIdx.x; i < h; i += blockDim.x)
+    for (int c = 0; c < 3; ++c) {
+      ssx[c * h + i] = sx[3 * (gk + i) + c];
+      ssn[c * h + i] = snx[3 * (gk + i) + c];
+    }
+  for (int i = threadIdx.x; i < 16; i += blockDim.x) fr[i] = P.frame[(size_t)k * 16 + i];
+  for (int i = threadIdx.x; i < 6 * nb; i += blockDim.x) bx[i] = P.bX[(size_t)k * nb * 6 + i];
   __syncthreads();
   // basis data at the boundary nodes: grad H'_n^m and V = grad(dX . grad H'_n^m)
   for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) harm_vals(bx + 6 * ib, p, Hs + (size_t)ib * ncoef);
