@@ -1776,7 +1776,7 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
   double* tX = bx + 6 * nb;               // [TT][6]   frame X, original x
   double* w1 = tX + 6 * TT;               // [TT][nb]
   double* ab = w1 + (size_t)nb * TT;      // [TT][big]: a,b per node, later the moment vector v
-  double* tq = ab + (size_t)big * TT;     // [TT][4]: solid angle, spare
+  double* tq = ab + (size_t)big * TT;     // [TT][4]: solid angle, Lambda
   cd* mom = (cd*)(tq + 4 * TT);           // [TT][ncoef][5]
   __shared__ int plist[16];
   for (int i = threadIdx.x; i < 16; i += blockDim.x) fr[i] = P.frame[(size_t)k * 16 + i];
@@ -1810,12 +1810,20 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       const double* Y = bx + 6 * ib;
       const double z[3] = {Y[0] - tX[6 * t], Y[1] - tX[6 * t + 1], Y[2] - tX[6 * t + 2]};
       const double wr = w1[(size_t)t * nb + ib] / sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+      w1[(size_t)t * nb + ib] = wr;  // w1/R from here on
       double zx[3];
       cross3(z, Y + 3, zx);
       double* o = ab + (size_t)t * big + 6 * ib;
       for (int c = 0; c < 3; ++c) { o[c] = wr * zx[c]; o[3 + c] = wr * z[c]; }
     }
     __syncthreads();
+    // Lambda = sum (w1/R) dX per (target, component), into the spare slots of tq
+    for (int it = threadIdx.x; it < nt * 3; it += blockDim.x) {
+      const int t = it / 3, c = it % 3;
+      double l = 0.0;
+      for (int ib = 0; ib < nb; ++ib) l += w1[(size_t)t * nb + ib] * bx[6 * ib + 3 + c];
+      tq[4 * t + 1 + c] = l;
+    }
     // boundary moments chi (1 complex) and alpha (4 complex) per coefficient;
     // a thread serves one coefficient for kTG targets, so each basis value is
     // read once per group (basis is component-major: consecutive kk coalesce)
@@ -1877,15 +1885,8 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
     // Lambda = -omega + sum (w1/R) dX
     for (int t = threadIdx.x; t < nt; t += blockDim.x) {
       cd* M = mom + (size_t)t * ncoef * 5;
-      const double* X = tX + 6 * t;
       const double om = tq[4 * t];
-      double lam[3] = {0, 0, 0};
-      for (int ib = 0; ib < nb; ++ib) {
-        const double* Y = bx + 6 * ib;
-        const double z[3] = {Y[0] - X[0], Y[1] - X[1], Y[2] - X[2]};
-        const double wr = w1[(size_t)t * nb + ib] / sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
-        for (int c = 0; c < 3; ++c) lam[c] += wr * Y[3 + c];
-      }
+      const double lam[3] = {tq[4 * t + 1], tq[4 * t + 2], tq[4 * t + 3]};
       M[0] = cmk(om, 0.0);
       for (int c = 1; c < 5; ++c) M[c] = cmk(0.0, 0.0);
       for (int m = 0; m <= 1; ++m) {
@@ -1899,12 +1900,14 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
         Q[3] = cadd(cscale(g[2], -om), csub(cscale(g[1], lam[0]), cscale(g[0], lam[1])));
       }
     }
-    // recursion in degree, all (target, m) of one degree in parallel
+    // recursion in degree, all (target, m, component) of one degree in parallel
     for (int n = 1; n <= p; ++n) {
       __syncthreads();
       const double dn = (double)n;
-      for (int it = threadIdx.x; it < nt * (n + 1); it += blockDim.x) {
-        const int t = it / (n + 1), m = it % (n + 1);
+      const int ncomp = n >= 2 ? 5 : 1;  // degree 1 of components 1..4 is the seed
+      for (int it = threadIdx.x; it < nt * (n + 1) * ncomp; it += blockDim.x) {
+        const int c = it % ncomp, tm = it / ncomp;
+        const int t = tm / (n + 1), m = tm % (n + 1);
         cd* M = mom + (size_t)t * ncoef * 5;
         const double* X = tX + 6 * t;
         const double xZ = X[0];
@@ -1918,17 +1921,10 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
           return ((-mm) & 1) ? cmk(-v.re, -v.im) : v;
         };
         cd* E = M + (size_t)hidx(n, m) * 5;
-        {
-          const cd lo = get(m - 1, 0), mid = get(m, 0), hi = get(m + 1, 0);
-          const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
-          E[0] = cadd(cadd(cscale(E[0], 1.0 / dn), cscale(mid, xZ)), t2);
-        }
-        if (n >= 2)
-          for (int c = 1; c < 5; ++c) {
-            const cd lo = get(m - 1, c), mid = get(m, c), hi = get(m + 1, c);
-            const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
-            E[c] = cscale(cadd(E[c], cscale(cadd(cscale(mid, xZ), t2), dn)), 1.0 / (dn - 1.0));
-          }
+        const cd lo = get(m - 1, c), mid = get(m, c), hi = get(m + 1, c);
+        const cd t2 = cscale(cmuli(cadd(cmul(w, lo), cmul(wc, hi))), 0.5);
+        if (c == 0) E[0] = cadd(cadd(cscale(E[0], 1.0 / dn), cscale(mid, xZ)), t2);
+        else E[c] = cscale(cadd(E[c], cscale(cadd(cscale(mid, xZ), t2), dn)), 1.0 / (dn - 1.0));
       }
     }
     __syncthreads();
