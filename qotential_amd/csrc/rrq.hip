@@ -1987,22 +1987,30 @@ __global__ void __launch_bounds__(64 * kWM) rrq_weights_kernel(RRQTables T, RRQP
     const int j = 16 * mt + li;
     const bool jin = j < m2;
     const int jc = jin ? j : m2 - 1;
-    for (int t0 = 0; t0 < nm; t0 += 16) {
-      const int t = t0 + li;
-      const bool tin = t < nm;
-      const int tc = tin ? t : nm - 1;
-      qd4 acc = qd4{0.0, 0.0, 0.0, 0.0};
+    // two 16-target tiles per pass: two independent accumulator chains share
+    // every loaded G value
+    for (int t0 = 0; t0 < nm; t0 += 32) {
+      const int ta = t0 + li, tb = t0 + 16 + li;
+      const bool ina = ta < nm, inb = tb < nm;
+      const int tca = ina ? ta : nm - 1, tcb = inb ? tb : nm - 1;
+      qd4 acc0 = qd4{0.0, 0.0, 0.0, 0.0}, acc1 = qd4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 5
       for (int k0 = 0; k0 < H5; k0 += 4) {
         const int r = k0 + lk;
         const bool rin = r < H5;
         const int rc = rin ? r : H5 - 1;
         const double a = (jin & rin) ? G[(size_t)rc * m2 + jc] : 0.0;
-        const double b = (tin & rin) ? V[(size_t)rc * nm + tc] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        const double b0 = (ina & rin) ? V[(size_t)rc * nm + tca] : 0.0;
+        const double b1 = (inb & rin) ? V[(size_t)rc * nm + tcb] : 0.0;
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
       }
-      // D[4q + lk][li]: output row 16mt + 4q + lk, target t0 + li
-      if (tin) {
+      // D[4q + lk][li]: output row 16mt + 4q + lk, target t0 + li (and t0 + 16 + li)
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int t = half ? tb : ta;
+        if (!(half ? inb : ina)) continue;
+        const qd4& acc = half ? acc1 : acc0;
         double* o = wout + (size_t)mine[t] * m2;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
