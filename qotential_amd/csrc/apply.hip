@@ -32,9 +32,9 @@ __device__ __forceinline__ double pair_correction_part(int j, int part, int h, c
     const double rr = dx * dx + dy * dy + dz * dz;
     double an = 0.0, dn = 0.0;
     if (rr > 1.0e-28) {
-      const double r = sqrt(rr);
-      an = pi4inv * sw[g] / r;
-      dn = pi4inv * sw[g] * (dx * snx[3 * g] + dy * snx[3 * g + 1] + dz * snx[3 * g + 2]) / (r * rr);
+      const double ir = rsqrt_nr(rr);
+      an = pi4inv * sw[g] * ir;
+      dn = an * (dx * snx[3 * g] + dy * snx[3 * g + 1] + dz * snx[3 * g + 2]) * (ir * ir);
     }
     acc = acc + (w[i] - an) * ubn[g] - (w[h + i] - dn) * ub[g];
   }

@@ -274,11 +274,8 @@ void Engine::get_pairs(i64* offs, i64* idx1) {
 void Engine::build_levels() {
   d_lgeo.alloc(sizeof(double) * 7 * (size_t)tab->ntot * npan);
   d_balls.alloc(sizeof(double) * 4 * (size_t)tab->nsubtot * npan);
-  const size_t lds = level_geom_smem(h);
-  set_lds((const void*)level_geom_kernel, lds);
-  hipLaunchKernelGGL(level_geom_kernel, dim3(level_geom_blocks(npan)), dim3(256), lds, stream, tab->LA, npan,
-                     (const double*)d_sx.ptr, (const double*)d_rts.ptr, (const double*)d_rps.ptr,
-                     (double*)d_lgeo.ptr, (double*)d_balls.ptr);
+  launch_level_geom(tab->LA, npan, (const double*)d_sx.ptr, (const double*)d_rts.ptr, (const double*)d_rps.ptr,
+                    (double*)d_lgeo.ptr, (double*)d_balls.ptr, stream);
   HIPCHK(hipGetLastError());
   levels_built = true;
 }

@@ -44,6 +44,49 @@ std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host) {
   return ch;
 }
 
+// balls of a chunk's sub-patches (node range [c0, ...) of level l, sub-patches
+// [sb0, sb1)) from the chunk's x, y, z, w in LDS, summed in node order
+__device__ __forceinline__ void chunk_balls(const LevelArgs& A, const double* cn, int k0, int np_, int l, int c0,
+                                            int sb0, int sb1, double* __restrict__ balls) {
+  const int nsc = sb1 - sb0;
+  for (int e = threadIdx.x; e < np_ * nsc; e += blockDim.x) {
+    const int pp = e / nsc, s = sb0 + e % nsc;
+    const int s_all = A.sub_base[l] + s;
+    const int i0 = A.suboff[A.sub_base[l] + l + s] - c0, i1 = A.suboff[A.sub_base[l] + l + s + 1] - c0;
+    const double* L = cn + (size_t)pp * 4 * kChunk;
+    // four independent sums, each in node order (the oracle's), in one pass
+    double swsum = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int j = i0; j < i1; ++j) {
+      const double w = L[3 * kChunk + j];
+      swsum += w;
+      a0 += L[j] * w;
+      a1 += L[kChunk + j] * w;
+      a2 += L[2 * kChunk + j] * w;
+    }
+    const double qp[3] = {a0 / swsum, a1 / swsum, a2 / swsum};
+    // a maximum is exact in any order: two interleaved partial maxima
+    double rm0 = 0.0, rm1 = 0.0;
+    int j = i0;
+    for (; j + 1 < i1; j += 2) {
+      const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
+      const double e0 = L[j + 1] - qp[0], e1 = L[kChunk + j + 1] - qp[1], e2 = L[2 * kChunk + j + 1] - qp[2];
+      rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
+      rm1 = fmax(rm1, e0 * e0 + e1 * e1 + e2 * e2);
+    }
+    if (j < i1) {
+      const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
+      rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    const double rmax2 = fmax(rm0, rm1);
+    const double r2 = A.subtyp[s_all] == 1 ? A.dist2 * fmax(rmax2, 0.5 * swsum) : A.dist2 * fmax(rmax2, swsum);
+    double* B = balls + (size_t)(k0 + pp) * 4 * A.nsubtot + 4 * s_all;
+    B[0] = qp[0];
+    B[1] = qp[1];
+    B[2] = qp[2];
+    B[3] = r2;
+  }
+}
+
 __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, const double* __restrict__ sx,
                                                          const double* __restrict__ rts,
                                                          const double* __restrict__ rps,
@@ -111,46 +154,139 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
       }
     }
     __syncthreads();
-    // balls of the chunk's sub-patches, summed in node order from LDS
-    const int nsc = sb1 - sb0;
-    for (int e = threadIdx.x; e < np_ * nsc; e += blockDim.x) {
-      const int pp = e / nsc, s = sb0 + e % nsc;
-      const int s_all = A.sub_base[l] + s;
-      const int i0 = A.suboff[A.sub_base[l] + l + s] - c0, i1 = A.suboff[A.sub_base[l] + l + s + 1] - c0;
-      const double* L = cn + (size_t)pp * 4 * kChunk;
-      // four independent sums, each in node order (the oracle's), in one pass
-      double swsum = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-      for (int j = i0; j < i1; ++j) {
-        const double w = L[3 * kChunk + j];
-        swsum += w;
-        a0 += L[j] * w;
-        a1 += L[kChunk + j] * w;
-        a2 += L[2 * kChunk + j] * w;
-      }
-      const double qp[3] = {a0 / swsum, a1 / swsum, a2 / swsum};
-      // a maximum is exact in any order: two interleaved partial maxima
-      double rm0 = 0.0, rm1 = 0.0;
-      int j = i0;
-      for (; j + 1 < i1; j += 2) {
-        const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
-        const double e0 = L[j + 1] - qp[0], e1 = L[kChunk + j + 1] - qp[1], e2 = L[2 * kChunk + j + 1] - qp[2];
-        rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
-        rm1 = fmax(rm1, e0 * e0 + e1 * e1 + e2 * e2);
-      }
-      if (j < i1) {
-        const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
-        rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
-      }
-      const double rmax2 = fmax(rm0, rm1);
-      const double r2 = A.subtyp[s_all] == 1 ? A.dist2 * fmax(rmax2, 0.5 * swsum) : A.dist2 * fmax(rmax2, swsum);
-      double* B = balls + (size_t)(k0 + pp) * 4 * A.nsubtot + 4 * s_all;
-      B[0] = qp[0];
-      B[1] = qp[1];
-      B[2] = qp[2];
-      B[3] = r2;
-    }
+    chunk_balls(A, cn, k0, np_, l, c0, sb0, sb1, balls);
     __syncthreads();  // the chunk buffer is rewritten by the next chunk
   }
+}
+
+// Level geometry on the FP64 matrix cores (orders whose h fits KS blocks of 4):
+// per workgroup kLP panels, per wave tiles of 16 level nodes,
+//   D(node, 9*pp + c) = sum_q IM(node, q) S_pp(q, c)   (c: x, y, z, x_u, x_v)
+// as chained v_mfma_f64_16x16x4 over blocks of 4 panel nodes.  The matrix core
+// forms each block as fma(a3,b3, fma(a2,b2, fma(a1,b1, fma(a0,b0, c)))) (checked
+// bit for bit on gfx950), so the chain is the oracle's sequential fma over q and
+// the level nodes -- and the balls -- are bit-identical to it.  The panel values
+// (B operand) stay in registers for the whole workgroup; a tile's results pass
+// through a small per-wave LDS stage to one lane per (node, panel).
+constexpr int kLV = 4 * 9;  // output values per node (kLP panels x 9)
+size_t level_geom_mfma_smem() { return sizeof(double) * ((size_t)kLP * 4 * kChunk + (size_t)4 * 16 * kLV); }
+
+template <int KS>
+__global__ void __launch_bounds__(256) level_geom_mfma_kernel(LevelArgs A, int npan, const double* __restrict__ sx,
+                                                              const double* __restrict__ rts,
+                                                              const double* __restrict__ rps,
+                                                              double* __restrict__ lgeo, double* __restrict__ balls) {
+  static_assert(kLP * 9 == kLV && kLV <= 48, "three 16-column tiles");
+  extern __shared__ double smem[];
+  const int h = A.h, k0 = blockIdx.x * kLP;
+  const int np_ = min(kLP, npan - k0);
+  const size_t nt_ = A.ntot;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  double* cn = smem;                                                   // [kLP][4][kChunk]
+  double* stg = smem + (size_t)kLP * 4 * kChunk + (size_t)wv * 16 * kLV;  // [16 nodes][kLV], this wave
+  // B operand: lane supplies B[lk][li] of block s, tile t = S(q = 4s + lk, column 16t + li)
+  double bf[KS][3];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const int col = 16 * t + li, pp = col / 9, c = col % 9, q = 4 * s + lk;
+      double v = 0.0;
+      if ((col < kLV) & (pp < np_) & (q < h)) {
+        const size_t g = (size_t)(k0 + pp) * h + q;
+        v = c < 3 ? sx[3 * g + c] : (c < 6 ? rts[3 * g + c - 3] : rps[3 * g + c - 6]);
+      }
+      bf[s][t] = v;
+    }
+  for (int ch = 0; ch < A.nchunk; ++ch) {
+    const int* C = A.chunks + 5 * ch;
+    const int l = C[0], c0 = C[1], c1 = C[2], sb0 = C[3], sb1 = C[4];
+    const int off = A.lev_off[l];
+    const int ntile = (c1 - c0 + 15) / 16;
+    for (int tl = wv; tl < ntile; tl += 4) {
+      // A operand: lane supplies A[li][lk] of block s = IM(node c0 + 16 tl + li, q = 4s + lk),
+      // from the row-major copy zero-padded to whole 16-entry blocks
+      const int nd = min(c0 + 16 * tl + li, c1 - 1);
+      const double* row = A.imp + (size_t)(off + nd) * A.pad;
+      double af[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) af[s] = row[4 * s + lk];
+      qd4 acc[3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) acc[t] = qd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int t = 0; t < 3; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], bf[s][t], acc[t], 0, 0, 0);
+      // this lane holds D[4j + lk][16t + li] in slot j of tile t
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        const int col = 16 * t + li;
+        if (col < kLV) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) stg[(4 * j + lk) * kLV + col] = acc[t][j];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // one lane per (node, panel)
+      {
+        const int nl = lane >> 2, pp = lane & 3;
+        const int i = c0 + 16 * tl + nl;
+        if ((i < c1) & (pp < np_)) {
+          const double* gp = stg + nl * kLV + 9 * pp;
+          const double n0 = gp[7] * gp[5] - gp[8] * gp[4];
+          const double n1 = gp[8] * gp[3] - gp[6] * gp[5];
+          const double n2 = gp[6] * gp[4] - gp[7] * gp[3];
+          const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+          const double w = sp * A.wt[off + i];
+          double* G = lgeo + (size_t)(k0 + pp) * 7 * nt_ + off + i;
+          __builtin_nontemporal_store(gp[0], G);
+          __builtin_nontemporal_store(gp[1], G + nt_);
+          __builtin_nontemporal_store(gp[2], G + 2 * nt_);
+          __builtin_nontemporal_store(n0 / sp, G + 3 * nt_);
+          __builtin_nontemporal_store(n1 / sp, G + 4 * nt_);
+          __builtin_nontemporal_store(n2 / sp, G + 5 * nt_);
+          __builtin_nontemporal_store(w, G + 6 * nt_);
+          double* L = cn + (size_t)pp * 4 * kChunk + (i - c0);
+          L[0] = gp[0];
+          L[kChunk] = gp[1];
+          L[2 * kChunk] = gp[2];
+          L[3 * kChunk] = w;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __syncthreads();
+    chunk_balls(A, cn, k0, np_, l, c0, sb0, sb1, balls);
+    __syncthreads();  // the chunk buffer is rewritten by the next chunk
+  }
+}
+
+void launch_level_geom(const LevelArgs& A, int npan, const double* sx, const double* rts, const double* rps,
+                       double* lgeo, double* balls, hipStream_t s) {
+  const int ks = (A.h + 3) / 4;
+  const dim3 grid(level_geom_blocks(npan)), blk(256);
+  if (ks <= 9 && 4 * ks <= A.pad) {
+    const size_t lds = level_geom_mfma_smem();
+#define QOL_LEVEL_MFMA(KS_)                                                                                    \
+  case KS_:                                                                                                    \
+    hipLaunchKernelGGL(level_geom_mfma_kernel<KS_>, grid, blk, lds, s, A, npan, sx, rts, rps, lgeo, balls);  \
+    return;
+    switch (ks) {
+      QOL_LEVEL_MFMA(1) QOL_LEVEL_MFMA(2) QOL_LEVEL_MFMA(3) QOL_LEVEL_MFMA(4) QOL_LEVEL_MFMA(5)
+      QOL_LEVEL_MFMA(6) QOL_LEVEL_MFMA(7) QOL_LEVEL_MFMA(8) QOL_LEVEL_MFMA(9)
+      default: break;
+    }
+#undef QOL_LEVEL_MFMA
+  }
+  const size_t lds = level_geom_smem(A.h);
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)level_geom_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(level_geom_kernel, grid, blk, lds, s, A, npan, sx, rts, rps, lgeo, balls);
 }
 
 // Classify every (panel, target) pair: the first level whose sub-patch balls all

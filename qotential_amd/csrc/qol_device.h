@@ -132,6 +132,24 @@ __device__ __forceinline__ double bernstein_rho(cd s) {
   return a >= 1.0 ? a : 1.0 / a;
 }
 
+// 1/sqrt(x) for finite x > 0: the hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double t = x * y;
+  y = fma(0.5 * y, fma(-t, y, 1.0), y);
+  t = x * y;
+  y = fma(0.5 * y, fma(-t, y, 1.0), y);
+  return y;
+}
+
+// 1/x for finite nonzero x: the hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
 // ---- solid harmonics H'_n^m (m = 0..n), polar axis = frame x, W = y + i z ----
 __device__ __forceinline__ int hidx(int n, int m) { return n * (n + 1) / 2 + m; }
 

@@ -55,6 +55,8 @@ __global__ void level_geom_kernel(LevelArgs A, int npan, const double* sx, const
 size_t level_geom_smem(int h);
 std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host);
 int level_geom_blocks(int npan);
+void launch_level_geom(const LevelArgs& A, int npan, const double* sx, const double* rts, const double* rps,
+                       double* lgeo, double* balls, hipStream_t s);
 __global__ void classify_kernel(LevelArgs A, const int* offs, const int* tidx, const double* tx,
                                 const double* balls, int8_t* label, int* ppanel, int* nrrq, int* lcount);
 

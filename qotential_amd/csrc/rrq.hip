@@ -1580,7 +1580,7 @@ __device__ double solid_angle_edge(const double* cf, const double* tgl, const do
       const double rz = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
       const double qz = q[0] * z[0] + q[1] * z[1] + q[2] * z[2];
       const double qzx = q[0] * zx[0] + q[1] * zx[1] + q[2] * zx[2];
-      total += hh * wgl[i] * qzx / (rz * (rz + qz));
+      total += hh * wgl[i] * qzx * rcp_nr(rz * (rz + qz));
     }
     len = 2.0 * (hi - lo);
     lo = hi;
@@ -1993,18 +1993,27 @@ __global__ void __launch_bounds__(64 * kWM) rrq_weights_kernel(RRQTables T, RRQP
       const int ta = t0 + li, tb = t0 + 16 + li;
       const bool ina = ta < nm, inb = tb < nm;
       const int tca = ina ? ta : nm - 1, tcb = inb ? tb : nm - 1;
+      // K split into even and odd steps as well: four independent chains
       qd4 acc0 = qd4{0.0, 0.0, 0.0, 0.0}, acc1 = qd4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 5
-      for (int k0 = 0; k0 < H5; k0 += 4) {
-        const int r = k0 + lk;
-        const bool rin = r < H5;
-        const int rc = rin ? r : H5 - 1;
+      qd4 acc2 = qd4{0.0, 0.0, 0.0, 0.0}, acc3 = qd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 3
+      for (int k0 = 0; k0 < H5; k0 += 8) {
+        const int r = k0 + lk, s2 = k0 + 4 + lk;
+        const bool rin = r < H5, sin_ = s2 < H5;
+        const int rc = rin ? r : H5 - 1, sc = sin_ ? s2 : H5 - 1;
         const double a = (jin & rin) ? G[(size_t)rc * m2 + jc] : 0.0;
         const double b0 = (ina & rin) ? V[(size_t)rc * nm + tca] : 0.0;
         const double b1 = (inb & rin) ? V[(size_t)rc * nm + tcb] : 0.0;
+        const double a2 = (jin & sin_) ? G[(size_t)sc * m2 + jc] : 0.0;
+        const double b2 = (ina & sin_) ? V[(size_t)sc * nm + tca] : 0.0;
+        const double b3 = (inb & sin_) ? V[(size_t)sc * nm + tcb] : 0.0;
         acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc2, 0, 0, 0);
+        acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b3, acc3, 0, 0, 0);
       }
+      acc0 += acc2;
+      acc1 += acc3;
       // D[4q + lk][li]: output row 16mt + 4q + lk, target t0 + li (and t0 + 16 + li)
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
