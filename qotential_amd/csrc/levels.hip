@@ -139,13 +139,13 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
         const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
         const double w = sp * A.wt[off + i];
         double* G = lgeo + (size_t)(k0 + pp) * 7 * nt_ + off + i;
-        __builtin_nontemporal_store(gp[0], G);
-        __builtin_nontemporal_store(gp[1], G + nt_);
-        __builtin_nontemporal_store(gp[2], G + 2 * nt_);
-        __builtin_nontemporal_store(n0 / sp, G + 3 * nt_);
-        __builtin_nontemporal_store(n1 / sp, G + 4 * nt_);
-        __builtin_nontemporal_store(n2 / sp, G + 5 * nt_);
-        __builtin_nontemporal_store(w, G + 6 * nt_);
+        G[0] = gp[0];
+        G[nt_] = gp[1];
+        G[2 * nt_] = gp[2];
+        G[3 * nt_] = n0 / sp;
+        G[4 * nt_] = n1 / sp;
+        G[5 * nt_] = n2 / sp;
+        G[6 * nt_] = w;
         double* L = cn + (size_t)pp * 4 * kChunk + li;
         L[0] = gp[0];
         L[kChunk] = gp[1];
@@ -242,13 +242,13 @@ __global__ void __launch_bounds__(256) level_geom_mfma_kernel(LevelArgs A, int n
           const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
           const double w = sp * A.wt[off + i];
           double* G = lgeo + (size_t)(k0 + pp) * 7 * nt_ + off + i;
-          __builtin_nontemporal_store(gp[0], G);
-          __builtin_nontemporal_store(gp[1], G + nt_);
-          __builtin_nontemporal_store(gp[2], G + 2 * nt_);
-          __builtin_nontemporal_store(n0 / sp, G + 3 * nt_);
-          __builtin_nontemporal_store(n1 / sp, G + 4 * nt_);
-          __builtin_nontemporal_store(n2 / sp, G + 5 * nt_);
-          __builtin_nontemporal_store(w, G + 6 * nt_);
+          G[0] = gp[0];
+          G[nt_] = gp[1];
+          G[2 * nt_] = gp[2];
+          G[3 * nt_] = n0 / sp;
+          G[4 * nt_] = n1 / sp;
+          G[5 * nt_] = n2 / sp;
+          G[6 * nt_] = w;
           double* L = cn + (size_t)pp * 4 * kChunk + (i - c0);
           L[0] = gp[0];
           L[kChunk] = gp[1];
