@@ -19,8 +19,9 @@ namespace qol {
 // match it bit for bit.
 // Work goes by chunks of whole sub-patches of one level (at most kChunk nodes,
 // one node per thread; the host builds the chunk table): the chunk's nodes are
-// written to HBM with streaming stores and kept in LDS, and the balls of its
-// sub-patches are summed from LDS in node order, so nothing is read back.
+// written to HBM (plain stores: streaming ones measured slower) and kept in LDS,
+// and the balls of its sub-patches are summed from LDS in node order, so nothing
+// is read back.
 // Output: lgeo[panel][7][ntot] (x,y,z,nx,ny,nz,w), balls[panel][nsubtot][4].
 constexpr int kLP = 4;
 constexpr int kChunk = 256;
