@@ -55,8 +55,8 @@ def kernel_flops(p, npan, counts, nrrq, nrrq_panels):
     nb = 3 * nq
     ntot = int(sum(info["nl"]))
     return {
-        # 9 sequential FMAs per (level node, panel node)
-        "levels": ("level_geom_kernel", 18.0 * h * ntot * npan),
+        # 9 sequential FMAs per (level node, panel node); matrix cores up to order 8
+        "levels": ("level_geom_mfma_kernel" if (h + 3) // 4 <= 9 else "level_geom_kernel", 18.0 * h * ntot * npan),
         # per level node: 19 for the S/D entry (incl. one rsqrt), 4h for the two contractions
         "direct": ("direct_mfma_kernel", float(sum(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h)
                                                    for l in range(4)))),
