@@ -350,6 +350,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   d_w1.alloc(sizeof(double) * 3 * (size_t)nq * nnz);
   d_oe.alloc(sizeof(double) * 3 * (size_t)nnz);
   d_root.alloc(sizeof(double) * 9 * (size_t)nnz);
+  d_eorder.alloc(sizeof(int) * rrq_edge_order_ints(nnz));
   d_vt.alloc(sizeof(double) * 5 * (size_t)h * nnz);
   HIPCHK(hipEventRecord(ev_start, stream));
   // s2: RRQ set-up and solve for every panel
@@ -400,7 +401,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   rec(6, 0, s3);
   launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                   (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
-                  (double*)d_root.ptr, (double*)d_oe.ptr, nnz, s3, tev[8][0]);
+                  (double*)d_root.ptr, (double*)d_oe.ptr, (int*)d_eorder.ptr, nnz, s3, tev[8][0]);
   rec(6, 1, s3);
   HIPCHK(hipEventRecord(ev_edge, s3));
   // main: moments once the edges and the basis data are in, weights once the

@@ -54,7 +54,7 @@ struct Engine {
   DevBuf d_sx, d_snx, d_sw, d_rts, d_rps, d_sxbd, d_rv;
   DevBuf d_tx, d_nball, d_cnt, d_offs, d_tidx;
   DevBuf d_label, d_ppanel, d_nrrq, d_lcount, d_lstart, d_ltot, d_lpairs, d_wout, d_val, d_omega, d_ias;
-  DevBuf d_lgeo, d_balls, d_pq, d_w1, d_oe, d_root, d_vt;
+  DevBuf d_lgeo, d_balls, d_pq, d_w1, d_oe, d_root, d_eorder, d_vt;
   DevBuf d_frame, d_edge, d_bX, d_cx, d_basis, d_Fq, d_Msl, d_Fn, d_G, d_Binv, d_scr;
   DevBuf d_ub, d_ubn, d_u, d_tcount, d_toffs, d_tcur, d_tpairs;
   RRQPanels RP{};

@@ -102,9 +102,10 @@ __global__ void rrq_project_kernel(RRQTables T, RRQPanels P, const int* lpairs, 
                                    const int* tidx, const double* tx, const double* sx, int exterior,
                                    double side_tol, double* pq);
 // root + SSQ kernel, then (after event `mid`, if given) the solid-angle kernel
+size_t rrq_edge_order_ints(int64_t nnz);
 void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
                      const int* tidx, const double* tx, const double* pq, double* w1, double* root, double* oe,
-                     int64_t nnz, hipStream_t s, hipEvent_t mid);
+                     int* order, int64_t nnz, hipStream_t s, hipEvent_t mid);
 __global__ void rrq_target_kernel(RRQTables T, RRQPanels P, const int* lpairs, const int* lstart, const int* ltot,
                                   int npan, const int* tidx, const double* tx, const int* nrrq, const double* w1g,
                                   const double* oeg, double* vt, double* omega_out, double* ias_out);

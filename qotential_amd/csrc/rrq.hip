@@ -1701,9 +1701,10 @@ __global__ void __launch_bounds__(256) rrq_solid_kernel(RRQTables T, RRQPanels P
                                                         const int* __restrict__ ltot, const int* __restrict__ ppanel,
                                                         const int* __restrict__ tidx, const double* __restrict__ tx,
                                                         const double* __restrict__ pq, const double* __restrict__ root,
-                                                        double* __restrict__ oe) {
-  const int it = blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= 3 * ltot[4]) return;
+                                                        const int* __restrict__ perm, double* __restrict__ oe) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= 3 * ltot[4]) return;
+  const int it = perm[gid];  // items in order of their cost bucket
   const int r = it / 3, e = it % 3;
   const int pair = lpairs[rrq_list_base(ltot) + r];
   const int k = ppanel[pair];
@@ -1719,15 +1720,106 @@ __global__ void __launch_bounds__(256) rrq_solid_kernel(RRQTables T, RRQPanels P
   oe[it] = solid_angle_edge<NQ>(cf, T.tgl, T.wgl, X, q, cmk(rt[0], rt[1]), rt[2] != 0.0, rho_thr);
 }
 
+// The solid-angle march of a (pair, edge) item takes about 2 log2(1/d) intervals,
+// d the distance of the edge's complex root from [-1, 1] (one interval without a
+// root).  Items are grouped by that cost -- a counting sort into kSB buckets,
+// costliest first -- so the lanes of a wave run similar marches.  Each item
+// writes only its own result, so the order changes nothing but the time.
+constexpr int kSB = 64;
+
+__device__ __forceinline__ int solid_bucket(const double* rt) {
+  if (rt[2] == 0.0) return 0;
+  const double dx = fmax(fabs(rt[0]) - 1.0, 0.0), im = fabs(rt[1]);
+  const double d = sqrt(dx * dx + im * im);
+  if (!(d > 0.0)) return kSB - 1;  // on the segment, or not a number
+  int e;
+  (void)frexp(d, &e);  // d in [2^(e-1), 2^e)
+  return min(max(2 - e, 0), kSB - 1);
+}
+
+// counts per bucket: a histogram per workgroup in LDS, then one global add per
+// non-empty bucket (kSI items per thread)
+constexpr int kSI = 4;
+__global__ void __launch_bounds__(256) solid_count_kernel(const int* __restrict__ ltot, const double* __restrict__ root,
+                                                          int* __restrict__ cnt, int* __restrict__ key) {
+  __shared__ int lh[kSB];
+  for (int b = threadIdx.x; b < kSB; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const int n = 3 * ltot[4];
+  const int base = blockIdx.x * blockDim.x * kSI + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kSI; ++u) {
+    const int it = base + u * blockDim.x;
+    if (it < n) {
+      const int b = solid_bucket(root + 3 * (size_t)it);
+      key[it] = b;
+      atomicAdd(lh + b, 1);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSB; b += blockDim.x)
+    if (lh[b]) atomicAdd(cnt + b, lh[b]);
+}
+
+// bucket starts, costliest bucket first (one thread); cnt[kSB..2kSB) receive them
+__global__ void solid_offsets_kernel(int* __restrict__ cnt) {
+  if (threadIdx.x != 0) return;
+  int s = 0;
+  for (int b = kSB - 1; b >= 0; --b) {
+    cnt[kSB + b] = s;
+    s += cnt[b];
+  }
+}
+
+// the permutation: ranks within the workgroup from LDS, one global range per
+// non-empty bucket per workgroup
+__global__ void __launch_bounds__(256) solid_scatter_kernel(const int* __restrict__ ltot, const int* __restrict__ key,
+                                                            int* __restrict__ cnt, int* __restrict__ perm) {
+  __shared__ int lh[kSB], lb[kSB];
+  for (int b = threadIdx.x; b < kSB; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const int n = 3 * ltot[4];
+  const int base = blockIdx.x * blockDim.x * kSI + threadIdx.x;
+  int bk[kSI], rk[kSI];
+#pragma unroll
+  for (int u = 0; u < kSI; ++u) {
+    const int it = base + u * blockDim.x;
+    bk[u] = it < n ? key[it] : -1;
+    rk[u] = bk[u] >= 0 ? atomicAdd(lh + bk[u], 1) : 0;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSB; b += blockDim.x) lb[b] = lh[b] ? atomicAdd(cnt + kSB + b, lh[b]) : 0;
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kSI; ++u)
+    if (bk[u] >= 0) perm[lb[bk[u]] + rk[u]] = base + u * blockDim.x;
+}
+
+__global__ void zero_ints_kernel(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+size_t rrq_edge_order_ints(int64_t nnz) { return (size_t)2 * kSB + (size_t)6 * nnz; }
+
 void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
                      const int* tidx, const double* tx, const double* pq, double* w1, double* root, double* oe,
-                     int64_t nnz, hipStream_t s, hipEvent_t mid) {
+                     int* order, int64_t nnz, hipStream_t s, hipEvent_t mid) {
   const dim3 grid((unsigned)((3 * nnz + 255) / 256)), blk(256);
+  const dim3 gsort((unsigned)((3 * nnz + 256 * kSI - 1) / (256 * kSI)));
+  int* cnt = order;                  // [2 kSB]: counts, then bucket starts
+  int* key = order + 2 * kSB;        // [3 nnz]
+  int* perm = key + 3 * nnz;         // [3 nnz]
 #define QOL_EDGE(NQ_)                                                                                      \
   case NQ_:                                                                                                \
     hipLaunchKernelGGL(rrq_root_kernel<NQ_>, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, w1, root); \
     if (mid) (void)hipEventRecord(mid, s);                                                                 \
-    hipLaunchKernelGGL(rrq_solid_kernel<NQ_>, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, pq, root, oe); \
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(2 * kSB), 0, s, cnt, 2 * kSB);                      \
+    hipLaunchKernelGGL(solid_count_kernel, gsort, blk, 0, s, ltot, (const double*)root, cnt, key);        \
+    hipLaunchKernelGGL(solid_offsets_kernel, dim3(1), dim3(64), 0, s, cnt);                               \
+    hipLaunchKernelGGL(solid_scatter_kernel, gsort, blk, 0, s, ltot, (const int*)key, cnt, perm);         \
+    hipLaunchKernelGGL(rrq_solid_kernel<NQ_>, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, pq, root,  \
+                       (const int*)perm, oe);                                                             \
     break;
   switch (T.nq) {
     QOL_EDGE(4) QOL_EDGE(5) QOL_EDGE(6) QOL_EDGE(7) QOL_EDGE(8) QOL_EDGE(9) QOL_EDGE(10) QOL_EDGE(11)
