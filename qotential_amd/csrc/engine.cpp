@@ -393,10 +393,9 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   HIPCHK(hipStreamWaitEvent(s3, ev_lists, 0));
   HIPCHK(hipStreamWaitEvent(s3, ev_setup, 0));
   rec(5, 0, s3);
-  hipLaunchKernelGGL(rrq_project_kernel, dim3(blocks((size_t)nnz, 256)), dim3(256), 0, s3, tab->RT, RP,
-                     (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+  launch_rrq_project(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol,
-                     (double*)d_pq.ptr);
+                     (double*)d_pq.ptr, (int*)d_eorder.ptr, nnz, s3);
   rec(5, 1, s3);
   rec(6, 0, s3);
   launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,

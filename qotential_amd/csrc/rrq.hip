@@ -1589,40 +1589,59 @@ __device__ double solid_angle_edge(const double* cf, const double* tgl, const do
 }
 
 // Gauss-Newton projection of x (original coordinates) onto the patch, started
-// at the nearest node (ssx: the panel's nodes, x(3,h)); returns the signed
-// distance along the uv normal and that normal
-__device__ double project(const double* cx, const double* __restrict__ kt, int h, int korder,
-                          const double* ssx, const double* uvn,
-                          const double* x, double* nstar) {
-  double best = 1e300, u = 1.0 / 3.0, v = 1.0 / 3.0;
+// at the nearest node (ssx: the panel's nodes, x(3,h)), in three pieces so the
+// iterations can be split over two launches: the start, Gauss-Newton steps
+// it0 <= it < it1 (true once a step is below 1e-15), and the end (the normal at
+// the projection and the signed distance along it).
+__device__ void project_start(const double* ssx, const double* uvn, int h, const double* x, double& u, double& v) {
+  double best = 1e300;
+  u = 1.0 / 3.0;
+  v = 1.0 / 3.0;
   for (int i = 0; i < h; ++i) {
     const double d0 = ssx[3 * i] - x[0], d1 = ssx[3 * i + 1] - x[1], d2 = ssx[3 * i + 2] - x[2];
     const double d = d0 * d0 + d1 * d1 + d2 * d2;
     if (d < best) { best = d; u = uvn[2 * i]; v = uvn[2 * i + 1]; }
   }
-  double y[3], yu[3], yv[3];
-  for (int it = 0; it < 40; ++it) {
-    geom_eval(cx, kt, h, korder, u, v, y, yu, yv);
-    const double r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
-    const double a11 = yu[0] * yu[0] + yu[1] * yu[1] + yu[2] * yu[2];
-    const double a12 = yu[0] * yv[0] + yu[1] * yv[1] + yu[2] * yv[2];
-    const double a22 = yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2];
-    const double b1 = -(yu[0] * r[0] + yu[1] * r[1] + yu[2] * r[2]);
-    const double b2 = -(yv[0] * r[0] + yv[1] * r[1] + yv[2] * r[2]);
-    const double det = a11 * a22 - a12 * a12;
-    double un = u + (a22 * b1 - a12 * b2) / det, vn = v + (a11 * b2 - a12 * b1) / det;
-    if (un < 0.0) un = 0.0;
-    if (vn < 0.0) vn = 0.0;
-    if (un + vn > 1.0) {
-      const double ex = 0.5 * (un + vn - 1.0);
-      un -= ex; vn -= ex;
-      if (un < 0.0) { vn += un; un = 0.0; }
-      if (vn < 0.0) { un += vn; vn = 0.0; }
-    }
-    const double step = fabs(un - u) + fabs(vn - v);
-    u = un; v = vn;
-    if (step < 1e-15) break;
+}
+
+// One Gauss-Newton update of (u, v) from the point y and its tangents, clamped
+// to the triangle; true when the step is below 1e-15
+__device__ __forceinline__ bool gn_step(const double* x, const double* y, const double* yu, const double* yv,
+                                        double& u, double& v) {
+  const double r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+  const double a11 = yu[0] * yu[0] + yu[1] * yu[1] + yu[2] * yu[2];
+  const double a12 = yu[0] * yv[0] + yu[1] * yv[1] + yu[2] * yv[2];
+  const double a22 = yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2];
+  const double b1 = -(yu[0] * r[0] + yu[1] * r[1] + yu[2] * r[2]);
+  const double b2 = -(yv[0] * r[0] + yv[1] * r[1] + yv[2] * r[2]);
+  const double det = a11 * a22 - a12 * a12;
+  double un = u + (a22 * b1 - a12 * b2) / det, vn = v + (a11 * b2 - a12 * b1) / det;
+  if (un < 0.0) un = 0.0;
+  if (vn < 0.0) vn = 0.0;
+  if (un + vn > 1.0) {
+    const double ex = 0.5 * (un + vn - 1.0);
+    un -= ex; vn -= ex;
+    if (un < 0.0) { vn += un; un = 0.0; }
+    if (vn < 0.0) { un += vn; vn = 0.0; }
   }
+  const double step = fabs(un - u) + fabs(vn - v);
+  u = un; v = vn;
+  return step < 1e-15;
+}
+
+__device__ bool project_steps(const double* cx, const double* __restrict__ kt, int h, int korder, const double* x,
+                              double& u, double& v, int it0, int it1) {
+  double y[3], yu[3], yv[3];
+  for (int it = it0; it < it1; ++it) {
+    geom_eval(cx, kt, h, korder, u, v, y, yu, yv);
+    if (gn_step(x, y, yu, yv, u, v)) return true;
+  }
+  return false;
+}
+
+__device__ double project_end(const double* cx, const double* __restrict__ kt, int h, int korder, const double* x,
+                              double u, double v, double* nstar) {
+  double y[3], yu[3], yv[3];
   geom_eval(cx, kt, h, korder, u, v, y, yu, yv);
   double nn[3];
   cross3(yu, yv, nn);
@@ -1644,30 +1663,189 @@ __device__ double project(const double* cx, const double* __restrict__ kt, int h
 
 __device__ __forceinline__ int rrq_list_base(const int* ltot) { return ltot[0] + ltot[1] + ltot[2] + ltot[3]; }
 
-__global__ void __launch_bounds__(256) rrq_project_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
-                                                          const int* __restrict__ ltot,
-                                                          const int* __restrict__ ppanel,
-                                                          const int* __restrict__ tidx,
-                                                          const double* __restrict__ tx,
-                                                          const double* __restrict__ sx, int exterior,
-                                                          double side_tol, double* __restrict__ pq) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ltot[4]) return;
-  const int pair = lpairs[rrq_list_base(ltot) + r];
-  const int k = ppanel[pair], h = T.h;
+__global__ void zero_ints_kernel(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Projection in two launches: every pair takes up to kPI Gauss-Newton steps;
+// the few that have not converged (a list built with an atomic counter) continue
+// in the second launch from the saved (u, v), so a wave no longer runs as long
+// as its slowest pair.  The step sequence of every pair is the single loop's,
+// so the results are the same.
+constexpr int kPI = 8, kPMax = 40;
+
+__device__ void project_write(RRQTables T, RRQPanels P, int k, const double* x, double u, double v, int exterior,
+                              double side_tol, double* o) {
+  const int h = T.h;
   const double* fr = P.frame + (size_t)k * 16;
-  const size_t g = (size_t)tidx[pair];
-  const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
   double ns[3];
-  const double d = project(P.cx + (size_t)k * 3 * h, T.ktab, h, T.p - 1, sx + (size_t)k * 3 * h, T.uvn, x, ns);
+  const double d = project_end(P.cx + (size_t)k * 3 * h, T.ktab, h, T.p - 1, x, u, v, ns);
   double side;
   if (fabs(d) > side_tol * fr[14]) side = d > 0.0 ? 1.0 : -1.0;
   else side = (exterior ? 1.0 : -1.0) * fr[13];
   const double qo[3] = {-side * ns[0], -side * ns[1], -side * ns[2]};
   double q[3];
   rotate(fr, qo, q);
-  double* o = pq + 4 * (size_t)r;
   o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = side;
+}
+
+__global__ void __launch_bounds__(256) rrq_project_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
+                                                          const int* __restrict__ ltot,
+                                                          const int* __restrict__ ppanel,
+                                                          const int* __restrict__ tidx,
+                                                          const double* __restrict__ tx,
+                                                          const double* __restrict__ sx, int exterior,
+                                                          double side_tol, double* __restrict__ pq,
+                                                          int* __restrict__ rest, int* __restrict__ nrest) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ltot[4]) return;
+  const int pair = lpairs[rrq_list_base(ltot) + r];
+  const int k = ppanel[pair], h = T.h;
+  const size_t g = (size_t)tidx[pair];
+  const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
+  double u, v;
+  project_start(sx + (size_t)k * 3 * h, T.uvn, h, x, u, v);
+  double* o = pq + 4 * (size_t)r;
+  if (project_steps(P.cx + (size_t)k * 3 * h, T.ktab, h, T.p - 1, x, u, v, 0, kPI)) {
+    project_write(T, P, k, x, u, v, exterior, side_tol, o);
+  } else {
+    o[0] = u; o[1] = v;  // the state, for the second launch
+    rest[atomicAdd(nrest, 1)] = r;
+  }
+}
+
+__global__ void __launch_bounds__(256) rrq_project_rest_kernel(RRQTables T, RRQPanels P,
+                                                               const int* __restrict__ lpairs,
+                                                               const int* __restrict__ ltot,
+                                                               const int* __restrict__ ppanel,
+                                                               const int* __restrict__ tidx,
+                                                               const double* __restrict__ tx, int exterior,
+                                                               double side_tol, double* __restrict__ pq,
+                                                               const int* __restrict__ rest,
+                                                               const int* __restrict__ nrest) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= *nrest) return;
+  const int r = rest[gid];
+  const int pair = lpairs[rrq_list_base(ltot) + r];
+  const int k = ppanel[pair], h = T.h;
+  const size_t g = (size_t)tidx[pair];
+  const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
+  double* o = pq + 4 * (size_t)r;
+  double u = o[0], v = o[1];
+  (void)project_steps(P.cx + (size_t)k * 3 * h, T.ktab, h, T.p - 1, x, u, v, kPI, kPMax);
+  project_write(T, P, k, x, u, v, exterior, side_tol, o);
+}
+
+// The second launch runs few pairs and so is bound by each step's latency: for
+// the orders up to 8 it holds the panel's coefficients in registers and unrolls
+// the evaluation (the same operations in the same order as geom_eval, so the
+// same results).  In the first launch, which is bound by throughput, the same
+// unrolled form measured slower (it needs more registers, so fewer waves).
+template <int N>
+__device__ __forceinline__ void geom_eval_reg(const double (&cr)[3][N * (N + 1) / 2], const double* __restrict__ kt,
+                                              double u, double v, double* x, double* xu, double* xv) {
+  const double a = 2.0 * u + v - 1.0, b = 1.0 - v, bb = b * b;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { x[c] = 0.0; xu[c] = 0.0; xv[c] = 0.0; }
+  const double xx = 2.0 * v - 1.0;
+  double Q = 1.0, Qu = 0.0, Qv = 0.0, Qm = 0.0, Qum = 0.0, Qvm = 0.0;
+#pragma unroll
+  for (int am = 0; am < N; ++am) {
+    if (am == 1) {
+      Qm = Q; Qum = Qu; Qvm = Qv;
+      Q = a; Qu = 2.0; Qv = 1.0;
+    } else if (am >= 2) {
+      const double e1 = kt[2 * am], e2 = kt[2 * am + 1];
+      const double Qn = e1 * a * Q - e2 * bb * Qm;
+      const double Qun = e1 * (2.0 * Q + a * Qu) - e2 * bb * Qum;
+      const double Qvn = e1 * (Q + a * Qv) - e2 * (bb * Qvm - 2.0 * b * Qm);
+      Qm = Q; Qum = Qu; Qvm = Qv;
+      Q = Qn; Qu = Qun; Qv = Qvn;
+    }
+    const double al = 2.0 * am + 1.0;
+    double p = 1.0, dp = 0.0, pm = 1.0, dpm = 0.0;
+#pragma unroll
+    for (int bd = 0; am + bd < N; ++bd) {
+      // table entry of (am, bd): terms are stored am-major after the 2N factors
+      const double* tt = kt + 2 * N + 4 * (am * N - am * (am - 1) / 2 + bd);
+      if (bd == 1) {
+        p = 0.5 * ((al + 2.0) * xx + al);
+        dp = al + 2.0;
+      } else if (bd >= 2) {
+        const double w = tt[1] + tt[2] * xx;
+        const double pn = w * p - tt[3] * pm;
+        const double dpn = w * dp + 2.0 * tt[2] * p - tt[3] * dpm;
+        pm = p; p = pn; dpm = dp; dp = dpn;
+      }
+      const int d = am + bd, idx = d * (d + 1) / 2 + bd;
+      const double sc = tt[0];
+      const double f = sc * Q * p;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x[c] += f * cr[c][idx];
+      const double fu = sc * Qu * p, fv = sc * (Qv * p + Q * dp);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { xu[c] += fu * cr[c][idx]; xv[c] += fv * cr[c][idx]; }
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(64) rrq_project_rest_reg_kernel(RRQTables T, RRQPanels P,
+                                                                  const int* __restrict__ lpairs,
+                                                                  const int* __restrict__ ltot,
+                                                                  const int* __restrict__ ppanel,
+                                                                  const int* __restrict__ tidx,
+                                                                  const double* __restrict__ tx, int exterior,
+                                                                  double side_tol, double* __restrict__ pq,
+                                                                  const int* __restrict__ rest,
+                                                                  const int* __restrict__ nrest) {
+  constexpr int H = N * (N + 1) / 2;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= *nrest) return;
+  const int r = rest[gid];
+  const int pair = lpairs[rrq_list_base(ltot) + r];
+  const int k = ppanel[pair];
+  const size_t g = (size_t)tidx[pair];
+  const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
+  const double* cx = P.cx + (size_t)k * 3 * H;
+  double cr[3][H];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int i = 0; i < H; ++i) cr[c][i] = cx[c * H + i];
+  double* o = pq + 4 * (size_t)r;
+  double u = o[0], v = o[1];
+  double y[3], yu[3], yv[3];
+  for (int it = kPI; it < kPMax; ++it) {
+    geom_eval_reg<N>(cr, T.ktab, u, v, y, yu, yv);
+    if (gn_step(x, y, yu, yv, u, v)) break;
+  }
+  project_write(T, P, k, x, u, v, exterior, side_tol, o);
+}
+
+void launch_rrq_project(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
+                        const int* tidx, const double* tx, const double* sx, int exterior, double side_tol,
+                        double* pq, int* order, int64_t nnz, hipStream_t s) {
+  int* nrest = order;      // [1]
+  int* rest = order + 1;   // [nnz]
+  const dim3 grid((unsigned)((nnz + 255) / 256)), blk(256);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, nrest, 1);
+  hipLaunchKernelGGL(rrq_project_kernel, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, sx, exterior,
+                     side_tol, pq, rest, nrest);
+  const dim3 rgrid((unsigned)((nnz + 63) / 64)), rblk(64);
+#define QOL_PROJ(NN)                                                                                            \
+  case NN:                                                                                                      \
+    hipLaunchKernelGGL(rrq_project_rest_reg_kernel<NN>, rgrid, rblk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, \
+                       exterior, side_tol, pq, (const int*)rest, (const int*)nrest);                             \
+    break;
+  switch (T.p) {
+    QOL_PROJ(2) QOL_PROJ(3) QOL_PROJ(4) QOL_PROJ(5) QOL_PROJ(6) QOL_PROJ(7) QOL_PROJ(8)
+    default:
+      hipLaunchKernelGGL(rrq_project_rest_kernel, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, exterior,
+                         side_tol, pq, (const int*)rest, (const int*)nrest);
+  }
+#undef QOL_PROJ
 }
 
 // per (pair, edge): complex root and SSQ weights -> w1[r][3NQ], root[r][3][3] (re, im, ok)
@@ -1793,11 +1971,6 @@ __global__ void __launch_bounds__(256) solid_scatter_kernel(const int* __restric
 #pragma unroll
   for (int u = 0; u < kSI; ++u)
     if (bk[u] >= 0) perm[lb[bk[u]] + rk[u]] = base + u * blockDim.x;
-}
-
-__global__ void zero_ints_kernel(int* __restrict__ p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
 }
 
 size_t rrq_edge_order_ints(int64_t nnz) { return (size_t)2 * kSB + (size_t)6 * nnz; }
