@@ -33,6 +33,15 @@ int64_t qol_geo_count(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nm
 int qol_geo_build(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
                   const double* mnrz, int64_t iterate_aspect, double* sx, double* snx,
                   double* sw, double* rts, double* rps, double* charts);
+/* qol_geo_build on the GPU: the same arguments and outputs, with the per-chart
+ * work of stellarator_geo / stellarator_geo_charts (utils/stellarator_geo_mex.c:
+ * 636-749: tensor grid, spectral derivatives, node interpolation, normals and
+ * weights) in one GPU wave per chart; the chart list and diagonals are built on
+ * the host as in qol_geo_build.  Nodes agree with qol_geo_build to the rounding
+ * of the device sin/cos.  Synchronous; host output arrays. */
+int qol_geo_build_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                      const double* mnrz, int64_t iterate_aspect, double* sx, double* snx,
+                      double* sw, double* rts, double* rps, double* charts);
 /* exact chart boundary per triangle (ichart = 1 input of rrq): replaces the
  * uvbd table + stellarator_geo_uv2x loop of stellarator_grf_core_mod.f90:613-629,
  * 819-828.  sxbd(3, 3*nquad, ntri), rv(3, 3, ntri). */

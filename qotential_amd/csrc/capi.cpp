@@ -154,6 +154,30 @@ int qol_engine_stream(void* h, void** stream) {
   return guarded([&] { *stream = (void*)E(h)->stream; });
 }
 
+int qol_geo_build_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                      int64_t iterate_aspect, double* sx, double* snx, double* sw, double* rts, double* rps,
+                      double* charts) {
+  return guarded([&] {
+    Surface s;
+    s.nfp = nfp;
+    for (int64_t k = 0; k < nmode; ++k) {
+      s.m.push_back(mnrz[4 * k]);
+      s.n.push_back(mnrz[4 * k + 1]);
+      s.rc.push_back(mnrz[4 * k + 2]);
+      s.zs.push_back(mnrz[4 * k + 3]);
+    }
+    GeometryPlan P;
+    plan_geometry(s, mp, np, p, iterate_aspect != 0, P);
+    build_geometry_gpu(s, P, sx, snx, sw, rts, rps);
+    if (charts)
+      for (size_t k = 0; k < P.charts.size(); ++k) {
+        const Chart& c = P.charts[k];
+        const double v[6] = {c.a1, c.b1, c.a2, c.b2, c.jj, c.ii};
+        for (int j = 0; j < 6; ++j) charts[6 * k + j] = v[j];
+      }
+  });
+}
+
 int qol_hbm_copy(int64_t nbytes, double* ms) {
   return guarded([&] {
     if (nbytes <= 0 || nbytes % 16) throw std::runtime_error("qol_hbm_copy: nbytes must be a positive multiple of 16");

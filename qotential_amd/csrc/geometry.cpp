@@ -174,32 +174,50 @@ std::vector<Chart> build_charts(const Surface& s, i64 mp, i64 np, i64 p, bool it
   return L;
 }
 
-void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, Geometry& g) {
+void plan_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, GeometryPlan& g) {
   const OrderTables& T = order_tables(p);
-  const i64 h = T.h, p2 = p * p;
-  g.p = p; g.h = h; g.mp = mp; g.np = np;
+  const i64 h = T.h;
+  g.p = p; g.h = h;
+  g.ts1 = kPi / (double)np;
+  g.ts2 = kPi / (double)mp;
   g.charts = build_charts(s, mp, np, p, iterate_aspect);
+  g.x0.assign(p, 0.0);
+  g.D.assign(p * p, 0.0);
+  std::vector<double> w0(p);
+  gl_nodes(p, g.x0.data(), w0.data(), g.D.data());
+  g.diag.resize(g.charts.size());
+  for (size_t k = 0; k < g.charts.size(); ++k)
+    g.diag[k] = chart_diagonal_A(s, g.charts[k], g.ts1, g.ts2, g.x0.data(), p) ? 0 : 1;
+  // interpolation weights for the four (diagonal, side) combinations
+  g.W.clear();
+  for (int dg = 0; dg < 2; ++dg)
+    for (int sd = 0; sd < 2; ++sd) {
+      std::vector<double> t1(h), t2(h), Mt;
+      for (i64 i = 0; i < h; ++i) corner_map(dg == 0, sd, T.uvn[2 * i], T.uvn[2 * i + 1], &t1[i], &t2[i]);
+      tensor_weights(t1, t2, g.x0.data(), p, Mt);
+      g.W.insert(g.W.end(), Mt.begin(), Mt.end());
+    }
+  g.wn = T.wn;
+}
+
+void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, Geometry& g) {
+  GeometryPlan P;
+  plan_geometry(s, mp, np, p, iterate_aspect, P);
+  const i64 h = P.h, p2 = p * p;
+  g.p = p; g.h = h; g.mp = mp; g.np = np;
+  g.charts = P.charts;
   const i64 nq = (i64)g.charts.size();
   g.ntri = 2 * nq;
   g.nsrc = g.ntri * h;
   g.sx.assign(3 * g.nsrc, 0.0); g.snx.assign(3 * g.nsrc, 0.0); g.sw.assign(g.nsrc, 0.0);
   g.rts.assign(3 * g.nsrc, 0.0); g.rps.assign(3 * g.nsrc, 0.0);
-  std::vector<double> x0(p), w0(p), D(p * p);
-  gl_nodes(p, x0.data(), w0.data(), D.data());
-  const double ts1 = kPi / (double)np, ts2 = kPi / (double)mp;
-  // interpolation weights for the four (diagonal, side) combinations
-  std::vector<double> Mt[2][2];
-  for (int dg = 0; dg < 2; ++dg)
-    for (int sd = 0; sd < 2; ++sd) {
-      std::vector<double> t1(h), t2(h);
-      for (i64 i = 0; i < h; ++i) corner_map(dg == 0, sd, T.uvn[2 * i], T.uvn[2 * i + 1], &t1[i], &t2[i]);
-      tensor_weights(t1, t2, x0.data(), p, Mt[dg][sd]);
-    }
+  const double* x0 = P.x0.data();
+  const double* D = P.D.data();
   std::vector<double> grid(3 * p2), d2(3 * p2), d1(3 * p2);
   for (i64 k = 0; k < nq; ++k) {
     const Chart& c = g.charts[k];
     for (i64 b = 0; b < p; ++b)
-      for (i64 a = 0; a < p; ++a) chart_eval(s, c, ts1, ts2, x0[b], x0[a], &grid[3 * (a + b * p)]);
+      for (i64 a = 0; a < p; ++a) chart_eval(s, c, P.ts1, P.ts2, x0[b], x0[a], &grid[3 * (a + b * p)]);
     // d/dt2 (first grid index) and d/dt1 (second grid index), spectral
     for (i64 b = 0; b < p; ++b)
       for (i64 a = 0; a < p; ++a)
@@ -210,9 +228,9 @@ void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect
           d2[3 * (a + b * p) + d] = s2;
           d1[3 * (a + b * p) + d] = s1;
         }
-    const int dg = chart_diagonal_A(s, c, ts1, ts2, x0.data(), p) ? 0 : 1;
+    const int dg = P.diag[k];
     for (int sd = 0; sd < 2; ++sd) {
-      const std::vector<double>& W = Mt[dg][sd];
+      const double* W = P.W.data() + (size_t)(2 * dg + sd) * p2 * h;
       const i64 off = (2 * k + sd) * h;
       for (i64 i = 0; i < h; ++i) {
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -235,7 +253,7 @@ void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect
           g.rts[3 * (off + i) + d] = 2.0 * xt[d];
           g.rps[3 * (off + i) + d] = 2.0 * xp[d];
         }
-        g.sw[off + i] = sp * T.wn[i] * 4.0;
+        g.sw[off + i] = sp * P.wn[i] * 4.0;
       }
     }
   }

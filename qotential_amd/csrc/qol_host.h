@@ -58,6 +58,24 @@ struct Geometry {
 };
 
 void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, Geometry& g);
+
+// What the per-chart work needs, built on the host: the chart list (aspect
+// splits), each chart's diagonal (0: A), the Gauss-Legendre grid x0 and its
+// differentiation matrix D (p x p), the tensor interpolation weights of the
+// four (diagonal, side) cases, W[(2*diag + side)*p^2*h + m + i*p^2], and the
+// VR weights wn.  build_geometry (host) and build_geometry_gpu use the same plan.
+struct GeometryPlan {
+  i64 p = 0, h = 0;
+  double ts1 = 0.0, ts2 = 0.0;
+  std::vector<Chart> charts;
+  std::vector<int> diag;
+  std::vector<double> x0, D, W, wn;
+};
+void plan_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, GeometryPlan& g);
+// the per-chart grids, spectral derivatives and node interpolation on the GPU
+// (device 0, default stream); outputs to host arrays laid out as build_geometry's
+void build_geometry_gpu(const Surface& s, const GeometryPlan& P, double* sx, double* snx, double* sw, double* rts,
+                        double* rps);
 // exact chart points of triangle itri (0-based) at reference-triangle (u,v)
 void chart_uv2x(const Surface& s, const Geometry& g, i64 itri, i64 nuv, const double* uv, double* x);
 

@@ -11,11 +11,12 @@ from ._lib import check, dbl, i64, lib, ptr
 VMEC_BENCH = np.array([[0, 0, 1.0, 0.0], [1, 0, 0.3, 0.3], [1, 1, 0.06, -0.06]], dtype=np.float64)
 
 
-def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None):
+def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None, gpu=False):
     """Triangulated surface: dict(sx, snx, sw, rts, rps (nsrc,3|nsrc), ntri, p, h).
 
     modes=None -> the reference's built-in 7-term surface (single aspect pass);
     modes=(nmode,4) array of (m, n, Rc, Zs) -> VMEC form with nfp periods.
+    gpu=True runs the per-chart work on the GPU (qol_geo_build_gpu).
     """
     L = lib()
     nmode = 0 if modes is None else len(modes)
@@ -28,8 +29,9 @@ def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None):
     nsrc = ntri * h
     sx = np.zeros((nsrc, 3)); snx = np.zeros((nsrc, 3)); sw = np.zeros(nsrc)
     rts = np.zeros((nsrc, 3)); rps = np.zeros((nsrc, 3)); charts = np.zeros((ntri // 2, 6))
-    check(L.qol_geo_build(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it), ptr(sx), ptr(snx),
-                          ptr(sw), ptr(rts), ptr(rps), ptr(charts)), "qol_geo_build")
+    build = L.qol_geo_build_gpu if gpu else L.qol_geo_build
+    check(build(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it), ptr(sx), ptr(snx),
+                ptr(sw), ptr(rts), ptr(rps), ptr(charts)), "qol_geo_build_gpu" if gpu else "qol_geo_build")
     return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=charts,
                 modes=modes, nfp=nfp, iterate_aspect=it)
 
