@@ -22,7 +22,7 @@ extern "C" {
 const char* qol_last_error(void);
 int64_t qol_last_status(void);
 
-/* ---------------- workload generation (host only, no GPU) ----------------
+/* ---------------- workload generation (host; qol_geo_build_gpu on the GPU) ----------------
  * Replaces stellarator_geo_ntri / stellarator_geo / stellarator_geo_charts
  * (utils/stellarator_geo_mex.c:548-552, 758-770, 787-877).  nmode == 0 selects
  * the built-in 7-term surface; otherwise mnrz holds nmode rows (m, n, Rc, Zs)
