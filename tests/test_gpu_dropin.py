@@ -108,3 +108,38 @@ def test_getnearquad_triplets_vs_oracle(oracle):
     assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
     for a, b in ((g[2], o[2]), (g[3], o[3])):
         assert np.max(np.abs(a - b)) < 1e-10 * np.abs(b).max()
+
+
+def test_rrq_mex_from_concurrent_threads():
+    """The drop-in is callable from several host threads at once (each call has
+    its own engine and stream; per-order tables are built once under a lock):
+    every thread's result equals the serial one bit for bit."""
+    import threading
+    say("qol_rrq_mex_ from 4 host threads at once")
+    p = 6
+    geo = W.geometry(16, 40, p, modes=W.VMEC_BENCH)
+    jobs = []
+    for j, k in enumerate((11, 222, 403, 777)):
+        sx, snx, sw, rts, rps = _panel(geo, k)
+        jobs.append((_targets(sx, snx, sw, n=30, seed=40 + j), sx, snx, sw, rts, rps))
+    serial = [R.rrq(*a, p) for a in jobs]
+    out = [None] * len(jobs)
+    errs = []
+
+    def run(i):
+        try:
+            for _ in range(3):
+                out[i] = R.rrq(*jobs[i], p)
+        except Exception as e:  # reported by the main thread
+            errs.append(repr(e))
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(jobs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errs, errs
+    assert all(not t.is_alive() for t in th)
+    for a, b in zip(out, serial):
+        for key in ("As", "Ad", "Omega", "Ias"):
+            assert np.array_equal(a[key], b[key]), key
