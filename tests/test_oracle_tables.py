@@ -26,7 +26,7 @@ def test_vr_rules_exact(oracle):
             assert abs((w * uv[:, 0] ** a * uv[:, 1] ** b).sum() - ex) < 1e-14
 
 
-def test_level_counts_match_survey(oracle):
+def test_level_counts_per_order(oracle):
     # node and sub-patch counts per level (patch_refine_mod.f90:100-421 restated)
     expect = {4: [21, 78, 300, 825], 6: [36, 136, 528, 1444], 8: [55, 174, 604, 1767],
               10: [78, 256, 912, 2626], 12: [105, 354, 1284, 3657]}
