@@ -272,7 +272,7 @@ void Engine::get_pairs(i64* offs, i64* idx1) {
 }
 
 void Engine::build_levels() {
-  d_lgeo.alloc(sizeof(double) * 7 * (size_t)tab->ntot * npan);
+  d_lgeo.alloc(sizeof(double) * 8 * (size_t)tab->ntot * npan);
   d_balls.alloc(sizeof(double) * 4 * (size_t)tab->nsubtot * npan);
   launch_level_geom(tab->LA, npan, (const double*)d_sx.ptr, (const double*)d_rts.ptr, (const double*)d_rps.ptr,
                     (double*)d_lgeo.ptr, (double*)d_balls.ptr, stream);
@@ -560,7 +560,16 @@ void Engine::far(const double* ub, const double* ubn, double* u) {
 void Engine::level_data(double* lgeo, double* balls) {
   build_levels();
   HIPCHK(hipStreamSynchronize(stream));
-  if (lgeo) HIPCHK(hipMemcpy(lgeo, d_lgeo.ptr, sizeof(double) * 7 * (size_t)tab->ntot * npan, hipMemcpyDeviceToHost));
+  if (lgeo) {
+    // device layout: 8 doubles per (panel, node): x, y, z, nx, ny, nz, w, pad;
+    // the caller's: [panel][7][ntot]
+    const size_t nt_ = (size_t)tab->ntot, n = nt_ * npan;
+    std::vector<double> g(8 * n);
+    HIPCHK(hipMemcpy(g.data(), d_lgeo.ptr, sizeof(double) * 8 * n, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < (size_t)npan; ++k)
+      for (size_t i = 0; i < nt_; ++i)
+        for (int c = 0; c < 7; ++c) lgeo[(k * 7 + c) * nt_ + i] = g[8 * (k * nt_ + i) + c];
+  }
   if (balls) HIPCHK(hipMemcpy(balls, d_balls.ptr, sizeof(double) * 4 * (size_t)tab->nsubtot * npan, hipMemcpyDeviceToHost));
 }
 

@@ -22,7 +22,8 @@ namespace qol {
 // written to HBM (plain stores: streaming ones measured slower) and kept in LDS,
 // and the balls of its sub-patches are summed from LDS in node order, so nothing
 // is read back.
-// Output: lgeo[panel][7][ntot] (x,y,z,nx,ny,nz,w), balls[panel][nsubtot][4].
+// Output: lgeo[panel][ntot][8] (x,y,z,nx,ny,nz,w, pad: 64 bytes per node, so one
+// node step of the direct kernel reads two cache lines), balls[panel][nsubtot][4].
 constexpr int kLP = 4;
 constexpr int kChunk = 256;
 
@@ -139,14 +140,11 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
         const double n2 = gp[6] * gp[4] - gp[7] * gp[3];
         const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
         const double w = sp * A.wt[off + i];
-        double* G = lgeo + (size_t)(k0 + pp) * 7 * nt_ + off + i;
-        G[0] = gp[0];
-        G[nt_] = gp[1];
-        G[2 * nt_] = gp[2];
-        G[3 * nt_] = n0 / sp;
-        G[4 * nt_] = n1 / sp;
-        G[5 * nt_] = n2 / sp;
-        G[6 * nt_] = w;
+        double2* G = (double2*)(lgeo + 8 * ((size_t)(k0 + pp) * nt_ + off + i));
+        G[0] = double2{gp[0], gp[1]};
+        G[1] = double2{gp[2], n0 / sp};
+        G[2] = double2{n1 / sp, n2 / sp};
+        G[3] = double2{w, 0.0};
         double* L = cn + (size_t)pp * 4 * kChunk + li;
         L[0] = gp[0];
         L[kChunk] = gp[1];
@@ -242,14 +240,11 @@ __global__ void __launch_bounds__(256) level_geom_mfma_kernel(LevelArgs A, int n
           const double n2 = gp[6] * gp[4] - gp[7] * gp[3];
           const double sp = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
           const double w = sp * A.wt[off + i];
-          double* G = lgeo + (size_t)(k0 + pp) * 7 * nt_ + off + i;
-          G[0] = gp[0];
-          G[nt_] = gp[1];
-          G[2 * nt_] = gp[2];
-          G[3 * nt_] = n0 / sp;
-          G[4 * nt_] = n1 / sp;
-          G[5 * nt_] = n2 / sp;
-          G[6 * nt_] = w;
+          double2* G = (double2*)(lgeo + 8 * ((size_t)(k0 + pp) * nt_ + off + i));
+          G[0] = double2{gp[0], gp[1]};
+          G[1] = double2{gp[2], n0 / sp};
+          G[2] = double2{n1 / sp, n2 / sp};
+          G[3] = double2{w, 0.0};
           double* L = cn + (size_t)pp * 4 * kChunk + (i - c0);
           L[0] = gp[0];
           L[kChunk] = gp[1];
@@ -434,7 +429,7 @@ __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int*
   if (live) {
     const size_t t = (size_t)tidx[pair];
     x0 = tx[3 * t]; x1 = tx[3 * t + 1]; x2 = tx[3 * t + 2];
-    G = lgeo + (size_t)ppanel[pair] * 7 * ntot + A.lev_off[l];
+    G = lgeo + 8 * ((size_t)ppanel[pair] * ntot + A.lev_off[l]);
   }
   const int n = A.lev_n[l];
   const double* B = A.imp + (size_t)pad * A.lev_off[l];
@@ -450,8 +445,9 @@ __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int*
     const int i = k0 + kq;
     const bool in = i < n;
     const int ic = in ? i : n - 1;
-    const double gx = G[ic], gy = G[ntot + ic], gz = G[2 * ntot + ic];
-    const double nx = G[3 * ntot + ic], ny = G[4 * ntot + ic], nz = G[5 * ntot + ic], gw = G[6 * ntot + ic];
+    const double2* g2 = (const double2*)(G + 8 * ic);
+    const double2 ga = g2[0], gb = g2[1], gc = g2[2], gd = g2[3];
+    const double gx = ga.x, gy = ga.y, gz = gb.x, nx = gb.y, ny = gc.x, nz = gc.y, gw = gd.x;
     double bq[NT];
 #pragma unroll
     for (int q = 0; q < NT; ++q) bq[q] = B[(size_t)ic * pad + 16 * q + m];
