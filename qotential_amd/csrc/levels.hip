@@ -204,12 +204,22 @@ __global__ void __launch_bounds__(256) level_geom_mfma_kernel(LevelArgs A, int n
     const int ntile = (c1 - c0 + 15) / 16;
     for (int tl = wv; tl < ntile; tl += 4) {
       // A operand: lane supplies A[li][lk] of block s = IM(node c0 + 16 tl + li, q = 4s + lk),
-      // from the row-major copy zero-padded to whole 16-entry blocks
+      // from the column-major copy, so one load reads 16 consecutive nodes of 4
+      // columns (4 cache lines instead of the 16 rows of the row-major copy);
+      // columns past h are zero
       const int nd = min(c0 + 16 * tl + li, c1 - 1);
-      const double* row = A.imp + (size_t)(off + nd) * A.pad;
+      const int nl = A.lev_n[l], st = 4 * nl;  // 4 columns
+      const double* cb = A.interp + (size_t)h * off + nd;  // column 0 of this lane's node
+      const double* col = cb + (size_t)nl * lk;
       double af[KS];
 #pragma unroll
-      for (int s = 0; s < KS; ++s) af[s] = row[4 * s + lk];
+      for (int s = 0; s < KS - 1; ++s) af[s] = col[s * st];
+      {
+        // only the last block can pass column h: the load stays in the table
+        const int q = 4 * (KS - 1) + lk;
+        const double v = cb[(size_t)nl * min(q, h - 1)];
+        af[KS - 1] = q < h ? v : 0.0;
+      }
       qd4 acc[3];
 #pragma unroll
       for (int t = 0; t < 3; ++t) acc[t] = qd4{0.0, 0.0, 0.0, 0.0};
