@@ -316,10 +316,11 @@ void Engine::eval_levels_only() {
 // Stages and streams (all at the default priority).  The per-panel RRQ set-up
 // (frame kernel, then system kernel) and solve need only the surface, so they
 // run for every panel on s2 from the start, beside the level geometry and
-// classification on the main stream.  After the pair lists: the direct levels on
-// s1 (matrix cores) and the per-pair projection, root/SSQ and solid angle on s3
-// (after the frame kernel); the moments join s3 and the system kernel, the
-// weights the solve, and the main stream joins s1 last.
+// classification on the main stream.  After the pair lists: the per-pair
+// projection, root/SSQ and solid angle on s3 (after the frame kernel), and the
+// direct levels on s1 (matrix cores) once the projection is done; the moments
+// join s3 and the system kernel, the weights the solve, and the main stream
+// joins s1 last.
 // tev[i] brackets stage i: levels, classify+lists, direct, rrq setup, rrq solve,
 // projection, root + SSQ (solid angle from tev[8][0]), moments, -, weights.
 void Engine::eval(int exterior, double side_tol, bool want_omega) {
@@ -380,15 +381,6 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   classify_lists();
   rec(1, 1, stream);
   HIPCHK(hipEventRecord(ev_lists, stream));
-  // s1: direct levels
-  HIPCHK(hipStreamWaitEvent(s1, ev_lists, 0));
-  rec(2, 0, s1);
-  direct(s1);
-  rec(2, 1, s1);
-  // with the density resident, the level-served pairs' corrections follow at once
-  const bool fuse = density_set;
-  if (fuse) launch_correction(0, 3, s1);
-  HIPCHK(hipEventRecord(ev_direct, s1));
   // s3: projection, root + SSQ, solid angle (needs the lists and the set-up)
   HIPCHK(hipStreamWaitEvent(s3, ev_lists, 0));
   HIPCHK(hipStreamWaitEvent(s3, ev_setup, 0));
@@ -397,6 +389,16 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol,
                      (double*)d_pq.ptr, (int*)d_eorder.ptr, nnz, s3);
   rec(5, 1, s3);
+  // s1: direct levels, after the projection: the projection heads the chain the
+  // moments wait for, and the direct kernel's 121k waves would crowd it out
+  HIPCHK(hipStreamWaitEvent(s1, tev[5][1], 0));
+  rec(2, 0, s1);
+  direct(s1);
+  rec(2, 1, s1);
+  // with the density resident, the level-served pairs' corrections follow at once
+  const bool fuse = density_set;
+  if (fuse) launch_correction(0, 3, s1);
+  HIPCHK(hipEventRecord(ev_direct, s1));
   rec(6, 0, s3);
   launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                   (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
