@@ -26,8 +26,12 @@ namespace qol {
 // node step of the direct kernel reads two cache lines), balls[panel][nsubtot][4].
 constexpr int kLP = 4;
 constexpr int kChunk = 256;
+// LDS stride of one panel's chunk buffer (4 x kChunk doubles): 8 doubles of
+// padding put the kLP panels of one node in different LDS banks (an unpadded
+// stride, a multiple of the bank count, serialised their accesses 4 ways)
+constexpr int kCP = 4 * kChunk + 8;
 
-size_t level_geom_smem(int h) { return sizeof(double) * (size_t)kLP * 4 * kChunk; }
+size_t level_geom_smem(int h) { return sizeof(double) * (size_t)kLP * kCP; }
 int level_geom_blocks(int npan) { return (npan + kLP - 1) / kLP; }
 
 std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host) {
@@ -55,7 +59,7 @@ __device__ __forceinline__ void chunk_balls(const LevelArgs& A, const double* cn
     const int pp = e / nsc, s = sb0 + e % nsc;
     const int s_all = A.sub_base[l] + s;
     const int i0 = A.suboff[A.sub_base[l] + l + s] - c0, i1 = A.suboff[A.sub_base[l] + l + s + 1] - c0;
-    const double* L = cn + (size_t)pp * 4 * kChunk;
+    const double* L = cn + (size_t)pp * kCP;
     // four independent sums, each in node order (the oracle's), in one pass
     double swsum = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int j = i0; j < i1; ++j) {
@@ -97,7 +101,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
   const int h = A.h, k0 = blockIdx.x * kLP;
   const int np_ = min(kLP, npan - k0);
   const size_t nt_ = A.ntot;
-  double* cn = smem;  // [kLP][4][kChunk]: x, y, z, w of the chunk's nodes
+  double* cn = smem;  // [kLP][kCP]: x, y, z, w of the chunk's nodes (4 x kChunk) and padding
   for (int ch = 0; ch < A.nchunk; ++ch) {
     const int* C = A.chunks + 5 * ch;
     const int l = C[0], c0 = C[1], c1 = C[2], sb0 = C[3], sb1 = C[4];
@@ -145,7 +149,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
         G[1] = double2{gp[2], n0 / sp};
         G[2] = double2{n1 / sp, n2 / sp};
         G[3] = double2{w, 0.0};
-        double* L = cn + (size_t)pp * 4 * kChunk + li;
+        double* L = cn + (size_t)pp * kCP + li;
         L[0] = gp[0];
         L[kChunk] = gp[1];
         L[2 * kChunk] = gp[2];
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
 // (B operand) stay in registers for the whole workgroup; a tile's results pass
 // through a small per-wave LDS stage to one lane per (node, panel).
 constexpr int kLV = 4 * 9;  // output values per node (kLP panels x 9)
-size_t level_geom_mfma_smem() { return sizeof(double) * ((size_t)kLP * 4 * kChunk + (size_t)4 * 16 * kLV); }
+size_t level_geom_mfma_smem() { return sizeof(double) * ((size_t)kLP * kCP + (size_t)4 * 16 * kLV); }
 
 template <int KS>
 __global__ void __launch_bounds__(256) level_geom_mfma_kernel(LevelArgs A, int npan, const double* __restrict__ sx,
@@ -181,8 +185,8 @@ __global__ void __launch_bounds__(256) level_geom_mfma_kernel(LevelArgs A, int n
   const int np_ = min(kLP, npan - k0);
   const size_t nt_ = A.ntot;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
-  double* cn = smem;                                                   // [kLP][4][kChunk]
-  double* stg = smem + (size_t)kLP * 4 * kChunk + (size_t)wv * 16 * kLV;  // [16 nodes][kLV], this wave
+  double* cn = smem;                                           // [kLP][kCP]
+  double* stg = smem + (size_t)kLP * kCP + (size_t)wv * 16 * kLV;  // [16 nodes][kLV], this wave
   // B operand: lane supplies B[lk][li] of block s, tile t = S(q = 4s + lk, column 16t + li)
   double bf[KS][3];
 #pragma unroll
@@ -255,7 +259,7 @@ __global__ void __launch_bounds__(256) level_geom_mfma_kernel(LevelArgs A, int n
           G[1] = double2{gp[2], n0 / sp};
           G[2] = double2{n1 / sp, n2 / sp};
           G[3] = double2{w, 0.0};
-          double* L = cn + (size_t)pp * 4 * kChunk + (i - c0);
+          double* L = cn + (size_t)pp * kCP + (i - c0);
           L[0] = gp[0];
           L[kChunk] = gp[1];
           L[2 * kChunk] = gp[2];
