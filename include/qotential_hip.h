@@ -48,6 +48,30 @@ int qol_geo_build_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nm
 int qol_geo_chart_boundary(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
                            const double* mnrz, int64_t iterate_aspect, int64_t nquad,
                            double* sxbd, double* rv);
+/* The adaptive chart list of stellarator_charts (utils/stellarator_geo_mex.c:
+ * 779-808, build_charts_adaptive :427-546): aspect splits iterated, then for
+ * restol > 0 four-way splits until the mean curvature's tensor-Legendre tail
+ * (panel_resolution, :359-411) is at most restol, then 2:1 balance between face
+ * neighbours.  Returns the chart count (charts(6, n): a1 b1 a2 b2 jj ii; may be
+ * NULL to count only), -1 when the list would pass cap (cap <= 0: 200000), -2 on
+ * error.  A triangle list has 2 n triangles. */
+int64_t qol_geo_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                       const double* mnrz, double restol, int64_t cap, double* charts);
+/* panel_resolution of one chart (6 values as above); -1 on error */
+double qol_geo_curvature_tail(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                              const double* mnrz, const double* chart);
+/* the geometry and the chart boundary of a given chart list, as qol_geo_build,
+ * qol_geo_build_gpu and qol_geo_chart_boundary (stellarator_geo_charts,
+ * utils/stellarator_geo_mex.c:837-877) */
+int qol_geo_build_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                         const double* mnrz, int64_t nchart, const double* charts, double* sx,
+                         double* snx, double* sw, double* rts, double* rps);
+int qol_geo_build_charts_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                             const double* mnrz, int64_t nchart, const double* charts, double* sx,
+                             double* snx, double* sw, double* rts, double* rps);
+int qol_geo_chart_boundary_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                                  const double* mnrz, int64_t nchart, const double* charts,
+                                  int64_t nquad, double* sxbd, double* rv);
 /* near-surface targets of the benchmark workloads (no reference counterpart:
  * the reference drivers use the surface nodes themselves as targets) */
 int qol_make_targets(int64_t p, int64_t ntri, const double* sx, const double* snx,

@@ -178,6 +178,29 @@ int qol_geo_build_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nm
   });
 }
 
+int qol_geo_build_charts_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                             int64_t nchart, const double* charts, double* sx, double* snx, double* sw, double* rts,
+                             double* rps) {
+  return guarded([&] {
+    if (nchart < 0 || (nchart > 0 && !charts)) throw std::runtime_error("qol_geo_build_charts_gpu: bad chart list");
+    Surface s;
+    s.nfp = nfp;
+    for (int64_t k = 0; k < nmode; ++k) {
+      s.m.push_back(mnrz[4 * k]);
+      s.n.push_back(mnrz[4 * k + 1]);
+      s.rc.push_back(mnrz[4 * k + 2]);
+      s.zs.push_back(mnrz[4 * k + 3]);
+    }
+    std::vector<Chart> L(nchart);
+    for (int64_t k = 0; k < nchart; ++k)
+      L[k] = Chart{charts[6 * k], charts[6 * k + 1], charts[6 * k + 2], charts[6 * k + 3], charts[6 * k + 4],
+                   charts[6 * k + 5]};
+    GeometryPlan P;
+    plan_geometry(s, mp, np, p, L, P);
+    build_geometry_gpu(s, P, sx, snx, sw, rts, rps);
+  });
+}
+
 int qol_hbm_copy(int64_t nbytes, double* ms) {
   return guarded([&] {
     if (nbytes <= 0 || nbytes % 16) throw std::runtime_error("qol_hbm_copy: nbytes must be a positive multiple of 16");

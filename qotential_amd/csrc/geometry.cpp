@@ -5,11 +5,14 @@
 //   surface evaluation          utils/stellarator_geo_mex.c:60-102 (built-in 7-term
 //                               surface and the VMEC Fourier form)
 //   chart list, aspect splits   :124-159 (single pass) and :427-479 (iterated)
+//   curvature refinement, 2:1   :282-301, 359-411, 480-546 (restol > 0)
+//   balance
 //   chart -> two triangles      :636-749 (tensor Gauss-Legendre grid, spectral
 //                               derivatives, diagonal choice, sw = |t1 x t2| w 4)
 //   exact chart boundary        :555-582 (uv2x_core), used for ichart = 1
 // The 1-D tensor interpolation uses the barycentric form; the reference keeps a
 // monomial solve (BARYCENTRIC 0, :147), which agrees to rounding for p <= 16.
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -114,7 +117,217 @@ bool chart_diagonal_A(const Surface& s, const Chart& c, double ts1, double ts2, 
   return d1 + 1e-13 > d2;
 }
 
+// V(i, k) = P_k(x0_i), the Legendre polynomials at the p grid nodes (column-major)
+void legendre_vandermonde(const double* x0, i64 p, double* V) {
+  for (i64 i = 0; i < p; ++i) {
+    V[i] = 1.0;
+    if (p > 1) V[i + p] = x0[i];
+    for (i64 k = 1; k + 1 < p; ++k)
+      V[i + (k + 1) * p] = ((2.0 * k + 1.0) * x0[i] * V[i + k * p] - k * V[i + (k - 1) * p]) / (k + 1.0);
+  }
+}
+
+// inverse of a p x p column-major matrix: LU with partial pivoting, then p solves
+std::vector<double> lu_inverse(std::vector<double> A, i64 p) {
+  std::vector<double> X(p * p, 0.0);
+  for (i64 i = 0; i < p; ++i) X[i + i * p] = 1.0;
+  for (i64 k = 0; k < p; ++k) {
+    i64 piv = k;
+    for (i64 i = k + 1; i < p; ++i)
+      if (std::fabs(A[i + k * p]) > std::fabs(A[piv + k * p])) piv = i;
+    if (piv != k)
+      for (i64 j = 0; j < p; ++j) {
+        std::swap(A[k + j * p], A[piv + j * p]);
+        std::swap(X[k + j * p], X[piv + j * p]);
+      }
+    for (i64 i = k + 1; i < p; ++i) {
+      const double f = A[i + k * p] / A[k + k * p];
+      for (i64 j = k + 1; j < p; ++j) A[i + j * p] -= f * A[k + j * p];
+      for (i64 j = 0; j < p; ++j) X[i + j * p] -= f * X[k + j * p];
+    }
+  }
+  for (i64 j = 0; j < p; ++j)
+    for (i64 i = p - 1; i >= 0; --i) {
+      double v = X[i + j * p];
+      for (i64 k = i + 1; k < p; ++k) v -= A[i + k * p] * X[k + j * p];
+      X[i + j * p] = v / A[i + i * p];
+    }
+  return X;
+}
+
+// the four quarters of a chart: quarter q takes the upper half in t1 when q & 1
+// and in t2 when q & 2
+Chart quarter(const Chart& c, int q) {
+  Chart k = c;
+  k.a1 = c.a1 / 2.0;
+  k.a2 = c.a2 / 2.0;
+  k.b1 = c.b1 + ((q & 1) ? c.a1 / 2.0 : -c.a1 / 2.0);
+  k.b2 = c.b2 + ((q & 2) ? c.a2 / 2.0 : -c.a2 / 2.0);
+  return k;
+}
+
+// parameter box of a chart: [t_lo, t_hi, p_lo, p_hi] in surface angles
+void chart_box(const Chart& c, double ts1, double ts2, double* bx) {
+  bx[0] = ((c.b1 - c.a1) + 2.0 * c.jj) * ts1;
+  bx[1] = ((c.b1 + c.a1) + 2.0 * c.jj) * ts1;
+  bx[2] = ((c.b2 - c.a2) + 2.0 * c.ii) * ts2;
+  bx[3] = ((c.b2 + c.a2) + 2.0 * c.ii) * ts2;
+}
+
+struct CurvaturePlan {
+  i64 p;
+  double ts1, ts2;
+  std::vector<double> x0, D, Vinv;
+  CurvaturePlan(i64 mp, i64 np, i64 p_) : p(p_), ts1(kPi / (double)np), ts2(kPi / (double)mp) {
+    x0.assign(p, 0.0);
+    D.assign(p * p, 0.0);
+    std::vector<double> w0(p), V(p * p);
+    gl_nodes(p, x0.data(), w0.data(), D.data());
+    legendre_vandermonde(x0.data(), p, V.data());
+    Vinv = lu_inverse(V, p);
+  }
+};
+
+// Mean curvature H = (E N - 2 F M + G L) / (2 (E G - F^2)) at the p x p grid
+// nodes from spectral first and second derivatives of the chart map; its tensor
+// Legendre coefficients C = V^-1 H V^-T; the result is the root of the share of
+// sum C^2 held by degrees p-2 and p-1 in either variable.
+double curvature_tail(const Surface& s, const Chart& c, const CurvaturePlan& P) {
+  const i64 p = P.p, n2 = p * p;
+  const double* D = P.D.data();
+  std::vector<double> X(3 * n2), X1(3 * n2), X2(3 * n2), X11(3 * n2), X12(3 * n2), X22(3 * n2);
+  // grid node m = a + b p sits at (t1, t2) = (x0[b], x0[a])
+  for (i64 b = 0; b < p; ++b)
+    for (i64 a = 0; a < p; ++a) chart_eval(s, c, P.ts1, P.ts2, P.x0[b], P.x0[a], &X[3 * (a + b * p)]);
+  // d/dt1 runs over b, d/dt2 over a
+  auto d_t1 = [&](const std::vector<double>& f, std::vector<double>& g) {
+    for (i64 b = 0; b < p; ++b)
+      for (i64 a = 0; a < p; ++a)
+        for (int d = 0; d < 3; ++d) {
+          double v = 0.0;
+          for (i64 k = 0; k < p; ++k) v += D[b + k * p] * f[3 * (a + k * p) + d];
+          g[3 * (a + b * p) + d] = v;
+        }
+  };
+  auto d_t2 = [&](const std::vector<double>& f, std::vector<double>& g) {
+    for (i64 b = 0; b < p; ++b)
+      for (i64 a = 0; a < p; ++a)
+        for (int d = 0; d < 3; ++d) {
+          double v = 0.0;
+          for (i64 k = 0; k < p; ++k) v += D[a + k * p] * f[3 * (k + b * p) + d];
+          g[3 * (a + b * p) + d] = v;
+        }
+  };
+  d_t1(X, X1);
+  d_t2(X, X2);
+  d_t1(X1, X11);
+  d_t1(X2, X12);
+  d_t2(X2, X22);
+  std::vector<double> H(n2), T(n2);
+  for (i64 m = 0; m < n2; ++m) {
+    const double* u = &X1[3 * m];
+    const double* v = &X2[3 * m];
+    double nv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    double nn = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    if (nn <= 0.0) nn = 1.0;
+    for (double& x : nv) x /= nn;
+    auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    const double E = dot(u, u), F = dot(u, v), G = dot(v, v);
+    const double L = dot(nv, &X11[3 * m]), M = dot(nv, &X12[3 * m]), N = dot(nv, &X22[3 * m]);
+    double den = 2.0 * (E * G - F * F);
+    if (den == 0.0) den = 1.0;
+    H[m] = (E * N - 2.0 * F * M + G * L) / den;
+  }
+  // C = Vinv H Vinv^T, H(a, b) = H[a + b p]
+  const double* W = P.Vinv.data();
+  for (i64 b = 0; b < p; ++b)
+    for (i64 a = 0; a < p; ++a) {
+      double v = 0.0;
+      for (i64 k = 0; k < p; ++k) v += W[a + k * p] * H[k + b * p];
+      T[a + b * p] = v;
+    }
+  double tail = 0.0, tot = 0.0;
+  for (i64 b = 0; b < p; ++b)
+    for (i64 a = 0; a < p; ++a) {
+      double v = 0.0;
+      for (i64 k = 0; k < p; ++k) v += T[a + k * p] * W[b + k * p];
+      tot += v * v;
+      if (a >= p - 2 || b >= p - 2) tail += v * v;
+    }
+  return std::sqrt(tail / (tot > 0.0 ? tot : 1.0));
+}
+
 }  // namespace
+
+double chart_curvature_tail(const Surface& s, const Chart& c, i64 mp, i64 np, i64 p) {
+  return curvature_tail(s, c, CurvaturePlan(mp, np, p));
+}
+
+std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p, double restol, i64 cap,
+                                        bool* over) {
+  if (cap <= 0) cap = 200000;
+  *over = false;
+  std::vector<Chart> L = build_charts(s, mp, np, p, true);
+  if ((i64)L.size() > cap) { *over = true; return {}; }
+  if (!(restol > 0.0)) return L;
+  const CurvaturePlan P(mp, np, p);
+  // curvature: split in four until the tail passes; a chart that passed is not
+  // tested again (at most 12 levels)
+  std::vector<char> ok(L.size(), 0);
+  for (int level = 0; level < 12; ++level) {
+    const i64 n0 = (i64)L.size();
+    i64 nsplit = 0;
+    for (i64 k = 0; k < n0; ++k) {
+      if (ok[k]) continue;
+      if (curvature_tail(s, L[k], P) <= restol) { ok[k] = 1; continue; }
+      if ((i64)L.size() + 3 > cap) { *over = true; return {}; }
+      const Chart c = L[k];
+      L[k] = quarter(c, 0);
+      for (int q = 1; q < 4; ++q) { L.push_back(quarter(c, q)); ok.push_back(0); }
+      ++nsplit;
+    }
+    if (!nsplit) break;
+  }
+  // 2:1 balance: split a chart whose extent along a face it shares with a
+  // neighbour (one base cell apart at most, periodic in both angles) exceeds
+  // twice the neighbour's; sweeps until none is split (at most 20)
+  const double et1 = 1e-9 * P.ts1, et2 = 1e-9 * P.ts2;
+  for (int sweep = 0; sweep < 20; ++sweep) {
+    const i64 n0 = (i64)L.size();
+    std::vector<double> box(4 * n0);
+    for (i64 k = 0; k < n0; ++k) chart_box(L[k], P.ts1, P.ts2, &box[4 * k]);
+    std::vector<char> split(n0, 0);
+    for (i64 k = 0; k < n0; ++k) {
+      const double* bk = &box[4 * k];
+      for (i64 q = 0; q < n0 && !split[k]; ++q) {
+        if (q == k) continue;
+        const double dj = std::fabs(L[q].jj - L[k].jj), di = std::fabs(L[q].ii - L[k].ii);
+        if (dj > 1.5 && std::fabs(dj - (double)np) > 1.5) continue;
+        if (di > 1.5 && std::fabs(di - (double)mp) > 1.5) continue;
+        const double* bq = &box[4 * q];
+        if ((std::fabs(bk[1] - bq[0]) < et1 || std::fabs(bk[0] - bq[1]) < et1) &&
+            std::fmin(bk[3], bq[3]) - std::fmax(bk[2], bq[2]) > et2) {
+          // a face at constant t1: compare the extents in t2
+          if (bk[3] - bk[2] > 2.0 * (bq[3] - bq[2]) + et2) split[k] = 1;
+        } else if ((std::fabs(bk[3] - bq[2]) < et2 || std::fabs(bk[2] - bq[3]) < et2) &&
+                   std::fmin(bk[1], bq[1]) - std::fmax(bk[0], bq[0]) > et1) {
+          if (bk[1] - bk[0] > 2.0 * (bq[1] - bq[0]) + et1) split[k] = 1;
+        }
+      }
+    }
+    i64 nsplit = 0;
+    for (i64 k = 0; k < n0; ++k) {
+      if (!split[k]) continue;
+      if ((i64)L.size() + 3 > cap) { *over = true; return {}; }
+      const Chart c = L[k];
+      L[k] = quarter(c, 0);
+      for (int q = 1; q < 4; ++q) L.push_back(quarter(c, q));
+      ++nsplit;
+    }
+    if (!nsplit) break;
+  }
+  return L;
+}
 
 std::vector<Chart> build_charts(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect) {
   std::vector<double> x0(p), w0(p);
@@ -175,12 +388,16 @@ std::vector<Chart> build_charts(const Surface& s, i64 mp, i64 np, i64 p, bool it
 }
 
 void plan_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, GeometryPlan& g) {
+  plan_geometry(s, mp, np, p, build_charts(s, mp, np, p, iterate_aspect), g);
+}
+
+void plan_geometry(const Surface& s, i64 mp, i64 np, i64 p, const std::vector<Chart>& charts, GeometryPlan& g) {
   const OrderTables& T = order_tables(p);
   const i64 h = T.h;
   g.p = p; g.h = h;
   g.ts1 = kPi / (double)np;
   g.ts2 = kPi / (double)mp;
-  g.charts = build_charts(s, mp, np, p, iterate_aspect);
+  g.charts = charts;
   g.x0.assign(p, 0.0);
   g.D.assign(p * p, 0.0);
   std::vector<double> w0(p);
@@ -201,8 +418,12 @@ void plan_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect,
 }
 
 void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, Geometry& g) {
+  build_geometry(s, mp, np, p, build_charts(s, mp, np, p, iterate_aspect), g);
+}
+
+void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, const std::vector<Chart>& charts, Geometry& g) {
   GeometryPlan P;
-  plan_geometry(s, mp, np, p, iterate_aspect, P);
+  plan_geometry(s, mp, np, p, charts, P);
   const i64 h = P.h, p2 = p * p;
   g.p = p; g.h = h; g.mp = mp; g.np = np;
   g.charts = P.charts;

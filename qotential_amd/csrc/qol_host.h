@@ -50,6 +50,12 @@ struct Chart {
 };
 
 std::vector<Chart> build_charts(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect);
+// iterated aspect splits, then, for restol > 0, curvature refinement and 2:1
+// balance (utils/stellarator_geo_mex.c:427-546); empty when the list would pass cap
+std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p, double restol, i64 cap, bool* over);
+// the relative weight of the last two degrees in the tensor-Legendre spectrum of
+// the chart's mean curvature on its p x p grid (utils/stellarator_geo_mex.c:359-411)
+double chart_curvature_tail(const Surface& s, const Chart& c, i64 mp, i64 np, i64 p);
 
 struct Geometry {
   i64 p = 0, h = 0, ntri = 0, nsrc = 0, mp = 0, np = 0;
@@ -58,6 +64,7 @@ struct Geometry {
 };
 
 void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, Geometry& g);
+void build_geometry(const Surface& s, i64 mp, i64 np, i64 p, const std::vector<Chart>& charts, Geometry& g);
 
 // What the per-chart work needs, built on the host: the chart list (aspect
 // splits), each chart's diagonal (0: A), the Gauss-Legendre grid x0 and its
@@ -72,6 +79,7 @@ struct GeometryPlan {
   std::vector<double> x0, D, W, wn;
 };
 void plan_geometry(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect, GeometryPlan& g);
+void plan_geometry(const Surface& s, i64 mp, i64 np, i64 p, const std::vector<Chart>& charts, GeometryPlan& g);
 // the per-chart grids, spectral derivatives and node interpolation on the GPU
 // (device 0, default stream); outputs to host arrays laid out as build_geometry's
 void build_geometry_gpu(const Surface& s, const GeometryPlan& P, double* sx, double* snx, double* sw, double* rts,

@@ -30,6 +30,53 @@ Surface make_surface(int64_t nfp, int64_t nmode, const double* mnrz) {
   return s;
 }
 
+std::vector<Chart> unpack_charts(int64_t nchart, const double* cl) {
+  if (nchart < 0 || (nchart > 0 && !cl)) throw std::runtime_error("chart list: bad count or null pointer");
+  std::vector<Chart> L(nchart);
+  for (int64_t k = 0; k < nchart; ++k)
+    L[k] = Chart{cl[6 * k], cl[6 * k + 1], cl[6 * k + 2], cl[6 * k + 3], cl[6 * k + 4], cl[6 * k + 5]};
+  return L;
+}
+
+void copy_geometry(const Geometry& g, double* sx, double* snx, double* sw, double* rts, double* rps) {
+  std::memcpy(sx, g.sx.data(), sizeof(double) * g.sx.size());
+  std::memcpy(snx, g.snx.data(), sizeof(double) * g.snx.size());
+  std::memcpy(sw, g.sw.data(), sizeof(double) * g.sw.size());
+  std::memcpy(rts, g.rts.data(), sizeof(double) * g.rts.size());
+  std::memcpy(rps, g.rps.data(), sizeof(double) * g.rps.size());
+}
+
+void chart_boundary(const Surface& s, int64_t mp, int64_t np, int64_t p, std::vector<Chart> charts, int64_t nquad,
+                    double* sxbd, double* rv) {
+  Geometry g;
+  g.p = p; g.mp = mp; g.np = np;
+  g.charts = std::move(charts);
+  g.ntri = 2 * (int64_t)g.charts.size();
+  std::vector<double> t(nquad), w(nquad);
+  gl_nodes(nquad, t.data(), w.data(), nullptr);
+  std::vector<double> uv(2 * (3 * nquad + 3));
+  const double pi = 3.14159265358979323846, th1 = 2.0 * pi / 3.0, th2 = 4.0 * pi / 3.0, ti1 = 1.0 / th1;
+  for (int64_t k = 0; k < 3; ++k)
+    for (int64_t i = 0; i < nquad; ++i) {
+      const double t0 = (double)k * 2.0 * pi / 3.0, t1 = (double)(k + 1) * 2.0 * pi / 3.0;
+      const double tt = t0 + 0.5 * (1.0 + t[i]) * (t1 - t0);
+      double u = 0.0, v = 0.0;
+      if (tt < th1) v = 1.0 - ti1 * tt;
+      else if (tt < th2) u = ti1 * (tt - th1);
+      else { u = 1.0 - ti1 * (tt - th2); v = ti1 * (tt - th2); }
+      uv[2 * (k * nquad + i)] = u;
+      uv[2 * (k * nquad + i) + 1] = v;
+    }
+  const double vert[6] = {0, 0, 1, 0, 0, 1};
+  for (int i = 0; i < 6; ++i) uv[2 * 3 * nquad + i] = vert[i];
+  std::vector<double> x(3 * (3 * nquad + 3));
+  for (int64_t it = 0; it < g.ntri; ++it) {
+    chart_uv2x(s, g, it, 3 * nquad + 3, uv.data(), x.data());
+    std::memcpy(sxbd + 9 * nquad * it, x.data(), sizeof(double) * 9 * nquad);
+    std::memcpy(rv + 9 * it, x.data() + 9 * nquad, sizeof(double) * 9);
+  }
+}
+
 }  // namespace
 
 namespace qol {
@@ -59,11 +106,7 @@ int qol_geo_build(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
     Surface s = make_surface(nfp, nmode, mnrz);
     Geometry g;
     build_geometry(s, mp, np, p, iterate_aspect != 0, g);
-    std::memcpy(sx, g.sx.data(), sizeof(double) * g.sx.size());
-    std::memcpy(snx, g.snx.data(), sizeof(double) * g.snx.size());
-    std::memcpy(sw, g.sw.data(), sizeof(double) * g.sw.size());
-    std::memcpy(rts, g.rts.data(), sizeof(double) * g.rts.size());
-    std::memcpy(rps, g.rps.data(), sizeof(double) * g.rps.size());
+    copy_geometry(g, sx, snx, sw, rts, rps);
     if (charts)
       for (size_t k = 0; k < g.charts.size(); ++k) {
         const Chart& c = g.charts[k];
@@ -84,33 +127,67 @@ int qol_geo_chart_boundary(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64
                            int64_t iterate_aspect, int64_t nquad, double* sxbd, double* rv) {
   try {
     Surface s = make_surface(nfp, nmode, mnrz);
-    Geometry g;
-    g.p = p; g.mp = mp; g.np = np;
-    g.charts = build_charts(s, mp, np, p, iterate_aspect != 0);
-    g.ntri = 2 * (int64_t)g.charts.size();
-    std::vector<double> t(nquad), w(nquad);
-    gl_nodes(nquad, t.data(), w.data(), nullptr);
-    std::vector<double> uv(2 * (3 * nquad + 3));
-    const double pi = 3.14159265358979323846, th1 = 2.0 * pi / 3.0, th2 = 4.0 * pi / 3.0, ti1 = 1.0 / th1;
-    for (int64_t k = 0; k < 3; ++k)
-      for (int64_t i = 0; i < nquad; ++i) {
-        const double t0 = (double)k * 2.0 * pi / 3.0, t1 = (double)(k + 1) * 2.0 * pi / 3.0;
-        const double tt = t0 + 0.5 * (1.0 + t[i]) * (t1 - t0);
-        double u = 0.0, v = 0.0;
-        if (tt < th1) v = 1.0 - ti1 * tt;
-        else if (tt < th2) u = ti1 * (tt - th1);
-        else { u = 1.0 - ti1 * (tt - th2); v = ti1 * (tt - th2); }
-        uv[2 * (k * nquad + i)] = u;
-        uv[2 * (k * nquad + i) + 1] = v;
+    chart_boundary(s, mp, np, p, build_charts(s, mp, np, p, iterate_aspect != 0), nquad, sxbd, rv);
+    return 0;
+  } catch (const std::exception& e) {
+    qol::set_error(e.what());
+    return 1;
+  }
+}
+
+int64_t qol_geo_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                       double restol, int64_t cap, double* charts) {
+  try {
+    Surface s = make_surface(nfp, nmode, mnrz);
+    bool over = false;
+    const std::vector<Chart> L = build_charts_refined(s, mp, np, p, restol, cap, &over);
+    if (over) return -1;
+    if (charts)
+      for (size_t k = 0; k < L.size(); ++k) {
+        const Chart& c = L[k];
+        const double v[6] = {c.a1, c.b1, c.a2, c.b2, c.jj, c.ii};
+        std::memcpy(charts + 6 * k, v, sizeof(v));
       }
-    const double vert[6] = {0, 0, 1, 0, 0, 1};
-    for (int i = 0; i < 6; ++i) uv[2 * 3 * nquad + i] = vert[i];
-    std::vector<double> x(3 * (3 * nquad + 3));
-    for (int64_t it = 0; it < g.ntri; ++it) {
-      chart_uv2x(s, g, it, 3 * nquad + 3, uv.data(), x.data());
-      std::memcpy(sxbd + 9 * nquad * it, x.data(), sizeof(double) * 9 * nquad);
-      std::memcpy(rv + 9 * it, x.data() + 9 * nquad, sizeof(double) * 9);
-    }
+    return (int64_t)L.size();
+  } catch (const std::exception& e) {
+    qol::set_error(e.what());
+    return -2;
+  }
+}
+
+double qol_geo_curvature_tail(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                              const double* chart) {
+  try {
+    Surface s = make_surface(nfp, nmode, mnrz);
+    const Chart c{chart[0], chart[1], chart[2], chart[3], chart[4], chart[5]};
+    return chart_curvature_tail(s, c, mp, np, p);
+  } catch (const std::exception& e) {
+    qol::set_error(e.what());
+    return -1.0;
+  }
+}
+
+int qol_geo_build_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                         int64_t nchart, const double* charts, double* sx, double* snx, double* sw, double* rts,
+                         double* rps) {
+  try {
+    Surface s = make_surface(nfp, nmode, mnrz);
+    Geometry g;
+    build_geometry(s, mp, np, p, unpack_charts(nchart, charts), g);
+    copy_geometry(g, sx, snx, sw, rts, rps);
+    return 0;
+  } catch (const std::exception& e) {
+    qol::set_error(e.what());
+    return 1;
+  }
+}
+
+int qol_geo_chart_boundary_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                                  const double* mnrz, int64_t nchart, const double* charts, int64_t nquad,
+                                  double* sxbd, double* rv) {
+  try {
+    Surface s = make_surface(nfp, nmode, mnrz);
+    chart_boundary(s, mp, np, p, unpack_charts(nchart, charts), nquad, sxbd, rv);
     return 0;
   } catch (const std::exception& e) {
     qol::set_error(e.what());

@@ -11,39 +11,88 @@ from ._lib import check, dbl, i64, lib, ptr
 VMEC_BENCH = np.array([[0, 0, 1.0, 0.0], [1, 0, 0.3, 0.3], [1, 1, 0.06, -0.06]], dtype=np.float64)
 
 
-def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None, gpu=False):
+def _modes(modes):
+    nmode = 0 if modes is None else len(modes)
+    mn = np.zeros(4) if modes is None else np.ascontiguousarray(modes, dtype=np.float64)
+    return nmode, mn
+
+
+def charts(mp, np_, p, modes=None, nfp=5, restol=0.0, cap=0):
+    """The adaptive chart list (qol_geo_charts, stellarator_charts): (n, 6) rows
+    a1 b1 a2 b2 jj ii.  restol <= 0: aspect splits only."""
+    L = lib()
+    nmode, mn = _modes(modes)
+    n = L.qol_geo_charts(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), dbl(restol), i64(cap), None)
+    if n == -1:
+        raise RuntimeError("qol_geo_charts: the chart list would exceed cap; loosen restol")
+    if n < 0:
+        raise RuntimeError("qol_geo_charts: %s" % L.qol_last_error().decode())
+    cl = np.zeros((n, 6))
+    m = L.qol_geo_charts(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), dbl(restol), i64(cap), ptr(cl))
+    assert m == n
+    return cl
+
+
+def curvature_tail(mp, np_, p, chart, modes=None, nfp=5):
+    """panel_resolution of one chart (qol_geo_curvature_tail)."""
+    nmode, mn = _modes(modes)
+    c = np.ascontiguousarray(chart, dtype=np.float64)
+    v = lib().qol_geo_curvature_tail(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), ptr(c))
+    if v < 0:
+        raise RuntimeError("qol_geo_curvature_tail failed")
+    return v
+
+
+def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None, gpu=False, restol=None, cap=0):
     """Triangulated surface: dict(sx, snx, sw, rts, rps (nsrc,3|nsrc), ntri, p, h).
 
     modes=None -> the reference's built-in 7-term surface (single aspect pass);
     modes=(nmode,4) array of (m, n, Rc, Zs) -> VMEC form with nfp periods.
+    restol -> the adaptive chart list (aspect, curvature refinement for restol > 0,
+    2:1 balance) of qol_geo_charts, built by qol_geo_build_charts[_gpu].
     gpu=True runs the per-chart work on the GPU (qol_geo_build_gpu).
     """
     L = lib()
-    nmode = 0 if modes is None else len(modes)
-    mn = np.zeros(4) if modes is None else np.ascontiguousarray(modes, dtype=np.float64)
+    nmode, mn = _modes(modes)
+    h = p * (p + 1) // 2
+    if restol is not None:
+        cl = charts(mp, np_, p, modes, nfp, restol, cap)
+        ntri = 2 * len(cl)
+        nsrc = ntri * h
+        sx = np.zeros((nsrc, 3)); snx = np.zeros((nsrc, 3)); sw = np.zeros(nsrc)
+        rts = np.zeros((nsrc, 3)); rps = np.zeros((nsrc, 3))
+        build = L.qol_geo_build_charts_gpu if gpu else L.qol_geo_build_charts
+        check(build(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(len(cl)), ptr(cl), ptr(sx),
+                    ptr(snx), ptr(sw), ptr(rts), ptr(rps)),
+              "qol_geo_build_charts_gpu" if gpu else "qol_geo_build_charts")
+        return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=cl,
+                    modes=modes, nfp=nfp, iterate_aspect=1, restol=restol)
     it = (0 if modes is None else 1) if iterate_aspect is None else int(iterate_aspect)
     ntri = L.qol_geo_count(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it))
     if ntri <= 0:
         raise RuntimeError("geometry chart build failed")
-    h = p * (p + 1) // 2
     nsrc = ntri * h
     sx = np.zeros((nsrc, 3)); snx = np.zeros((nsrc, 3)); sw = np.zeros(nsrc)
-    rts = np.zeros((nsrc, 3)); rps = np.zeros((nsrc, 3)); charts = np.zeros((ntri // 2, 6))
+    rts = np.zeros((nsrc, 3)); rps = np.zeros((nsrc, 3)); cl = np.zeros((ntri // 2, 6))
     build = L.qol_geo_build_gpu if gpu else L.qol_geo_build
     check(build(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it), ptr(sx), ptr(snx),
-                ptr(sw), ptr(rts), ptr(rps), ptr(charts)), "qol_geo_build_gpu" if gpu else "qol_geo_build")
-    return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=charts,
-                modes=modes, nfp=nfp, iterate_aspect=it)
+                ptr(sw), ptr(rts), ptr(rps), ptr(cl)), "qol_geo_build_gpu" if gpu else "qol_geo_build")
+    return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=cl,
+                modes=modes, nfp=nfp, iterate_aspect=it, restol=None)
 
 
 def chart_boundary(geo):
     """exact chart boundary at the edge Gauss-Legendre nodes (ichart=1 inputs)."""
     L = lib()
     p = geo["p"]; nq = p + 2; ntri = geo["ntri"]
-    modes = geo["modes"]
-    nmode = 0 if modes is None else len(modes)
-    mn = np.zeros(4) if modes is None else np.ascontiguousarray(modes, dtype=np.float64)
+    nmode, mn = _modes(geo["modes"])
     sxbd = np.zeros((ntri, 3 * nq, 3)); rv = np.zeros((ntri, 3, 3))
+    if geo.get("restol") is not None:
+        cl = np.ascontiguousarray(geo["charts"])
+        check(L.qol_geo_chart_boundary_charts(i64(geo["mp"]), i64(geo["np"]), i64(p), i64(geo["nfp"]), i64(nmode),
+                                              ptr(mn), i64(len(cl)), ptr(cl), i64(nq), ptr(sxbd), ptr(rv)),
+              "qol_geo_chart_boundary_charts")
+        return sxbd, rv
     check(L.qol_geo_chart_boundary(i64(geo["mp"]), i64(geo["np"]), i64(p), i64(geo["nfp"]), i64(nmode), ptr(mn),
                                    i64(geo["iterate_aspect"]), i64(nq), ptr(sxbd), ptr(rv)), "qol_geo_chart_boundary")
     return sxbd, rv

@@ -34,3 +34,20 @@ def test_geometry_gpu_vs_host(mp, np_, p, vmec):
     # triangle counts of the benchmark surfaces (aspect splits of utils/stellarator_geo_mex.c:427-479)
     if vmec:
         assert host["ntri"] == {(16, 40): 1580, (48, 120): 14250}[(mp, np_)]
+
+
+def test_geometry_gpu_vs_host_w7x_refined():
+    """W7-X (288 modes, nfp 5), 12x36 charts at order 6 with curvature refinement
+    (restol 0.1): the refined chart list built on the GPU against the host."""
+    print("geometry W7-X 12x36 p=6 restol 0.1", flush=True)
+    import os
+    modes = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "w7x_modes.txt"))
+    host = W.geometry(12, 36, 6, modes=modes, restol=0.1)
+    gpu = W.geometry(12, 36, 6, modes=modes, restol=0.1, gpu=True)
+    assert host["ntri"] == gpu["ntri"] == 5500
+    assert np.array_equal(gpu["charts"], host["charts"])
+    assert np.abs(gpu["sx"] - host["sx"]).max() < 1e-14 * np.abs(host["sx"]).max()
+    for key in ("rts", "rps"):
+        assert np.abs(gpu[key] - host[key]).max() < 1e-12 * np.abs(host[key]).max(), key
+    assert np.abs(gpu["snx"] - host["snx"]).max() < 1e-12
+    assert np.abs(gpu["sw"] / host["sw"] - 1.0).max() < 1e-12
