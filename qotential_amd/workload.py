@@ -56,17 +56,8 @@ def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None, gpu=False, rest
     nmode, mn = _modes(modes)
     h = p * (p + 1) // 2
     if restol is not None:
-        cl = charts(mp, np_, p, modes, nfp, restol, cap)
-        ntri = 2 * len(cl)
-        nsrc = ntri * h
-        sx = np.zeros((nsrc, 3)); snx = np.zeros((nsrc, 3)); sw = np.zeros(nsrc)
-        rts = np.zeros((nsrc, 3)); rps = np.zeros((nsrc, 3))
-        build = L.qol_geo_build_charts_gpu if gpu else L.qol_geo_build_charts
-        check(build(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(len(cl)), ptr(cl), ptr(sx),
-                    ptr(snx), ptr(sw), ptr(rts), ptr(rps)),
-              "qol_geo_build_charts_gpu" if gpu else "qol_geo_build_charts")
-        return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=cl,
-                    modes=modes, nfp=nfp, iterate_aspect=1, restol=restol)
+        return geometry_of_charts(mp, np_, p, charts(mp, np_, p, modes, nfp, restol, cap), modes, nfp, gpu,
+                                  restol=restol)
     it = (0 if modes is None else 1) if iterate_aspect is None else int(iterate_aspect)
     ntri = L.qol_geo_count(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it))
     if ntri <= 0:
@@ -79,6 +70,25 @@ def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None, gpu=False, rest
                 ptr(sw), ptr(rts), ptr(rps), ptr(cl)), "qol_geo_build_gpu" if gpu else "qol_geo_build")
     return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=cl,
                 modes=modes, nfp=nfp, iterate_aspect=it, restol=None)
+
+
+def geometry_of_charts(mp, np_, p, cl, modes=None, nfp=5, gpu=False, restol=0.0):
+    """Triangulated surface of a given chart list (rows a1 b1 a2 b2 jj ii) at
+    patch order p (qol_geo_build_charts[_gpu], stellarator_geo_charts)."""
+    L = lib()
+    nmode, mn = _modes(modes)
+    cl = np.ascontiguousarray(cl, dtype=np.float64).reshape(-1, 6)
+    h = p * (p + 1) // 2
+    ntri = 2 * len(cl)
+    nsrc = ntri * h
+    sx = np.zeros((nsrc, 3)); snx = np.zeros((nsrc, 3)); sw = np.zeros(nsrc)
+    rts = np.zeros((nsrc, 3)); rps = np.zeros((nsrc, 3))
+    build = L.qol_geo_build_charts_gpu if gpu else L.qol_geo_build_charts
+    check(build(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(len(cl)), ptr(cl), ptr(sx),
+                ptr(snx), ptr(sw), ptr(rts), ptr(rps)),
+          "qol_geo_build_charts_gpu" if gpu else "qol_geo_build_charts")
+    return dict(sx=sx, snx=snx, sw=sw, rts=rts, rps=rps, ntri=ntri, p=p, h=h, mp=mp, np=np_, charts=cl,
+                modes=modes, nfp=nfp, iterate_aspect=1, restol=restol)
 
 
 def chart_boundary(geo):
