@@ -57,6 +57,14 @@ int qol_geo_chart_boundary(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64
  * error.  A triangle list has 2 n triangles. */
 int64_t qol_geo_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
                        const double* mnrz, double restol, int64_t cap, double* charts);
+/* qol_geo_charts with each curvature pass's tails evaluated on the GPU (one wave
+ * per chart); the same list up to the rounding of the device sin/cos near restol */
+int64_t qol_geo_charts_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                           const double* mnrz, double restol, int64_t cap, double* charts);
+/* panel_resolution (utils/stellarator_geo_mex.c:359-411) of nchart charts on the
+ * GPU: tails(nchart) */
+int qol_geo_curvature_tails_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                                const double* mnrz, int64_t nchart, const double* charts, double* tails);
 /* panel_resolution of one chart (6 values as above); -1 on error */
 double qol_geo_curvature_tail(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
                               const double* mnrz, const double* chart);

@@ -29,6 +29,7 @@ def lib():
         _LIB.qol_last_error.restype = C.c_char_p
         _LIB.qol_geo_count.restype = i64
         _LIB.qol_geo_charts.restype = i64
+        _LIB.qol_geo_charts_gpu.restype = i64
         _LIB.qol_geo_curvature_tail.restype = C.c_double
         _LIB.qol_last_status.restype = i64
     return _LIB

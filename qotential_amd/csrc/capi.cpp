@@ -201,6 +201,54 @@ int qol_geo_build_charts_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int
   });
 }
 
+int64_t qol_geo_charts_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                           double restol, int64_t cap, double* charts) {
+  int64_t n = -2;
+  const int st = guarded([&] {
+    Surface s;
+    s.nfp = nfp;
+    for (int64_t k = 0; k < nmode; ++k) {
+      s.m.push_back(mnrz[4 * k]);
+      s.n.push_back(mnrz[4 * k + 1]);
+      s.rc.push_back(mnrz[4 * k + 2]);
+      s.zs.push_back(mnrz[4 * k + 3]);
+    }
+    bool over = false;
+    const std::vector<Chart> L = build_charts_refined(s, mp, np, p, restol, cap, &over, true);
+    if (over) { n = -1; return; }
+    if (charts)
+      for (size_t k = 0; k < L.size(); ++k) {
+        const Chart& c = L[k];
+        const double v[6] = {c.a1, c.b1, c.a2, c.b2, c.jj, c.ii};
+        for (int j = 0; j < 6; ++j) charts[6 * k + j] = v[j];
+      }
+    n = (int64_t)L.size();
+  });
+  return st == 0 ? n : -2;
+}
+
+int qol_geo_curvature_tails_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                                int64_t nchart, const double* charts, double* tails) {
+  return guarded([&] {
+    if (nchart < 0 || (nchart > 0 && (!charts || !tails)))
+      throw std::runtime_error("qol_geo_curvature_tails_gpu: bad chart list");
+    Surface s;
+    s.nfp = nfp;
+    for (int64_t k = 0; k < nmode; ++k) {
+      s.m.push_back(mnrz[4 * k]);
+      s.n.push_back(mnrz[4 * k + 1]);
+      s.rc.push_back(mnrz[4 * k + 2]);
+      s.zs.push_back(mnrz[4 * k + 3]);
+    }
+    std::vector<Chart> L(nchart);
+    for (int64_t k = 0; k < nchart; ++k)
+      L[k] = Chart{charts[6 * k], charts[6 * k + 1], charts[6 * k + 2], charts[6 * k + 3], charts[6 * k + 4],
+                   charts[6 * k + 5]};
+    const std::vector<double> t = curvature_tails_gpu(s, L, mp, np, p);
+    for (int64_t k = 0; k < nchart; ++k) tails[k] = t[k];
+  });
+}
+
 int qol_hbm_copy(int64_t nbytes, double* ms) {
   return guarded([&] {
     if (nbytes <= 0 || nbytes % 16) throw std::runtime_error("qol_hbm_copy: nbytes must be a positive multiple of 16");

@@ -263,8 +263,15 @@ double chart_curvature_tail(const Surface& s, const Chart& c, i64 mp, i64 np, i6
   return curvature_tail(s, c, CurvaturePlan(mp, np, p));
 }
 
+void curvature_tables(i64 p, std::vector<double>& x0, std::vector<double>& D, std::vector<double>& Vinv) {
+  const CurvaturePlan P(1, 1, p);
+  x0 = P.x0;
+  D = P.D;
+  Vinv = P.Vinv;
+}
+
 std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p, double restol, i64 cap,
-                                        bool* over) {
+                                        bool* over, bool gpu) {
   if (cap <= 0) cap = 200000;
   *over = false;
   std::vector<Chart> L = build_charts(s, mp, np, p, true);
@@ -277,9 +284,23 @@ std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p,
   for (int level = 0; level < 12; ++level) {
     const i64 n0 = (i64)L.size();
     i64 nsplit = 0;
+    // this pass's tails: a chart is tested before anything of this pass
+    // replaces it, so all of them can be evaluated up front
+    std::vector<double> tail(n0, 0.0);
+    if (gpu) {
+      std::vector<Chart> todo;
+      std::vector<i64> at;
+      for (i64 k = 0; k < n0; ++k)
+        if (!ok[k]) { todo.push_back(L[k]); at.push_back(k); }
+      const std::vector<double> t = curvature_tails_gpu(s, todo, mp, np, p);
+      for (size_t j = 0; j < at.size(); ++j) tail[at[j]] = t[j];
+    } else {
+      for (i64 k = 0; k < n0; ++k)
+        if (!ok[k]) tail[k] = curvature_tail(s, L[k], P);
+    }
     for (i64 k = 0; k < n0; ++k) {
       if (ok[k]) continue;
-      if (curvature_tail(s, L[k], P) <= restol) { ok[k] = 1; continue; }
+      if (tail[k] <= restol) { ok[k] = 1; continue; }
       if ((i64)L.size() + 3 > cap) { *over = true; return {}; }
       const Chart c = L[k];
       L[k] = quarter(c, 0);
@@ -291,19 +312,40 @@ std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p,
   // 2:1 balance: split a chart whose extent along a face it shares with a
   // neighbour (one base cell apart at most, periodic in both angles) exceeds
   // twice the neighbour's; sweeps until none is split (at most 20)
+  // Candidates are the charts of the base cells within one cell of k's, the
+  // neighbourhood of the reference's index filter (|djj| <= 1 or wrapped, the
+  // same in ii); bucketing the charts by base cell finds them without the
+  // all-pairs scan, and a chart is split when any candidate asks for it, so the
+  // visiting order does not change the result.
   const double et1 = 1e-9 * P.ts1, et2 = 1e-9 * P.ts2;
   for (int sweep = 0; sweep < 20; ++sweep) {
     const i64 n0 = (i64)L.size();
     std::vector<double> box(4 * n0);
     for (i64 k = 0; k < n0; ++k) chart_box(L[k], P.ts1, P.ts2, &box[4 * k]);
+    // charts by base cell (jj - 1) + np (ii - 1), as a counting sort
+    std::vector<i64> cstart(mp * np + 1, 0), cidx(n0);
+    auto cell = [&](const Chart& c) { return ((i64)c.jj - 1) + np * ((i64)c.ii - 1); };
+    for (i64 k = 0; k < n0; ++k) ++cstart[cell(L[k]) + 1];
+    for (i64 c = 0; c < mp * np; ++c) cstart[c + 1] += cstart[c];
+    {
+      std::vector<i64> fill(cstart.begin(), cstart.end() - 1);
+      for (i64 k = 0; k < n0; ++k) cidx[fill[cell(L[k])]++] = k;
+    }
     std::vector<char> split(n0, 0);
     for (i64 k = 0; k < n0; ++k) {
       const double* bk = &box[4 * k];
-      for (i64 q = 0; q < n0 && !split[k]; ++q) {
+      i64 cj[3], ci[3];
+      int nj = 0, ni = 0;
+      for (i64 d = -1; d <= 1; ++d) {
+        const i64 j = (((i64)L[k].jj - 1 + d) % np + np) % np, i = (((i64)L[k].ii - 1 + d) % mp + mp) % mp;
+        if (std::find(cj, cj + nj, j) == cj + nj) cj[nj++] = j;
+        if (std::find(ci, ci + ni, i) == ci + ni) ci[ni++] = i;
+      }
+      for (int a = 0; a < nj && !split[k]; ++a)
+        for (int b = 0; b < ni && !split[k]; ++b)
+          for (i64 e = cstart[cj[a] + np * ci[b]]; e < cstart[cj[a] + np * ci[b] + 1] && !split[k]; ++e) {
+        const i64 q = cidx[e];
         if (q == k) continue;
-        const double dj = std::fabs(L[q].jj - L[k].jj), di = std::fabs(L[q].ii - L[k].ii);
-        if (dj > 1.5 && std::fabs(dj - (double)np) > 1.5) continue;
-        if (di > 1.5 && std::fabs(di - (double)mp) > 1.5) continue;
         const double* bq = &box[4 * q];
         if ((std::fabs(bk[1] - bq[0]) < et1 || std::fabs(bk[0] - bq[1]) < et1) &&
             std::fmin(bk[3], bq[3]) - std::fmax(bk[2], bq[2]) > et2) {
@@ -313,7 +355,7 @@ std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p,
                    std::fmin(bk[1], bq[1]) - std::fmax(bk[0], bq[0]) > et1) {
           if (bk[1] - bk[0] > 2.0 * (bq[1] - bq[0]) + et1) split[k] = 1;
         }
-      }
+          }
     }
     i64 nsplit = 0;
     for (i64 k = 0; k < n0; ++k) {

@@ -114,7 +114,141 @@ __global__ void __launch_bounds__(64) chart_geometry_kernel(GeoDev G, int nch, c
   }
 }
 
+// The mean curvature's Legendre tail of one chart per workgroup (one wave), in
+// the host's operations and order (chart_curvature_tail): the chart map on the
+// p x p grid, first and second spectral derivatives, H at every node from the
+// fundamental forms, C = Vinv H Vinv^T, and the share of sum C^2 in degrees p-2
+// and p-1 of either variable, summed by one lane in the host's order.
+__global__ void __launch_bounds__(64) chart_curvature_kernel(GeoDev G, int nch, const double* __restrict__ charts,
+                                                             const double* __restrict__ Vinv,
+                                                             double* __restrict__ tail_out) {
+  extern __shared__ double sm[];
+  const int k = blockIdx.x;
+  if (k >= nch) return;
+  const int p = G.p, p2 = p * p;
+  double* X = sm;           // [p2][3], node m = a + b p at (t1, t2) = (x0[b], x0[a])
+  double* X1 = X + 3 * p2;  // d/dt1 (over b)
+  double* X2 = X1 + 3 * p2;  // d/dt2 (over a)
+  double* X11 = X2 + 3 * p2;
+  double* X12 = X11 + 3 * p2;
+  double* X22 = X12 + 3 * p2;
+  double* H = X22 + 3 * p2;  // [p2]
+  double* T = H + p2;        // [p2]
+  double* C = T + p2;        // [p2]
+  const double* c = charts + 6 * (size_t)k;
+  for (int m = threadIdx.x; m < p2; m += blockDim.x) {
+    const int a = m % p, b = m / p;
+    const double u = ((c[0] * G.x0[b] + c[1]) + 2.0 * c[4]) * G.ts1;
+    const double v = ((c[2] * G.x0[a] + c[3]) + 2.0 * c[5]) * G.ts2;
+    surface_dev(G, u, v, X + 3 * m);
+  }
+  __syncthreads();
+  const double* D = G.D;
+  for (int e = threadIdx.x; e < 3 * p2; e += blockDim.x) {
+    const int m = e / 3, d = e % 3, a = m % p, b = m / p;
+    double s1 = 0.0, s2 = 0.0;
+    for (int q = 0; q < p; ++q) s1 += D[b + q * p] * X[3 * (a + q * p) + d];
+    for (int q = 0; q < p; ++q) s2 += D[a + q * p] * X[3 * (q + b * p) + d];
+    X1[e] = s1;
+    X2[e] = s2;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 3 * p2; e += blockDim.x) {
+    const int m = e / 3, d = e % 3, a = m % p, b = m / p;
+    double s11 = 0.0, s12 = 0.0, s22 = 0.0;
+    for (int q = 0; q < p; ++q) s11 += D[b + q * p] * X1[3 * (a + q * p) + d];
+    for (int q = 0; q < p; ++q) s12 += D[b + q * p] * X2[3 * (a + q * p) + d];
+    for (int q = 0; q < p; ++q) s22 += D[a + q * p] * X2[3 * (q + b * p) + d];
+    X11[e] = s11;
+    X12[e] = s12;
+    X22[e] = s22;
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < p2; m += blockDim.x) {
+    const double* u = X1 + 3 * m;
+    const double* v = X2 + 3 * m;
+    double nv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    double nn = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    if (nn <= 0.0) nn = 1.0;
+    nv[0] /= nn; nv[1] /= nn; nv[2] /= nn;
+    const double E = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double F = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    const double Gg = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double* r11 = X11 + 3 * m;
+    const double* r12 = X12 + 3 * m;
+    const double* r22 = X22 + 3 * m;
+    const double L = nv[0] * r11[0] + nv[1] * r11[1] + nv[2] * r11[2];
+    const double M = nv[0] * r12[0] + nv[1] * r12[1] + nv[2] * r12[2];
+    const double N = nv[0] * r22[0] + nv[1] * r22[1] + nv[2] * r22[2];
+    double den = 2.0 * (E * Gg - F * F);
+    if (den == 0.0) den = 1.0;
+    H[m] = (E * N - 2.0 * F * M + Gg * L) / den;
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < p2; m += blockDim.x) {
+    const int a = m % p, b = m / p;
+    double v = 0.0;
+    for (int q = 0; q < p; ++q) v += Vinv[a + q * p] * H[q + b * p];
+    T[m] = v;
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < p2; m += blockDim.x) {
+    const int a = m % p, b = m / p;
+    double v = 0.0;
+    for (int q = 0; q < p; ++q) v += T[a + q * p] * Vinv[b + q * p];
+    C[m] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tail = 0.0, tot = 0.0;
+    for (int b = 0; b < p; ++b)
+      for (int a = 0; a < p; ++a) {
+        const double v = C[a + b * p];
+        tot += v * v;
+        if (a >= p - 2 || b >= p - 2) tail += v * v;
+      }
+    tail_out[k] = sqrt(tail / (tot > 0.0 ? tot : 1.0));
+  }
+}
+
 }  // namespace
+
+std::vector<double> curvature_tails_gpu(const Surface& s, const std::vector<Chart>& cl, i64 mp, i64 np, i64 p) {
+  const int nch = (int)cl.size();
+  std::vector<double> out(nch, 0.0);
+  if (nch == 0) return out;
+  if (p < 2 || p > QOL_MAXP) throw std::runtime_error("curvature on the GPU: patch order out of range");
+  std::vector<double> x0, D, Vinv;
+  curvature_tables(p, x0, D, Vinv);
+  std::vector<double> ch(6 * (size_t)nch), modes(4 * s.rc.size());
+  for (int k = 0; k < nch; ++k) {
+    const double v[6] = {cl[k].a1, cl[k].b1, cl[k].a2, cl[k].b2, cl[k].jj, cl[k].ii};
+    for (int j = 0; j < 6; ++j) ch[6 * (size_t)k + j] = v[j];
+  }
+  for (size_t k = 0; k < s.rc.size(); ++k) {
+    modes[4 * k] = s.m[k]; modes[4 * k + 1] = s.n[k]; modes[4 * k + 2] = s.rc[k]; modes[4 * k + 3] = s.zs[k];
+  }
+  DevBuf d_modes, d_x0, d_D, d_V, d_ch, d_out;
+  auto up = [](DevBuf& b, const void* src, size_t bytes) {
+    b.alloc(bytes > 0 ? bytes : 8);
+    if (bytes) HIPCHK(hipMemcpy(b.ptr, src, bytes, hipMemcpyHostToDevice));
+  };
+  up(d_modes, modes.data(), sizeof(double) * modes.size());
+  up(d_x0, x0.data(), sizeof(double) * x0.size());
+  up(d_D, D.data(), sizeof(double) * D.size());
+  up(d_V, Vinv.data(), sizeof(double) * Vinv.size());
+  up(d_ch, ch.data(), sizeof(double) * ch.size());
+  d_out.alloc(sizeof(double) * nch);
+  const double pi = 3.14159265358979323846;
+  GeoDev G{(int)p, 0, (int)s.rc.size(), (int)s.nfp, pi / (double)np, pi / (double)mp, (const double*)d_modes.ptr,
+           (const double*)d_x0.ptr, (const double*)d_D.ptr, nullptr, nullptr};
+  const size_t lds = sizeof(double) * 21 * (size_t)p * p;
+  hipLaunchKernelGGL(chart_curvature_kernel, dim3(nch), dim3(64), lds, 0, G, nch, (const double*)d_ch.ptr,
+                     (const double*)d_V.ptr, (double*)d_out.ptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out.data(), d_out.ptr, sizeof(double) * nch, hipMemcpyDeviceToHost));
+  return out;
+}
 
 void build_geometry_gpu(const Surface& s, const GeometryPlan& P, double* sx, double* snx, double* sw, double* rts,
                         double* rps) {

@@ -52,10 +52,19 @@ struct Chart {
 std::vector<Chart> build_charts(const Surface& s, i64 mp, i64 np, i64 p, bool iterate_aspect);
 // iterated aspect splits, then, for restol > 0, curvature refinement and 2:1
 // balance (utils/stellarator_geo_mex.c:427-546); empty when the list would pass cap
-std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p, double restol, i64 cap, bool* over);
+// gpu: each curvature pass evaluates its charts' tails in one GPU launch (the
+// tests of one pass depend only on the chart tested, so the result is the same)
+std::vector<Chart> build_charts_refined(const Surface& s, i64 mp, i64 np, i64 p, double restol, i64 cap, bool* over,
+                                        bool gpu = false);
 // the relative weight of the last two degrees in the tensor-Legendre spectrum of
 // the chart's mean curvature on its p x p grid (utils/stellarator_geo_mex.c:359-411)
 double chart_curvature_tail(const Surface& s, const Chart& c, i64 mp, i64 np, i64 p);
+// the tables it uses: Gauss-Legendre nodes x0 (p), differentiation matrix D (p x p)
+// and the inverse of the Legendre-Vandermonde matrix (p x p), column-major
+void curvature_tables(i64 p, std::vector<double>& x0, std::vector<double>& D, std::vector<double>& Vinv);
+// the same tail for every chart of cl, one GPU wave per chart (device 0); equal
+// to chart_curvature_tail up to the rounding of the device sin/cos
+std::vector<double> curvature_tails_gpu(const Surface& s, const std::vector<Chart>& cl, i64 mp, i64 np, i64 p);
 
 struct Geometry {
   i64 p = 0, h = 0, ntri = 0, nsrc = 0, mp = 0, np = 0;

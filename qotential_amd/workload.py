@@ -17,20 +17,32 @@ def _modes(modes):
     return nmode, mn
 
 
-def charts(mp, np_, p, modes=None, nfp=5, restol=0.0, cap=0):
+def charts(mp, np_, p, modes=None, nfp=5, restol=0.0, cap=0, gpu=False):
     """The adaptive chart list (qol_geo_charts, stellarator_charts): (n, 6) rows
-    a1 b1 a2 b2 jj ii.  restol <= 0: aspect splits only."""
+    a1 b1 a2 b2 jj ii.  restol <= 0: aspect splits only.  gpu=True evaluates the
+    curvature tails on the GPU (qol_geo_charts_gpu)."""
     L = lib()
     nmode, mn = _modes(modes)
-    n = L.qol_geo_charts(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), dbl(restol), i64(cap), None)
+    fn = L.qol_geo_charts_gpu if gpu else L.qol_geo_charts
+    n = fn(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), dbl(restol), i64(cap), None)
     if n == -1:
         raise RuntimeError("qol_geo_charts: the chart list would exceed cap; loosen restol")
     if n < 0:
         raise RuntimeError("qol_geo_charts: %s" % L.qol_last_error().decode())
     cl = np.zeros((n, 6))
-    m = L.qol_geo_charts(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), dbl(restol), i64(cap), ptr(cl))
+    m = fn(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), dbl(restol), i64(cap), ptr(cl))
     assert m == n
     return cl
+
+
+def curvature_tails_gpu(mp, np_, p, cl, modes=None, nfp=5):
+    """panel_resolution of every chart of cl on the GPU (qol_geo_curvature_tails_gpu)."""
+    nmode, mn = _modes(modes)
+    cl = np.ascontiguousarray(cl, dtype=np.float64).reshape(-1, 6)
+    out = np.zeros(len(cl))
+    check(lib().qol_geo_curvature_tails_gpu(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(len(cl)),
+                                            ptr(cl), ptr(out)), "qol_geo_curvature_tails_gpu")
+    return out
 
 
 def curvature_tail(mp, np_, p, chart, modes=None, nfp=5):
@@ -56,7 +68,7 @@ def geometry(mp, np_, p, modes=None, nfp=5, iterate_aspect=None, gpu=False, rest
     nmode, mn = _modes(modes)
     h = p * (p + 1) // 2
     if restol is not None:
-        return geometry_of_charts(mp, np_, p, charts(mp, np_, p, modes, nfp, restol, cap), modes, nfp, gpu,
+        return geometry_of_charts(mp, np_, p, charts(mp, np_, p, modes, nfp, restol, cap, gpu=gpu), modes, nfp, gpu,
                                   restol=restol)
     it = (0 if modes is None else 1) if iterate_aspect is None else int(iterate_aspect)
     ntri = L.qol_geo_count(i64(mp), i64(np_), i64(p), i64(nfp), i64(nmode), ptr(mn), i64(it))
