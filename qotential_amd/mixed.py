@@ -57,24 +57,34 @@ def targets(surf, nt, umin, umax, sides=0, seed=1):
 
 def layer_potential(surf, tx, density, exterior=True, side_tol=1e-10):
     """S[du/dn] - D[u] at targets tx for the density u given by
-    density(x) -> (u, grad u) at the surface nodes.  Returns u and per-order
-    statistics {p: dict(pairs, labels)}."""
-    u = np.zeros(tx.shape[0])
-    stats = {}
-    for p in sorted(surf["parts"]):
-        g = surf["parts"][p]
-        ub, gu = density(g["sx"])
-        ubn = (g["snx"] * gu).sum(1)
-        eng = Engine(g)
-        try:
-            eng.set_targets(tx)
-            nnz = eng.nearlist()
-            eng.eval(exterior=exterior, side_tol=side_tol)
-            far = eng.far(ub, ubn)
-            up, _ = eng.apply(ub, ubn, far)
-            lab = eng.weights()["label"]
-        finally:
-            eng.close()
-        u += up
-        stats[p] = dict(pairs=int(nnz), labels=np.bincount(lab, minlength=6)[1:].tolist())
+    density(x) -> (u, grad u) at the surface nodes.  Every order's evaluation is
+    launched before any is waited for (each engine works on its own streams, so
+    the orders overlap on the device); far fields and corrections are then added
+    in ascending order.  Returns u and per-order statistics {p: dict(pairs, labels)}."""
+    orders = sorted(surf["parts"])
+    engines, dens = {}, {}
+    try:
+        for p in orders:
+            g = surf["parts"][p]
+            ub, gu = density(g["sx"])
+            dens[p] = (ub, (g["snx"] * gu).sum(1))
+            e = Engine(g)
+            engines[p] = e
+            e.set_targets(tx)
+            e.nearlist()
+        for p in orders:
+            engines[p].eval(exterior=exterior, side_tol=side_tol)
+        u = np.zeros(tx.shape[0])
+        stats = {}
+        for p in orders:
+            e = engines[p]
+            ub, ubn = dens[p]
+            far = e.far(ub, ubn)
+            up, val = e.apply(ub, ubn, far)
+            u += up
+            lab = e.weights()["label"]
+            stats[p] = dict(pairs=int(val.size), labels=np.bincount(lab, minlength=6)[1:].tolist())
+    finally:
+        for e in engines.values():
+            e.close()
     return u, stats
