@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) level_geom_kernel(LevelArgs A, int npan, 
   }
 }
 
-// Level geometry on the FP64 matrix cores (orders whose h fits KS blocks of 4):
+// Level geometry on the FP64 matrix cores (h up to 80, orders up to 12, in KS = ceil(h/4) blocks of 4):
 // per workgroup kLP panels, per wave tiles of 16 level nodes,
 //   D(node, 9*pp + c) = sum_q IM(node, q) S_pp(q, c)   (c: x, y, z, x_u, x_v)
 // as chained v_mfma_f64_16x16x4 over blocks of 4 panel nodes.  The matrix core
@@ -280,7 +280,7 @@ void launch_level_geom(const LevelArgs& A, int npan, const double* sx, const dou
                        double* lgeo, double* balls, hipStream_t s) {
   const int ks = (A.h + 3) / 4;
   const dim3 grid(level_geom_blocks(npan)), blk(256);
-  if (ks <= 9 && 4 * ks <= A.pad) {
+  if (ks <= 20 && 4 * ks <= A.pad) {
     const size_t lds = level_geom_mfma_smem();
 #define QOL_LEVEL_MFMA(KS_)                                                                                    \
   case KS_:                                                                                                    \
@@ -288,7 +288,9 @@ void launch_level_geom(const LevelArgs& A, int npan, const double* sx, const dou
     return;
     switch (ks) {
       QOL_LEVEL_MFMA(1) QOL_LEVEL_MFMA(2) QOL_LEVEL_MFMA(3) QOL_LEVEL_MFMA(4) QOL_LEVEL_MFMA(5)
-      QOL_LEVEL_MFMA(6) QOL_LEVEL_MFMA(7) QOL_LEVEL_MFMA(8) QOL_LEVEL_MFMA(9)
+      QOL_LEVEL_MFMA(6) QOL_LEVEL_MFMA(7) QOL_LEVEL_MFMA(8) QOL_LEVEL_MFMA(9) QOL_LEVEL_MFMA(10)
+      QOL_LEVEL_MFMA(11) QOL_LEVEL_MFMA(12) QOL_LEVEL_MFMA(13) QOL_LEVEL_MFMA(14) QOL_LEVEL_MFMA(15)
+      QOL_LEVEL_MFMA(16) QOL_LEVEL_MFMA(17) QOL_LEVEL_MFMA(18) QOL_LEVEL_MFMA(19) QOL_LEVEL_MFMA(20)
       default: break;
     }
 #undef QOL_LEVEL_MFMA

@@ -32,7 +32,16 @@ def test_high_order_vs_oracle(p, oracle):
     e.eval(exterior=True, side_tol=1e-10)
     w = e.weights()
     offs, idx = e.pairs()
+    lg, bl = e.level_data()
     e.close()
+    h = geo["h"]
+    # level nodes and balls bit for bit (the matrix-core chain is the oracle's fma order)
+    for k in (0, 123, 399):
+        sl = slice(k * h, (k + 1) * h)
+        L = oracle.patch_levels(p, geo["sx"][sl], geo["rts"][sl], geo["rps"][sl])
+        assert np.array_equal(lg[k, 0:3].T, L["sx"]), (p, k)
+        assert np.array_equal(lg[k, 6], L["sw"]), (p, k)
+        assert np.array_equal(bl[k, :, 0:3], L["qpoint"]) and np.array_equal(bl[k, :, 3], L["qradii2"]), (p, k)
     h = geo["h"]
     oracle.set_side_tol(1e-10)
     lab_all = w["label"]
