@@ -21,7 +21,7 @@ def test_builtin_surface_matches_fixture(order, fixture4, fixture6):
 
 
 def test_vmec_chart_counts():
-    # BASELINE surface: 16x40 -> 1580 triangles at p=4, 48x120 -> 14250 at p=8
+    # the benchmark surface: 16x40 -> 1580 triangles at p=4, 48x120 -> 14250 at p=8
     assert W.geometry(16, 40, 4, modes=W.VMEC_BENCH)["ntri"] == 1580
     from qotential_amd._lib import lib, i64, ptr
     mn = np.ascontiguousarray(W.VMEC_BENCH)
