@@ -96,3 +96,22 @@ def test_curvature_tail_circular_torus(p, mp):
         got = W.curvature_tail(mp, 10, p, np.array(chart), modes=modes, nfp=5)
         assert want > 1e-10
         assert abs(got - want) < 0.01 * want, (chart, got, want)
+
+
+def test_chart_boundary_of_a_chart_list(w7x):
+    """qol_geo_chart_boundary_charts on the aspect-only list equals
+    qol_geo_chart_boundary bit for bit; on the refined list every edge node is
+    finite and each triangle's vertices are distinct and within its edges' span."""
+    g_it = W.geometry(12, 36, 6, modes=w7x, nfp=5)
+    g_cl = W.geometry(12, 36, 6, modes=w7x, nfp=5, restol=0.0)
+    assert np.array_equal(g_it["charts"], g_cl["charts"])
+    b_it, v_it = W.chart_boundary(g_it)
+    b_cl, v_cl = W.chart_boundary(g_cl)
+    assert np.array_equal(b_it, b_cl) and np.array_equal(v_it, v_cl)
+    g_rf = W.geometry(12, 36, 6, modes=w7x, nfp=5, restol=0.1)
+    b_rf, v_rf = W.chart_boundary(g_rf)
+    assert b_rf.shape == (5500, 3 * 8, 3) and np.all(np.isfinite(b_rf))
+    # each triangle's three vertices are distinct and lie within its edge nodes' span
+    span = np.ptp(b_rf, axis=1).max(axis=1)
+    d = np.linalg.norm(v_rf[:, 0] - v_rf[:, 1], axis=1)
+    assert np.all(d > 0) and np.all(d < 2 * span)
