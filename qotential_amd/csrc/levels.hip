@@ -484,11 +484,12 @@ __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int*
     const double dn = dx * nx + dy * ny + dz * nz;
     const double s = cw * y;
     const double d = cw * dn * (y * y * y);
+    // past the level's last node the A entries are exactly zero and the B row
+    // (the clamped node's) is finite, so B needs no mask
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
-      const double bv = in ? bq[q] : 0.0;
-      accS[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(s, bv, accS[q], 0, 0, 0);
-      accD[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(d, bv, accD[q], 0, 0, 0);
+      accS[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(s, bq[q], accS[q], 0, 0, 0);
+      accD[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(d, bq[q], accD[q], 0, 0, 0);
     }
   }
   // v_mfma_f64_16x16x4 operands and result: lane l supplies A[l%16][l/16] and
