@@ -97,43 +97,82 @@ void orc_rrq(i64 m, const double* tx, i64 n, const double* sx, const double* snx
 }
 
 /* CSR near list by the ball rule: for each panel k (hdim nodes each),
-   offs(npan+1) 0-based, idx 1-based target indices ascending.  Two passes. */
-i64 orc_nearlist_count(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
-                       const double* tx, i64* offs) {
-  offs[0] = 0;
+   offs(npan+1) 0-based, idx 1-based target indices ascending.  Two passes;
+   panels are independent, so each pass runs over panels in nthreads threads
+   (the result does not depend on the thread count). */
+static void panel_ball(i64 k, i64 hdim, const double* sx, const double* sw, double qp[3], double* qr2) {
+  double s = 0.0;
+  qp[0] = qp[1] = qp[2] = 0.0;
+  for (i64 i = 0; i < hdim; ++i)
+    for (int c = 0; c < 3; ++c) qp[c] += sx[3 * (k * hdim + i) + c];
+  for (i64 i = 0; i < hdim; ++i) s += sw[k * hdim + i];
+  double qr = 1.75 * std::sqrt(s);
+  for (int c = 0; c < 3; ++c) qp[c] /= (double)hdim;
+  *qr2 = qr * qr;
+}
+
+i64 orc_nearlist_count_mt(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
+                          const double* tx, i64* offs, i64 nthreads) {
+  std::vector<i64> cnt(npan, 0);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? (int)nthreads : 1)
   for (i64 k = 0; k < npan; ++k) {
-    double qp[3] = {0, 0, 0}, s = 0.0;
-    for (i64 i = 0; i < hdim; ++i) {
-      for (int c = 0; c < 3; ++c) qp[c] += sx[3 * (k * hdim + i) + c];
-    }
-    for (i64 i = 0; i < hdim; ++i) s += sw[k * hdim + i];
-    double qr = 1.75 * std::sqrt(s);
-    for (int c = 0; c < 3; ++c) qp[c] /= (double)hdim;
-    double qr2 = qr * qr;
-    i64 cnt = 0;
+    double qp[3], qr2;
+    panel_ball(k, hdim, sx, sw, qp, &qr2);
+    i64 c = 0;
     for (i64 j = 0; j < nt; ++j) {
       double d0 = tx[3 * j] - qp[0], d1 = tx[3 * j + 1] - qp[1], d2 = tx[3 * j + 2] - qp[2];
-      if (d0 * d0 + d1 * d1 + d2 * d2 < qr2) ++cnt;
+      if (d0 * d0 + d1 * d1 + d2 * d2 < qr2) ++c;
     }
-    offs[k + 1] = offs[k] + cnt;
+    cnt[k] = c;
   }
+  offs[0] = 0;
+  for (i64 k = 0; k < npan; ++k) offs[k + 1] = offs[k] + cnt[k];
   return offs[npan];
 }
 
-void orc_nearlist_fill(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
-                       const double* tx, const i64* offs, i64* idx) {
+void orc_nearlist_fill_mt(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
+                          const double* tx, const i64* offs, i64* idx, i64 nthreads) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? (int)nthreads : 1)
   for (i64 k = 0; k < npan; ++k) {
-    double qp[3] = {0, 0, 0}, s = 0.0;
-    for (i64 i = 0; i < hdim; ++i)
-      for (int c = 0; c < 3; ++c) qp[c] += sx[3 * (k * hdim + i) + c];
-    for (i64 i = 0; i < hdim; ++i) s += sw[k * hdim + i];
-    double qr = 1.75 * std::sqrt(s);
-    for (int c = 0; c < 3; ++c) qp[c] /= (double)hdim;
-    double qr2 = qr * qr;
+    double qp[3], qr2;
+    panel_ball(k, hdim, sx, sw, qp, &qr2);
     i64 o = offs[k];
     for (i64 j = 0; j < nt; ++j) {
       double d0 = tx[3 * j] - qp[0], d1 = tx[3 * j + 1] - qp[1], d2 = tx[3 * j + 2] - qp[2];
       if (d0 * d0 + d1 * d1 + d2 * d2 < qr2) idx[o++] = j + 1;
+    }
+  }
+}
+
+i64 orc_nearlist_count(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
+                       const double* tx, i64* offs) {
+  return orc_nearlist_count_mt(npan, hdim, sx, sw, nt, tx, offs, 1);
+}
+
+void orc_nearlist_fill(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
+                       const double* tx, const i64* offs, i64* idx) {
+  orc_nearlist_fill_mt(npan, hdim, sx, sw, nt, tx, offs, idx, 1);
+}
+
+/* Labels only (patch_refine_mod.f90:423-502 levels, :516-615 classification):
+   for every pair of the CSR (offs 0-based, idx 1-based), the level 1..4 that
+   serves it or 5 for RRQ.  Panels in nthreads threads. */
+void orc_labels(i64 npan, i64 order, const double* sx, const double* rts, const double* rps,
+                const double* tx, const i64* offs, const i64* idx, i64* label, i64 nthreads) {
+  const i64 h = order * (order + 1) / 2, korder = order - 1, orderff = order + 2;
+  patch_ref(korder, h, orderff, 4);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? (int)nthreads : 1)
+  for (i64 k = 0; k < npan; ++k) {
+    const i64 m = offs[k + 1] - offs[k];
+    if (m <= 0) continue;
+    PanelLevels lv;
+    build_patch_levels(korder, h, sx + 3 * k * h, rts + 3 * k * h, rps + 3 * k * h, orderff, 1.4, 4, lv);
+    for (i64 j = 0; j < m; ++j) {
+      const double* x = tx + 3 * (idx[offs[k] + j] - 1);
+      i64 lab = 5;
+      for (i64 il = 0; il < 4; ++il)
+        if (!inside_any_ball(lv.lev[il], x)) { lab = il + 1; break; }
+      label[offs[k] + j] = lab;
     }
   }
 }

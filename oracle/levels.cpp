@@ -91,7 +91,7 @@ void lap3dsdlpmat(i64 m, const double* tx, i64 n, const double* sx, const double
     }
 }
 
-static bool inside_any(const PanelLevel& L, const double* x) {
+bool inside_any_ball(const PanelLevel& L, const double* x) {
   for (i64 k = 0; k < L.nsub; ++k) {
     double d0 = L.qpoint[3 * k] - x[0], d1 = L.qpoint[3 * k + 1] - x[1], d2 = L.qpoint[3 * k + 2] - x[2];
     if (d0 * d0 + d1 * d1 + d2 * d2 < L.qradii2[k]) return true;
@@ -111,7 +111,7 @@ void lap3dsdlpmat_levels(i64 m, const double* tx, i64 npols, const PanelLevels& 
     ibat.clear(); inext.clear();
     for (i64 jj = 0; jj < nrem; ++jj) {
       i64 j = irem[jj];
-      if (inside_any(L, tx + 3 * j)) inext.push_back(j);
+      if (inside_any_ball(L, tx + 3 * j)) inext.push_back(j);
       else ibat.push_back(j);
     }
     i64 nbat = (i64)ibat.size(), nl = L.n;

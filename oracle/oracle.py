@@ -137,16 +137,28 @@ def rrq(order, tx, sx, snx, sw, rts, rps, exterior=True, ichart=0, sxbd_chart=No
     return dict(As=As, Ad=Ad, Omega=Om, Ias=Ia, label=lab, timeinfo=ti)
 
 
-def nearlist(hdim, sx, sw, tx):
+def nearlist(hdim, sx, sw, tx, nthreads=1):
     sx = _f(sx); sw = _f(sw); tx = _f(tx)
     npan = sw.shape[0] // hdim
     offs = np.zeros(npan + 1, dtype=np.int64)
-    nnz = lib().orc_nearlist_count(i64(npan), i64(hdim), _p(sx), _p(sw), i64(tx.shape[0]), _p(tx), _p(offs))
-    lib().orc_nearlist_count.restype = i64
+    lib().orc_nearlist_count_mt(i64(npan), i64(hdim), _p(sx), _p(sw), i64(tx.shape[0]), _p(tx), _p(offs),
+                                i64(nthreads))
     nnz = int(offs[-1])
     idx = np.zeros(nnz, dtype=np.int64)
-    lib().orc_nearlist_fill(i64(npan), i64(hdim), _p(sx), _p(sw), i64(tx.shape[0]), _p(tx), _p(offs), _p(idx))
+    lib().orc_nearlist_fill_mt(i64(npan), i64(hdim), _p(sx), _p(sw), i64(tx.shape[0]), _p(tx), _p(offs), _p(idx),
+                               i64(nthreads))
     return offs, idx
+
+
+def labels(order, geo, tx, offs, idx, nthreads=1):
+    """level label (1..4, or 5 for RRQ) of every pair of a CSR near list"""
+    h = order * (order + 1) // 2
+    npan = geo["sw"].shape[0] // h
+    lab = np.zeros(int(offs[-1]), dtype=np.int64)
+    lib().orc_labels(i64(npan), i64(order), _p(_f(geo["sx"])), _p(_f(geo["rts"])), _p(_f(geo["rps"])), _p(_f(tx)),
+                     _p(np.ascontiguousarray(offs, dtype=np.int64)), _p(np.ascontiguousarray(idx, dtype=np.int64)),
+                     _p(lab), i64(nthreads))
+    return lab
 
 
 def close_apply(order, geo, tx, offs, idx, ub, ubn, exterior=True, ichart=0, sxbd_all=None, rv_all=None,

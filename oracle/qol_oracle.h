@@ -80,6 +80,8 @@ void lap3dsdlpmat_levels(i64 m, const double* tx, i64 npols, const PanelLevels& 
                          i64* label);
 void lap3dsdlpmat(i64 m, const double* tx, i64 n, const double* sx, const double* snx,
                   const double* sw, double* As, double* Ad);
+/* the ball test of one level: x inside some sub-patch ball (strict <) */
+bool inside_any_ball(const PanelLevel& L, const double* x);
 
 /* ---- RRQ (rrq.cpp) ---- */
 struct RRQStats {

@@ -181,9 +181,12 @@ void Engine::set_targets(i64 nt_, const double* tx, bool device_ptr) {
                           stream));
   nnz = -1;
   evaluated = false;
+  tlists_nnz = -1;  // the per-target pair lists belong to the old targets
 }
 
 void Engine::alloc_pairs() {
+  tlists_nnz = -1;  // a new pair list: the per-target lists are rebuilt on the next apply
+  val_current = false;
   d_label.alloc((size_t)(nnz > 0 ? nnz : 1));
   d_ppanel.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
   d_wout.alloc(sizeof(double) * 2 * (size_t)h * (nnz > 0 ? nnz : 1));

@@ -25,6 +25,16 @@ def order_info(p):
                 nl=info[5:9].copy(), nsub=info[9:13].copy())
 
 
+def order_level(p, lev):
+    """one refinement level's reference table: uv, wt, interp (n, h), suboff, subtyp"""
+    info = order_info(p)
+    n, ns, h = int(info["nl"][lev - 1]), int(info["nsub"][lev - 1]), info["h"]
+    uv = np.zeros((n, 2)); wt = np.zeros(n); im = np.zeros((n, h), order="F")
+    so = np.zeros(ns + 1, dtype=np.int64); st = np.zeros(ns, dtype=np.int64)
+    check(lib().qol_order_level(i64(p), i64(lev), ptr(uv), ptr(wt), ptr(im), ptr(so), ptr(st)), "qol_order_level")
+    return dict(uv=uv, wt=wt, interpmat=im, suboff=so, subtyp=st)
+
+
 class Engine:
     """Close evaluation of S and D for a surface of npan triangles of order p.
 

@@ -141,3 +141,38 @@ def test_one_crowded_panel(geo, oracle):
     assert np.all(np.isfinite(val))
     assert np.array_equal(u[idx - 1], val)
     e.close()
+
+
+def test_engine_reuse_new_targets_same_nnz(geo):
+    """One engine reused with a second target set whose pair list has the same
+    number of pairs (targets permuted, plus targets with no pairs, so nt grows):
+    the per-target sums must be rebuilt, giving a fresh engine's potential."""
+    say("engine reuse: new targets, equal pair count")
+    ub, ubn = _density(geo)
+    tx, tri, _ = W.targets(geo, 300, -5.0, -1.0, sides=0, seed=11)
+    e = Engine(geo)
+    e.set_targets(tx)
+    e.nearlist()
+    e.eval(exterior=True, side_tol=1e-10)
+    u_a, _ = e.apply(ub, ubn, np.zeros(len(tx)))
+    offs, idx = e.pairs()
+    # second set: the same points in another order, then 200 points with no pairs
+    perm = np.random.default_rng(4).permutation(len(tx))
+    inv = np.argsort(perm)
+    far = tx.mean(0) + np.array([50.0, 0.0, 0.0]) + np.random.default_rng(5).standard_normal((200, 3))
+    tx_b = np.concatenate([tx[perm], far])
+    idx_b = np.concatenate([np.sort(inv[idx[offs[k]:offs[k + 1]] - 1]) + 1 for k in range(len(offs) - 1)])
+    e.set_targets(tx_b)
+    e.set_pairs(offs, idx_b)
+    e.eval(exterior=True, side_tol=1e-10)
+    u_b, _ = e.apply(ub, ubn, np.zeros(len(tx_b)))
+    e.close()
+    f = Engine(geo)
+    f.set_targets(tx_b)
+    f.set_pairs(offs, idx_b)
+    f.eval(exterior=True, side_tol=1e-10)
+    u_f, _ = f.apply(ub, ubn, np.zeros(len(tx_b)))
+    f.close()
+    assert np.array_equal(u_b, u_f)
+    assert np.abs(u_b[:len(tx)] - u_a[perm]).max() <= 1e-12 * np.abs(u_a).max()
+    assert not u_b[len(tx):].any()

@@ -1,8 +1,13 @@
 """The full benchmark workload (VMEC surface 48x120 charts, order 8, 14,250
-triangles, 200,000 targets on both sides, 1.94M pairs) through the engine,
-checked by properties that hold at any size: the Green's representation
-identity on every target, bit-identical repeated corrections, and the
-one-stream schedule giving the four-stream result bit for bit."""
+triangles, 200,000 targets on both sides, 1.94M pairs) through the engine.
+
+Against the oracle at full size: the near-list CSR and every pair's label bit
+for bit, the order-8 level nodes and balls of a sample of panels bit for bit,
+and that sample's weights.  Size-independent properties: the Green's
+representation identity on every target, bit-identical repeated corrections,
+and the one-stream schedule giving the four-stream result bit for bit."""
+import os
+
 import numpy as np
 import pytest
 
@@ -24,6 +29,68 @@ def bench_case():
     ub, gu = W.charge_field(geo["sx"], pos, q)
     ubn = (geo["snx"] * gu).sum(1)
     return geo, tx, dist, pos, q, ub, ubn
+
+
+NTHREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def test_bench_workload_lists_labels_levels_vs_oracle(bench_case, oracle):
+    geo, tx, dist, pos, q, ub, ubn = bench_case
+    h = geo["h"]
+    e = Engine(geo)
+    e.set_targets(tx)
+    nnz = e.nearlist()
+    offs, idx = e.pairs()
+    say(f"near list on the GPU: {nnz} pairs; oracle with {NTHREADS} threads")
+    o_offs, o_idx = oracle.nearlist(h, geo["sx"], geo["sw"], tx, nthreads=NTHREADS)
+    assert np.array_equal(offs, o_offs) and np.array_equal(idx, o_idx)
+    e.eval(exterior=True, side_tol=1e-10)
+    lab = e.weights()["label"]
+    o_lab = oracle.labels(8, geo, tx, offs, idx, nthreads=NTHREADS)
+    say("labels L1..L4, RRQ", np.bincount(lab, minlength=6)[1:].tolist())
+    assert np.array_equal(lab, o_lab)
+    # order-8 level nodes and balls (the chained FP64 MFMA of the level kernel)
+    lg, bl = e.level_data()
+    rng = np.random.default_rng(5)
+    sample = np.sort(rng.choice(geo["ntri"], 12, replace=False))
+    for k in sample:
+        sl = slice(k * h, (k + 1) * h)
+        L = oracle.patch_levels(8, geo["sx"][sl], geo["rts"][sl], geo["rps"][sl])
+        assert np.array_equal(lg[k, 0:3].T, L["sx"]), k
+        assert np.array_equal(lg[k, 6], L["sw"]), k
+        assert np.array_equal(bl[k, :, 0:3], L["qpoint"]) and np.array_equal(bl[k, :, 3], L["qradii2"]), k
+        assert np.max(np.abs(lg[k, 3:6].T - L["snx"])) < 1e-15
+    del lg, bl
+    # the sample's weights: level pairs to 1e-13 of the row maximum; RRQ pairs
+    # to the order-8 conditioning bound, and their action on the density to 1e-10
+    w = e.weights()
+    oracle.set_side_tol(1e-10)
+    try:
+        worst = [0.0, 0.0, 0.0]
+        for k in sample[:6]:
+            sl = slice(k * h, (k + 1) * h)
+            a, b = offs[k], offs[k + 1]
+            if b == a:
+                continue
+            r = oracle.rrq(8, tx[idx[a:b] - 1], geo["sx"][sl], geo["snx"][sl], geo["sw"][sl], geo["rts"][sl],
+                           geo["rps"][sl], exterior=True)
+            assert np.array_equal(w["label"][a:b], r["label"])
+            lv = r["label"] <= 4
+            for key, dens in (("As", ubn[sl]), ("Ad", ub[sl])):
+                g = w[key][a:b]
+                scale = np.abs(r[key]).max(axis=1, keepdims=True)
+                err = np.abs(g - r[key]) / scale
+                if lv.any():
+                    worst[0] = max(worst[0], err[lv].max())
+                if (~lv).any():
+                    worst[1] = max(worst[1], err[~lv].max())
+                    act = np.abs(g[~lv] @ dens - r[key][~lv] @ dens) / (np.abs(r[key][~lv]) @ np.abs(dens))
+                    worst[2] = max(worst[2], act.max())
+        say(f"sampled weights vs oracle: levels {worst[0]:.2e}, RRQ {worst[1]:.2e}, RRQ action {worst[2]:.2e}")
+        assert worst[0] < 1e-13 and worst[1] < 2e-7 and worst[2] < 1e-10
+    finally:
+        oracle.set_side_tol(1e300)
+    e.close()
 
 
 def _run(geo, tx, ub, ubn):

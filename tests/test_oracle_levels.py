@@ -122,9 +122,16 @@ def test_grf_fixture_order4_oracle(oracle, geo4, fixture4):
     oracle.set_side_tol(1e300)
     oracle.close_apply(4, geo4, geo4["sx"], offs, idx, ub, ubn, exterior=True, u=u, nthreads=8)
     err = np.abs(u).max() / np.abs(ubn).max()
-    assert 5e-4 < err < 2e-3, err
-    # the residual field has the reference's shape: large where the reference's is
+    # the oracle's value (the reference reports 1.343e-3): pinned to 5 per cent
+    assert abs(err - 1.808e-3) < 0.05 * 1.808e-3, err
+    # per node position of the order-4 VR rule, the largest residuals of the two
+    # runs agree within a factor 2.5 (the re-derived SSQ and solid angle give a
+    # residual field of the same size but not the same pattern)
     uf = fixture4["u"]
+    loc = np.arange(len(u)) % 10
+    for i in range(10):
+        r = np.abs(u[loc == i]).max() / np.abs(uf[loc == i]).max()
+        assert 0.4 < r < 2.5, (i, r)
     assert np.corrcoef(np.abs(u), np.abs(uf))[0, 1] > 0.5
 
 
