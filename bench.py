@@ -7,15 +7,24 @@ signed distance h 10^U, U ~ U[-6,-1], on both sides; near list by the
 reference's ball rule; density from 8 point charges 0.4 outside the surface.
 
 A step is one pass of the close-evaluation path over the resident batch:
-refinement-level classification, direct S/D on the levels, RRQ for the rest,
-the naive-block subtraction and the fixed-order scatter into the potential
-(the reference's "rrq+fill" region).  Geometry, targets, near list and density
-are in HBM before timing starts.  The GRF identity S[du/dn] - D[u] = u (inside)
-/ 0 (outside) is checked on every target after the timed steps.
+pair labelling, direct S/D on the refinement levels, RRQ for the rest, the
+naive-block subtraction and the fixed-order scatter into the potential (the
+reference's "rrq+fill" region).  Geometry, targets, near list and density are
+in HBM before timing starts.  Two paths are timed on the same workload:
+
+  plain  FP64 throughout, every pair labelled by the reference's ball rule;
+  fast   the learned level selector (qotential_amd/selector_model.npz) with the
+         ball rule below confidence 0.9, and split precision (FP32 matrix cores)
+         for the direct weights of levels 1..--split-levels.
+
+`value` is the path named by --path (default fast).  Both paths' Green's
+identity errors (S[du/dn] - D[u] = u inside, 0 outside, on every target) and
+the fast path's potential difference from the plain path are reported.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -25,58 +34,79 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+# the reference's own single-thread rate on its order-4 fixture run: 137,259
+# close pairs in 0.574 s of "rrq+fill" (web/stellarator/fixtures/native/order4-direct.json:20)
+REFERENCE_ORDER4_PAIRS_PER_S = 137259 / 0.574
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mp", type=int, default=48)
     ap.add_argument("--np", type=int, default=120)
     ap.add_argument("--order", type=int, default=8)
     ap.add_argument("--targets", type=int, default=200000)
+    ap.add_argument("--path", choices=("fast", "plain"), default="fast")
+    ap.add_argument("--split-levels", type=int, default=2,
+                    help="fast path: levels 1..N get their direct weights from the FP32 matrix cores")
+    ap.add_argument("--threshold", type=float, default=0.9, help="fast path: selector confidence threshold")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-pairs", type=int, default=600000)
+    ap.add_argument("--cpu-sample-pairs", type=int, default=300000)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (a plain copy "
                          "kernel of known size that scales the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
-def kernel_flops(p, npan, counts, nrrq, nrrq_panels):
-    """Algorithmic FP64 FLOPs per launch of the kernels that have a fixed count
-    (from the formulas, not from executed instructions).  counts[l] = pairs
-    served by level l (l = 0..3), nrrq = RRQ pairs, nrrq_panels = panels with
-    at least one RRQ pair."""
+def level_flops(p, counts):
+    """algorithmic FLOPs of the direct level kernels: per pair served at level l,
+    nl(l) (19 + 4h) (entries with one rsqrt, two contractions of h columns)"""
+    from qotential_amd.engine import order_info
+    info = order_info(p)
+    h = info["h"]
+    return [float(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h)) for l in range(4)]
+
+
+def kernel_flops(p, npan, counts, nrrq, nrrq_panels, split_levels):
+    """Algorithmic FLOPs per launch of the kernels that have a fixed count (from
+    the formulas, not from executed instructions).  counts[l] = pairs served by
+    level l+1, nrrq = RRQ pairs, nrrq_panels = panels with an RRQ pair."""
     from qotential_amd.engine import order_info
     info = order_info(p)
     h, nq, ncoef = info["h"], info["nq"], info["ncoef"]
     nb = 3 * nq
     ntot = int(sum(info["nl"]))
-    return {
-        # 9 sequential FMAs per (level node, panel node); matrix cores up to order 8
-        "levels": ("level_geom_mfma_kernel" if (h + 3) // 4 <= 9 else "level_geom_kernel", 18.0 * h * ntot * npan),
-        # per level node: 19 for the S/D entry (incl. one rsqrt), 4h for the two contractions
-        "direct": ("direct_mfma_kernel", float(sum(int(counts[l]) * int(info["nl"][l]) * (19 + 4 * h)
-                                                   for l in range(4)))),
-        # real Gauss-Jordan 2h^3 (rrq_real_inverse_kernel); quaternion Gauss-Jordan
-        # 32h^3, C = B Fn^T 2h^3 and the As rows C G1 8h^3 (rrq_quat_inverse_kernel)
-        "rrq_solve": ("rrq_real_inverse_kernel+rrq_quat_inverse_kernel", 44.0 * h ** 3 * nrrq_panels),
+    lf = level_flops(p, counts)
+    ns = max(0, min(4, split_levels))
+    out = {
+        # 9 sequential FMAs per (level node, panel node); matrix cores up to order 12
+        "levels": ("level_geom_mfma_kernel", 18.0 * h * ntot * npan, PEAK_FP64_TFLOPS),
+        # real Gauss-Jordan 2h^3; quaternion Gauss-Jordan 32h^3, C = B Fn^T 2h^3, As rows 8h^3
+        "rrq_solve": ("rrq_real_inverse_kernel+rrq_quat_inverse_kernel", 44.0 * h ** 3 * nrrq_panels,
+                      PEAK_FP64_TFLOPS),
         # boundary moments 48 nb ncoef, degree recursion ~200 ncoef
-        "rrq_moments": ("rrq_target_kernel", float(nrrq) * (48 * nb * ncoef + 200 * ncoef)),
+        "rrq_moments": ("rrq_target_kernel", float(nrrq) * (48 * nb * ncoef + 200 * ncoef), PEAK_FP64_TFLOPS),
         # W = G V: 2 (2h)(5h) per RRQ pair
-        "rrq_weights": ("rrq_weights_kernel", float(nrrq) * 20 * h * h),
+        "rrq_weights": ("rrq_weights_kernel", float(nrrq) * 20 * h * h, PEAK_FP64_TFLOPS),
     }
+    # the direct stage: levels 1..ns on FP32 matrix cores, the rest on FP64
+    if ns > 0:
+        out["direct_split"] = ("direct_split_kernel", sum(lf[:ns]), PEAK_FP32_TFLOPS)
+    if ns < 4:
+        out["direct"] = ("direct_mfma_kernel", sum(lf[ns:]), PEAK_FP64_TFLOPS)
+    return out
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_p8_48x120.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_p8_48x120.json")
 
 
 def measured_traffic(kernel, a):
     """HBM bytes per launch of `kernel` ('+'-joined names are summed) from the
-    FETCH_SIZE / WRITE_SIZE counter runs of this workload (scaled by a plain copy
-    of known size), or None when no such record exists for this configuration."""
+    FETCH_SIZE / WRITE_SIZE counter runs of this workload and path (scaled by a
+    plain copy of known size), or None when no such record exists."""
     try:
         with open(TRAFFIC_FILE) as f:
             rec = json.load(f)
@@ -93,14 +123,22 @@ def measured_traffic(kernel, a):
     return total
 
 
-def cpu_baseline(geo, tx, offs, idx, ub, ubn, sample_pairs):
-    """The oracle (CPU FP64 restatement, 1 thread) on a bounded sample of panels."""
-    from oracle import oracle as O
-    O.build()
-    O.set_side_tol(1e-10)
+def cpu_info():
+    info = {"nproc": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def _panel_sample(offs, idx, sample_pairs, seed):
     npan = len(offs) - 1
-    rng = np.random.default_rng(7)
-    order = rng.permutation(npan)
+    order = np.random.default_rng(seed).permutation(npan)
     chosen, tot = [], 0
     for k in order:
         c = int(offs[k + 1] - offs[k])
@@ -111,19 +149,91 @@ def cpu_baseline(geo, tx, offs, idx, ub, ubn, sample_pairs):
         if tot >= sample_pairs:
             break
     chosen = np.sort(np.array(chosen))
-    # restrict the CSR to the chosen panels
-    sub_offs = np.zeros(npan + 1, dtype=np.int64)
-    keep = np.zeros(npan, dtype=bool)
-    keep[chosen] = True
-    cnt = np.where(keep, np.diff(offs), 0)
-    sub_offs[1:] = np.cumsum(cnt)
+    cnt = np.zeros(npan, dtype=np.int64)
+    cnt[chosen] = np.diff(offs)[chosen]
+    sub_offs = np.concatenate([[0], np.cumsum(cnt)])
     sub_idx = np.concatenate([idx[offs[k]:offs[k + 1]] for k in chosen])
+    return sub_offs, sub_idx, tot, len(chosen)
+
+
+def cpu_baseline(geo, tx, offs, idx, ub, ubn, sample_pairs):
+    """The oracle (CPU FP64 restatement, OpenMP dynamic,4 over panels as the
+    reference's GRF loop) on a bounded random sample of panels of the same
+    workload: one thread and every host thread the process may use."""
+    from oracle import oracle as O
+    O.build()
+    O.set_side_tol(1e-10)
+    info = cpu_info()
+    nthr = max(1, min(info["nproc"], 64))
+    res = {}
+    for threads, pairs in ((1, sample_pairs // 2), (nthr, sample_pairs * min(nthr, 16) // 4)):
+        so, si, tot, npn = _panel_sample(offs, idx, pairs, 7 + threads)
+        t0 = time.perf_counter()
+        O.close_apply(geo["p"], geo, tx, so, si, ub, ubn, exterior=True, nthreads=threads)
+        dt = time.perf_counter() - t0
+        res[threads] = (tot / dt, tot, npn, dt)
+    v1, vn = res[1], res[nthr]
+    return dict(value=float(vn[0]), unit="pairs/s", cores=nthr, kind="port",
+                value_1thread=float(v1[0]), cpu=info,
+                sample=f"{vn[1]} pairs on {vn[2]} random panels of the same workload, oracle FP64 with {nthr} "
+                       f"OpenMP threads (dynamic,4 over panels) in {vn[3]:.1f} s; 1 thread: {v1[1]} pairs on "
+                       f"{v1[2]} panels in {v1[3]:.1f} s")
+
+
+def cpu_reference_configs():
+    """The oracle at one thread on (a) the CPU reference configuration (16 x 40
+    charts, order 4, 4,000 interior targets at h 10^U, U ~ U[-6,-1], point-charge
+    density, GRF check) and (b) the reference's order-4 fixture run (built-in
+    surface 12 x 36, every node a target, 137,259 pairs), beside the
+    reference's own single-thread rate on (b)."""
+    from oracle import oracle as O
+    from qotential_amd import workload as W
+    O.build()
+    out = {}
+    geo = W.geometry(16, 40, 4, modes=W.VMEC_BENCH)
+    tx, tri, dist = W.targets(geo, 4000, -6.0, -1.0, sides=-1, seed=1)
+    pos, q = W.point_charges(geo, nchg=8, offset=0.4, seed=2)
+    ub, gu = W.charge_field(geo["sx"], pos, q)
+    ubn = (geo["snx"] * gu).sum(1)
+    offs, idx = O.nearlist(geo["h"], geo["sx"], geo["sw"], tx, nthreads=8)
+    u = O.far_direct(tx, geo["sx"], geo["snx"], geo["sw"], ub, ubn, nthreads=8)
+    O.set_side_tol(1e-10)
     t0 = time.perf_counter()
-    O.close_apply(geo["p"], geo, tx, sub_offs, sub_idx, ub, ubn, exterior=True, nthreads=1)
+    O.close_apply(4, geo, tx, offs, idx, ub, ubn, exterior=True, u=u, nthreads=1)
     dt = time.perf_counter() - t0
-    return dict(value=float(tot / dt), unit="pairs/s", cores=1, kind="port",
-                sample=f"{tot} pairs on {len(chosen)} randomly chosen panels of the same workload "
-                       f"(oracle, FP64, 1 thread, {dt:.1f} s)")
+    uex, _ = W.charge_field(tx, pos, q)
+    out["config1_16x40_p4_4000_interior"] = dict(
+        pairs=int(offs[-1]), seconds=dt, pairs_per_s=offs[-1] / dt, threads=1,
+        grf_rel_err=float(np.abs(u - uex).max() / np.abs(uex).max()))
+    geo4 = W.geometry(12, 36, 4)
+    ub, gu = W.harmonic_exp(geo4["sx"])
+    ubn = (geo4["snx"] * gu).sum(1)
+    offs, idx = O.nearlist(10, geo4["sx"], geo4["sw"], geo4["sx"], nthreads=8)
+    O.set_side_tol(1e300)
+    t0 = time.perf_counter()
+    O.close_apply(4, geo4, geo4["sx"], offs, idx, ub, ubn, exterior=True, nthreads=1)
+    dt = time.perf_counter() - t0
+    O.set_side_tol(1e-10)
+    out["order4_fixture_12x36"] = dict(pairs=int(offs[-1]), seconds=dt, pairs_per_s=offs[-1] / dt, threads=1,
+                                       reference_pairs_per_s=REFERENCE_ORDER4_PAIRS_PER_S)
+    return out
+
+
+def timed_steps(eng, steps, warmup):
+    for _ in range(warmup):
+        eng.eval(exterior=True, side_tol=1e-10)
+        eng.apply_resident()
+    eng.sync()
+    stage_tot = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.eval(exterior=True, side_tol=1e-10)
+        eng.apply_resident()
+        s = eng.stage_ms()
+        stage_tot = s if stage_tot is None else {k: stage_tot[k] + s[k] for k in s}
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    return elapsed, {k: v / steps for k, v in stage_tot.items()}
 
 
 def main():
@@ -134,6 +244,7 @@ def main():
         sys.exit(2)
     from qotential_amd import workload as W
     from qotential_amd.engine import Engine
+    from qotential_amd.selector import MODEL_FILE, Selector
 
     t_setup = time.perf_counter()
     geo = W.geometry(a.mp, a.np, a.order, modes=W.VMEC_BENCH)
@@ -141,42 +252,49 @@ def main():
     pos, q = W.point_charges(geo, nchg=8, offset=0.4, seed=2)
     ub, gu = W.charge_field(geo["sx"], pos, q)
     ubn = (geo["snx"] * gu).sum(1)
+    uex, _ = W.charge_field(tx, pos, q)
+    uref = np.where(dist < 0.0, uex, 0.0)
+    scale = np.abs(uex).max()
     print(f"[bench] geometry {geo['ntri']} triangles p={a.order}, {a.targets} targets "
           f"({time.perf_counter() - t_setup:.1f}s)", flush=True)
-    eng = Engine(geo)
-    eng.set_targets(tx)
-    t0 = time.perf_counter()
-    nnz = eng.nearlist()
-    t_near = time.perf_counter() - t0
-    print(f"[bench] near list: {nnz} pairs ({t_near * 1e3:.1f} ms incl. host sync)", flush=True)
-    far = eng.far(ub, ubn)
-    print("[bench] far field done", flush=True)
-    eng.eval(exterior=True, side_tol=1e-10)
-    u0, _ = eng.apply(ub, ubn, far)  # uploads ub, ubn, u (= far) and checks the path once
-    w = eng.weights()
-    labels = np.bincount(w["label"], minlength=6)[1:6]
-    stage = eng.stage_ms()
-    print(f"[bench] labels (L1..L4, RRQ): {labels.tolist()}  stages ms: "
-          f"{ {k: round(v, 2) for k, v in stage.items()} }", flush=True)
 
-    for _ in range(a.warmup):
+    sel = Selector.load(MODEL_FILE)
+    runs = {}
+    for path in ("plain", "fast"):
+        eng = Engine(geo)
+        eng.set_targets(tx)
+        t0 = time.perf_counter()
+        nnz = eng.nearlist()
+        t_near = time.perf_counter() - t0
+        far = eng.far(ub, ubn)
+        if path == "fast":
+            eng.set_selector(sel, a.threshold)
+            eng.set_precision(a.split_levels)
         eng.eval(exterior=True, side_tol=1e-10)
-        eng.apply_resident()
-    eng.sync()
-    # timed region: K steps bracketed by device synchronisation
-    t0 = time.perf_counter()
-    stage_tot = {k: 0.0 for k in stage}
-    for _ in range(a.steps):
-        eng.eval(exterior=True, side_tol=1e-10)
-        eng.apply_resident()
-        s = eng.stage_ms()
-        for k in stage_tot:
-            stage_tot[k] += s[k]
-    eng.sync()
-    elapsed = time.perf_counter() - t0
-    ms_step = 1e3 * elapsed / a.steps
-    value = nnz * a.steps / elapsed
-    print(f"[bench] {a.steps} steps: {ms_step:.2f} ms/step, {value:.4g} pairs/s", flush=True)
+        u0, _ = eng.apply(ub, ubn, far)  # uploads ub, ubn, u (= far) and checks the path once
+        lab = eng.weights()["label"]
+        counts = np.bincount(lab, minlength=6)
+        elapsed, stage = timed_steps(eng, a.steps, a.warmup)
+        u, _ = eng.apply(ub, ubn, far)
+        offs, idx = eng.pairs()
+        csum = np.concatenate([[0], np.cumsum(lab == 5)])
+        rrq_panels = int(((csum[offs[1:]] - csum[offs[:-1]]) > 0).sum())
+        runs[path] = dict(eng=eng, nnz=nnz, t_near=t_near, counts=counts, elapsed=elapsed, stage=stage, u=u,
+                          u0=u0, rrq_panels=rrq_panels, offs=offs, idx=idx)
+        ms = 1e3 * elapsed / a.steps
+        print(f"[bench] {path}: near list {nnz} pairs ({t_near * 1e3:.1f} ms incl. host sync); served by own nodes, "
+              f"L1..L4, RRQ: {counts.tolist()}; {a.steps} steps {ms:.2f} ms/step, {nnz * a.steps / elapsed:.4g} "
+              f"pairs/s", flush=True)
+        if path == "plain":
+            eng.close()
+
+    nnz = runs["plain"]["nnz"]
+    grf = {k: float(np.abs(r["u"] - uref).max() / scale) for k, r in runs.items()}
+    diff = float(np.abs(runs["fast"]["u"] - runs["plain"]["u"]).max() / scale)
+    repeat = {k: float(np.abs(r["u"] - r["u0"]).max() / np.abs(r["u0"]).max()) for k, r in runs.items()}
+    value = {k: nnz * a.steps / r["elapsed"] for k, r in runs.items()}
+    print(f"[bench] GRF rel err plain {grf['plain']:.3e}, fast {grf['fast']:.3e}; fast vs plain: max |du| / max |u| "
+          f"= {diff:.2e}; speed-up {value['fast'] / value['plain']:.3f}", flush=True)
 
     if a.calib_copy_mib > 0:
         # plain copy kernel of known size for the HBM-traffic counter calibration (not timed)
@@ -188,51 +306,56 @@ def main():
         print(f"[bench] calibration copy: {a.calib_copy_mib} MiB read, {a.calib_copy_mib} MiB written "
               f"({cms.value:.3f} ms, {gbs:.0f} GB/s)", flush=True)
 
-    # GRF identity on every target
-    u, _ = eng.apply(ub, ubn, far)
-    uex, _ = W.charge_field(tx, pos, q)
-    inside = dist < 0.0
-    grf = float(np.abs(u - np.where(inside, uex, 0.0)).max() / np.abs(uex).max())
-    repeat = float(np.abs(u - u0).max() / np.abs(u0).max())
-    print(f"[bench] GRF rel err {grf:.3e}; repeat-run rel diff {repeat:.1e}", flush=True)
-
-    # roofline of the dominant kernel among those with an algorithmic count
-    # (FP64 compute bound: MI355X FP64 matrix and vector peaks are both 78.6 TFLOP/s)
-    offs, idx = eng.pairs()
-    lab = w["label"]
-    csum = np.concatenate([[0], np.cumsum(lab == 5)])
-    rrq_per_panel = csum[offs[1:]] - csum[offs[:-1]]
-    kf = kernel_flops(a.order, geo["ntri"], labels[:4], int(labels[4]), int((rrq_per_panel > 0).sum()))
-    per = {k: v / a.steps for k, v in stage_tot.items()}
+    # roofline of the dominant kernel (most algorithmic work) of the measured path
+    r = runs[a.path]
+    ns = a.split_levels if a.path == "fast" else 0
+    kf = kernel_flops(a.order, geo["ntri"], r["counts"][1:5], int(r["counts"][5]), r["rrq_panels"], ns)
+    stage_of = {"levels": "levels", "rrq_solve": "rrq_solve", "rrq_moments": "rrq_moments",
+                "rrq_weights": "rrq_weights", "direct_split": "direct", "direct": "direct"}
     kernels = {}
-    for stage, (kname, fl) in kf.items():
-        ms = per[stage]
-        kernels[kname] = {"ms": ms, "gflop": fl / 1e9, "tflops": fl / (ms * 1e-3) / 1e12}
-    # the dominant kernel is the one with the most algorithmic work (the direct
-    # level kernel on this workload, also the longest in the one-stream trace);
-    # on four streams the event times of the largest kernels are within a few
-    # per cent of each other, so a choice by event time would change between runs
+    for st, (kname, fl, peak) in kf.items():
+        ms = r["stage"][stage_of[st]]
+        if st.startswith("direct") and ns > 0 and ns < 4:
+            # both direct kernels share the stage's events: split the time by FLOPs at each peak
+            tot = sum(kf[s][1] / kf[s][2] for s in ("direct_split", "direct"))
+            ms = ms * (fl / peak) / tot
+        kernels[kname] = {"ms": ms, "gflop": fl / 1e9, "tflops": fl / (ms * 1e-3) / 1e12, "peak": peak}
     dom = max(kernels, key=lambda k: kernels[k]["gflop"])
-    achieved = kernels[dom]["tflops"]
-    roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(dom, a),
-            "flops_per_launch": kernels[dom]["gflop"] * 1e9, "ms_per_launch": kernels[dom]["ms"]}
+    kd = kernels[dom]
+    roof = {"kernel": dom, "bound": "mfma", "achieved": kd["tflops"], "peak": kd["peak"], "unit": "TFLOP/s",
+            "frac": kd["tflops"] / kd["peak"], "traffic": measured_traffic(dom, a),
+            "flops_per_launch": kd["gflop"] * 1e9, "ms_per_launch": kd["ms"]}
 
     base = None
     if not a.no_cpu_baseline:
-        base = cpu_baseline(geo, tx, offs, idx, ub, ubn, a.cpu_sample_pairs)
-        print(f"[bench] cpu baseline {base['value']:.4g} pairs/s ({base['sample']})", flush=True)
+        p0 = runs["plain"]
+        base = cpu_baseline(geo, tx, p0["offs"], p0["idx"], ub, ubn, a.cpu_sample_pairs)
+        print(f"[bench] cpu baseline {base['value']:.4g} pairs/s with {base['cores']} threads, "
+              f"{base['value_1thread']:.4g} with 1 ({base['cpu'].get('Model name', '?')})", flush=True)
+        base["reference_configs"] = cpu_reference_configs()
+        for k, v in base["reference_configs"].items():
+            print(f"[bench] cpu {k}: {v['pairs']} pairs, {v['pairs_per_s']:.4g} pairs/s (1 thread)", flush=True)
 
+    runs["fast"]["eng"].close()
+    fr = runs["fast"]["counts"]
     line = {
         "metric": "close-eval target-panel pairs/sec at GRF rel err <=1e-9 vs plain FP64 on GPU",
-        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"plain FP64 RRQ, stellarator {a.mp}x{a.np} charts, order {a.order}, "
+        "value": value[a.path], "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * runs[a.path]["elapsed"] / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.path == "plain" else f"f64 (levels 1..{a.split_levels} direct weights f32)",
+        "data": "synthetic",
+        "config": {"workload": f"{a.path} RRQ close evaluation, stellarator {a.mp}x{a.np} charts, order {a.order}, "
                                f"{geo['ntri']} triangles, {a.targets} near-surface targets both sides "
                                f"U[-6,-1], reference ball near list",
-                   "pairs": int(nnz), "labels_L1_L4_RRQ": labels.tolist(), "stage_ms": per, "kernels": kernels,
-                   "grf_rel_err": grf, "rel_err_vs_plain_fp64": 0.0, "repeat_rel_diff": repeat},
+                   "path": a.path, "pairs": int(nnz),
+                   "served_own_L1_L4_RRQ": {k: r_["counts"].tolist() for k, r_ in runs.items()},
+                   "pairs_per_s": value, "speedup_vs_plain_fp64": value["fast"] / value["plain"],
+                   "selector": {"model": os.path.relpath(str(MODEL_FILE), ROOT), "threshold": a.threshold},
+                   "split_levels": a.split_levels,
+                   "stage_ms": {k: r_["stage"] for k, r_ in runs.items()}, "kernels": kernels,
+                   "grf_rel_err": grf, "rel_err_vs_plain_fp64": diff if a.path == "fast" else 0.0,
+                   "repeat_rel_diff": repeat},
         "roofline": roof,
         "cpu_baseline": base,
     }

@@ -61,6 +61,15 @@ int64_t qol_geo_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t n
  * per chart); the same list up to the rounding of the device sin/cos near restol */
 int64_t qol_geo_charts_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
                            const double* mnrz, double restol, int64_t cap, double* charts);
+/* The chart list a driver's mesh initialisation builds (the reference's
+ * stellarator_mesh_init_r64, called at test/stellarator_bench.f90:60 and
+ * web/stellarator/fortran/stellarator_grf_core_mod.f90:472): restol > 0 as
+ * qol_geo_charts; otherwise the aspect-split list of qol_geo_build (one pass
+ * when iterate_aspect is 0).  Returns the chart count, -1 past cap (cap > 0),
+ * -2 on error; charts may be NULL to count only. */
+int64_t qol_geo_plan_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
+                            const double* mnrz, int64_t iterate_aspect, double restol, int64_t cap,
+                            double* charts);
 /* panel_resolution (utils/stellarator_geo_mex.c:359-411) of nchart charts on the
  * GPU: tails(nchart) */
 int qol_geo_curvature_tails_gpu(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode,
@@ -185,6 +194,50 @@ int qol_engine_level_data(void* engine, double* lgeo, double* balls);
 int qol_engine_triplets(void* engine, const int64_t* trow, int64_t* si, int64_t* sj, double* vs,
                         double* vd);
 int qol_engine_stream(void* engine, void** stream);
+/* Split precision (no reference counterpart; the reference computes every
+ * entry in FP64, patch_refine_mod.f90:569-602): pairs served at levels
+ * 1..split_levels get their direct S/D weights from the FP32 matrix cores
+ * (x - y in FP64, entries and chunked accumulation in FP32, totals in FP64);
+ * 0 (the default) keeps every level in FP64.  Takes effect at the next eval. */
+int qol_engine_set_precision(void* engine, int64_t split_levels);
+
+/* ---------------- learned level selector ----------------
+ * No reference counterpart: the reference labels every pair by the ball rule
+ * (src/patch_refine_mod.f90:541-613).  An MLP 10 -> 64 -> 64 -> 7 (ReLU; BF16
+ * weights, FP32 biases and accumulation) on per-pair geometry features picks
+ * class 0 (the panel's own nodes suffice, no correction), 1..4 (direct on that
+ * level), 5 (RRQ) or 6 (direct on level 4 where the rule asks for RRQ); below
+ * the confidence threshold the ball rule labels the pair.  params holds the
+ * kSelNP = 5319 FP32 values W1 (64x10), W2 (64x64), W3 (7x64) row-major, then
+ * b1 (64), b2 (64), b3 (7); norm holds the feature means (10) then 1/std (10).
+ * Features: feat(10, n), see qol_engine_pair_features. */
+int qol_selector_create(void** selector);
+int qol_selector_destroy(void* selector);
+int qol_selector_set_params(void* selector, const float* params, const float* norm);
+int qol_selector_get_params(void* selector, float* params, float* norm);
+/* one batch of all n samples: loss (class-weighted cross-entropy, mean) and its
+ * gradient with respect to params (weights as rounded to BF16) */
+int qol_selector_grad(void* selector, int64_t n, const float* feat, const int64_t* labels,
+                      const float* class_w, float* grad, double* loss);
+/* Adam (beta 0.9, 0.999; eps 1e-8) on FP32 master parameters, BF16 weights
+ * rounded after every step; loss_epoch(epochs) gets each epoch's mean loss */
+int qol_selector_train(void* selector, int64_t n, const float* feat, const int64_t* labels,
+                       const float* class_w, int64_t epochs, int64_t batch, double lr, uint64_t seed,
+                       double* loss_epoch);
+/* class probabilities prob(7, n) */
+int qol_selector_predict(void* selector, int64_t n, const float* feat, float* prob);
+/* the engine labels pairs with the selector (NULL: the ball rule); the model
+ * must outlive the engine's use of it */
+int qol_engine_set_selector(void* engine, void* selector, double threshold);
+/* every pair gets one label (0..5; -1 returns to the ball rule): evaluates all
+ * pairs at one level, by RRQ, or with the panel's own nodes (the selector's
+ * training labels come from such evaluations) */
+int qol_engine_force_label(void* engine, int64_t label);
+/* the ball rule with every level-served pair served `shift` (0..3) levels
+ * finer, at most level 4 (a uniform-refinement comparison for the selector) */
+int qol_engine_refine_levels(void* engine, int64_t shift);
+/* the selector's 10 features of every pair, feat(10, nnz) */
+int qol_engine_pair_features(void* engine, float* feat);
 
 /* No reference counterpart: one plain device-to-device copy of nbytes (a
  * multiple of 16) through hbm_copy_kernel, on buffers it allocates and frees,

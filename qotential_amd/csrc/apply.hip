@@ -8,6 +8,21 @@
 
 namespace qol {
 
+// naive S and D entries of a target against one panel node (zero at coincident points)
+__device__ __forceinline__ void naive_entries(double x0, double x1, double x2, const double* __restrict__ y,
+                                              const double* __restrict__ n, double w, double pi4inv, double& an,
+                                              double& dn) {
+  const double dx = x0 - y[0], dy = x1 - y[1], dz = x2 - y[2];
+  const double rr = dx * dx + dy * dy + dz * dz;
+  an = 0.0;
+  dn = 0.0;
+  if (rr > 1.0e-28) {
+    const double ir = rsqrt_nr(rr);
+    an = pi4inv * w * ir;
+    dn = an * (dx * n[0] + dy * n[1] + dz * n[2]) * (ir * ir);
+  }
+}
+
 // val(pair) = sum_i (S - S_naive)_i ubn_i - (K - K_naive)_i ub_i, i over the
 // panel's h nodes, left to right as in the reference loop.
 // near-correction value of pair j, part p of 4 (nodes p, p+4, ...):
@@ -28,17 +43,33 @@ __device__ __forceinline__ double pair_correction_part(int j, int part, int h, c
   double acc = 0.0;
   for (int i = part; i < h; i += 4) {
     const size_t g = (size_t)k * h + i;
-    const double dx = x0 - sx[3 * g], dy = x1 - sx[3 * g + 1], dz = x2 - sx[3 * g + 2];
-    const double rr = dx * dx + dy * dy + dz * dz;
-    double an = 0.0, dn = 0.0;
-    if (rr > 1.0e-28) {
-      const double ir = rsqrt_nr(rr);
-      an = pi4inv * sw[g] * ir;
-      dn = an * (dx * snx[3 * g] + dy * snx[3 * g + 1] + dz * snx[3 * g + 2]) * (ir * ir);
-    }
+    double an, dn;
+    naive_entries(x0, x1, x2, sx + 3 * g, snx + 3 * g, sw[g], pi4inv, an, dn);
     acc = acc + (w[i] - an) * ubn[g] - (w[h + i] - dn) * ub[g];
   }
   return acc;
+}
+
+// Weights of the pairs labelled 0 (the learned selector found the panel's own
+// nodes accurate enough): the naive h-node entries themselves, formed exactly
+// as the apply forms them, so each such pair's correction is exactly zero.
+__global__ void __launch_bounds__(256) naive_fill_kernel(int nnz, int h, const int8_t* __restrict__ label,
+                                                         const int* __restrict__ tidx, const int* __restrict__ ppanel,
+                                                         const double* __restrict__ tx, const double* __restrict__ sx,
+                                                         const double* __restrict__ snx,
+                                                         const double* __restrict__ sw, double* __restrict__ wout,
+                                                         double* __restrict__ val) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz || label[j] != 0) return;
+  const double pi4inv = 1.0 / (16.0 * atan(1.0));
+  const int t = tidx[j], k = ppanel[j];
+  const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
+  double* w = wout + (size_t)j * 2 * h;
+  for (int i = 0; i < h; ++i) {
+    const size_t g = (size_t)k * h + i;
+    naive_entries(x0, x1, x2, sx + 3 * g, snx + 3 * g, sw[g], pi4inv, w[i], w[h + i]);
+  }
+  if (val) val[j] = 0.0;
 }
 
 __global__ void __launch_bounds__(256) naive_apply_kernel(int nnz, int h, const int* __restrict__ tidx,

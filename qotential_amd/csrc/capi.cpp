@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -59,19 +60,29 @@ void check_orders(int64_t order, int64_t nquad, int64_t orderff, double distff) 
   if (distff != 1.4) throw std::runtime_error("distff must be 1.4 (the reference's level radius factor)");
 }
 
-// engine stage times (ms: levels, classify, direct, rrq setup, rrq solve,
-// projection, root + SSQ, moments+recursion, solid angle, weights) into the
-// reference's timeinfo slots (seconds, accumulated; lap3d_close_mod.f90:714-871):
-// 1 levels, 2 boundary precompute, 3 level direct, 4 SSQ, 5 solid angle
-// (with the projection that orients it), 6 moments and weights, 8 weights solve
-void add_timeinfo(const double* ms, double* timeinfo) {
-  timeinfo[0] += 1e-3 * ms[0];
-  timeinfo[1] += 1e-3 * ms[3];
-  timeinfo[2] += 1e-3 * (ms[1] + ms[2]);
-  timeinfo[3] += 1e-3 * ms[6];
-  timeinfo[4] += 1e-3 * (ms[5] + ms[8]);
-  timeinfo[5] += 1e-3 * (ms[7] + ms[9]);
-  timeinfo[7] += 1e-3 * ms[4];
+// One-panel engine per calling thread for the per-panel drop-ins, kept across
+// calls (streams, per-order tables and buffers are made once per thread and
+// order; each call uploads only the panel, its targets and pair list).  The
+// cache is deliberately never destroyed: thread-exit destructors could run
+// after the HIP runtime has shut down.
+struct PanelEngineCache {
+  std::unique_ptr<Engine> e;
+  int64_t p = -1, ichart = -1;
+};
+
+Engine& panel_engine(int64_t p, int64_t ichart, const double* sx, const double* snx, const double* sw,
+                     const double* rts, const double* rps, const double* sxbd, const double* rv) {
+  static thread_local PanelEngineCache* cache = nullptr;
+  if (!cache) cache = new PanelEngineCache;
+  if (!cache->e || cache->p != p || cache->ichart != ichart) {
+    cache->e.reset();
+    cache->e.reset(new Engine(p, 1, sx, snx, sw, rts, rps, ichart, sxbd, rv, false));
+    cache->p = p;
+    cache->ichart = ichart;
+  } else {
+    cache->e->set_surface(sx, snx, sw, rts, rps, sxbd, rv, false);
+  }
+  return *cache->e;
 }
 
 }  // namespace
@@ -148,6 +159,42 @@ int qol_engine_level_data(void* h, double* lgeo, double* balls) {
 
 int qol_engine_triplets(void* h, const int64_t* trow, int64_t* si, int64_t* sj, double* vs, double* vd) {
   return guarded([&] { E(h)->triplets(trow, si, sj, vs, vd); });
+}
+
+int qol_engine_set_precision(void* h, int64_t split_levels) {
+  return guarded([&] {
+    if (split_levels < 0 || split_levels > 4) throw std::runtime_error("split_levels must be 0..4");
+    E(h)->split_levels = (int)split_levels;
+  });
+}
+
+int qol_engine_set_selector(void* h, void* selector, double threshold) {
+  return guarded([&] {
+    if (threshold < 0.0 || threshold > 1.0) throw std::runtime_error("threshold must be in [0, 1]");
+    E(h)->sel = selector_args(selector, threshold);
+  });
+}
+
+int qol_engine_force_label(void* h, int64_t label) {
+  return guarded([&] {
+    if (label < -1 || label > 5) throw std::runtime_error("forced label must be -1 (ball rule) or 0..5");
+    SelectorArgs S;
+    if (label >= 0) { S.mode = 1; S.forced = (int)label; }
+    E(h)->sel = S;
+  });
+}
+
+int qol_engine_refine_levels(void* h, int64_t shift) {
+  return guarded([&] {
+    if (shift < 0 || shift > 3) throw std::runtime_error("level shift must be 0..3");
+    SelectorArgs S;
+    if (shift > 0) { S.mode = 3; S.forced = (int)shift; }
+    E(h)->sel = S;
+  });
+}
+
+int qol_engine_pair_features(void* h, float* feat) {
+  return guarded([&] { E(h)->pair_features(feat); });
 }
 
 int qol_engine_stream(void* h, void** stream) {
@@ -355,7 +402,7 @@ void qol_rrq_mex_(const int64_t* m, const double* tx, const int64_t* n, const do
     std::memset(Omega, 0, sizeof(double) * 4 * h * mm);
     std::memset(IalphaAsvestas, 0, sizeof(double) * mm);
     if (mm == 0) return;
-    Engine e(p, 1, sx, snx, sw, rts, rps, *ichart, sxbd_chart, rv_chart, false);
+    Engine& e = panel_engine(p, *ichart, sx, snx, sw, rts, rps, sxbd_chart, rv_chart);
     e.set_targets(mm, tx, false);
     std::vector<int64_t> offs = {0, mm}, idx(mm);
     for (int64_t j = 0; j < mm; ++j) idx[j] = j + 1;
@@ -450,8 +497,8 @@ void qol_lap3dsdlpmat_levels_mex_(const int64_t* m, const double* tx, const int6
     const int64_t h = g_slot.npols, mm = *m;
     // unit normals/weights are not needed for the level path; pass the tangents
     std::vector<double> snx(3 * h, 0.0), sw(h, 1.0);
-    Engine e(g_slot.order, 1, g_slot.sx.data(), snx.data(), sw.data(), g_slot.rts.data(), g_slot.rps.data(), 0,
-             nullptr, nullptr, false);
+    Engine& e = panel_engine(g_slot.order, 0, g_slot.sx.data(), snx.data(), sw.data(), g_slot.rts.data(),
+                             g_slot.rps.data(), nullptr, nullptr);
     e.set_targets(mm, tx, false);
     std::vector<int64_t> offs = {0, mm}, idx(mm);
     for (int64_t j = 0; j < mm; ++j) idx[j] = j + 1;

@@ -6,6 +6,7 @@
 // weights.  Every stage runs on the engine's HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -27,15 +28,53 @@ void set_status(int64_t s);
 void set_error(const std::string& s);
 
 
+void add_timeinfo(const double* ms, double* timeinfo) {
+  timeinfo[0] += 1e-3 * ms[0];
+  timeinfo[1] += 1e-3 * ms[3];
+  timeinfo[2] += 1e-3 * (ms[1] + ms[2]);
+  timeinfo[3] += 1e-3 * ms[6];
+  timeinfo[4] += 1e-3 * (ms[5] + ms[8]);
+  timeinfo[5] += 1e-3 * (ms[7] + ms[9]);
+  timeinfo[7] += 1e-3 * ms[4];
+}
+
+Arena*& thread_arena() {
+  static thread_local Arena* a = nullptr;
+  return a;
+}
+
 DevBuf::~DevBuf() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr && owned) (void)hipFree(ptr);
 }
 void DevBuf::alloc(size_t b) {
   if (b <= bytes && ptr) return;
-  if (ptr) { (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
+  if (ptr && owned) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+  owned = false;
   if (b == 0) return;
+  if (Arena* a = thread_arena()) {
+    const size_t off = (a->used + 255) & ~(size_t)255;
+    if (off > a->cap || b > a->cap - off)
+      throw std::runtime_error("device scratch exhausted: " + std::to_string(off + b) + " bytes needed, " +
+                               std::to_string(a->cap) + " available");
+    ptr = a->base + off;
+    bytes = b;
+    a->used = off + b;
+    return;
+  }
   HIPCHK(hipMalloc(&ptr, b));
   bytes = b;
+  owned = true;
+}
+
+void StreamSet::create_one_stream() {
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  s1 = s2 = s3 = stream;
+  for (auto& st : tev)
+    for (auto& e : st) HIPCHK(hipEventCreate(&e));
+  for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge})
+    HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
 }
 
 namespace {
@@ -56,14 +95,10 @@ void set_lds(const void* f, size_t bytes) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-DeviceTables::DeviceTables(i64 p, hipStream_t s) {
-  const OrderTables& T = order_tables(p);
-  h = (int)T.h;
-  ntot = (int)T.ntot;
-  nsubtot = (int)T.nsubtot;
-  std::vector<double> interp((size_t)T.h * T.ntot), wt(T.ntot);
-  std::vector<int> suboff, subtyp;
-  LA.h = h; LA.ntot = ntot; LA.nsubtot = nsubtot;
+// the four levels' offsets and sizes in the concatenated node and sub-patch
+// arrays, and the concatenated suboff / subtyp tables
+static void level_layout(const OrderTables& T, LevelArgs& LA, std::vector<int>& suboff, std::vector<int>& subtyp) {
+  LA.h = (int)T.h; LA.ntot = (int)T.ntot; LA.nsubtot = (int)T.nsubtot;
   int off = 0, sb = 0;
   for (int l = 0; l < 4; ++l) {
     const LevelTable& L = T.lev[l];
@@ -71,12 +106,26 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
     LA.lev_off[l] = off;
     LA.lev_nsub[l] = (int)L.nsub;
     LA.sub_base[l] = sb;
-    std::memcpy(interp.data() + (size_t)T.h * off, L.interpmat.data(), sizeof(double) * L.n * T.h);
-    std::memcpy(wt.data() + off, L.wt.data(), sizeof(double) * L.n);
     for (i64 q = 0; q <= L.nsub; ++q) suboff.push_back((int)L.suboff[q]);
     for (i64 q = 0; q < L.nsub; ++q) subtyp.push_back((int)L.subtyp[q]);
     off += (int)L.n;
     sb += (int)L.nsub;
+  }
+}
+
+DeviceTables::DeviceTables(i64 p, hipStream_t s) {
+  const OrderTables& T = order_tables(p);
+  h = (int)T.h;
+  ntot = (int)T.ntot;
+  nsubtot = (int)T.nsubtot;
+  std::vector<double> interp((size_t)T.h * T.ntot), wt(T.ntot);
+  std::vector<int> suboff, subtyp;
+  level_layout(T, LA, suboff, subtyp);
+  for (int l = 0; l < 4; ++l) {
+    const LevelTable& L = T.lev[l];
+    const int off = LA.lev_off[l];
+    std::memcpy(interp.data() + (size_t)T.h * off, L.interpmat.data(), sizeof(double) * L.n * T.h);
+    std::memcpy(wt.data() + off, L.wt.data(), sizeof(double) * L.n);
   }
   // row-major copy padded to whole 16-column MFMA tiles
   const int pad = 16 * ((h + 15) / 16);
@@ -88,6 +137,9 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
   }
   upload(d_imp, imp.data(), imp.size(), s);
   LA.imp = (const double*)d_imp.ptr;
+  const std::vector<float> impf(imp.begin(), imp.end());
+  upload(d_impf, impf.data(), impf.size(), s);
+  LA.impf = (const float*)d_impf.ptr;
   LA.pad = pad;
   upload(d_interp, interp.data(), interp.size(), s);
   upload(d_wt, wt.data(), wt.size(), s);
@@ -116,35 +168,63 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
   const std::vector<double> kt = koorn_eval_table((int)T.p);
   upload(d_ktab, kt.data(), kt.size(), s);
   RT.ktab = (const double*)d_ktab.ptr;
+  // the geometry's interpolant at the corners (0,0), (1,0), (0,1), as the level
+  // rows are formed: Koornwinder values times umatr
+  std::vector<double> corner(3 * (size_t)T.h), pol(T.h);
+  const double cuv[3][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}};
+  for (int c = 0; c < 3; ++c) {
+    koornwinder(cuv[c][0], cuv[c][1], T.p - 1, pol.data(), nullptr, nullptr);
+    for (i64 j = 0; j < T.h; ++j) {
+      double a = 0.0;
+      for (i64 q = 0; q < T.h; ++q) a += pol[q] * T.umatr[q + T.h * j];
+      corner[(size_t)c * T.h + j] = a;
+    }
+  }
+  upload(d_corner, corner.data(), corner.size(), s);
   HIPCHK(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------
 Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const double* sw, const double* rts,
-               const double* rps, i64 ichart_, const double* sxbd, const double* rv, bool device_ptrs)
+               const double* rps, i64 ichart_, const double* sxbd, const double* rv, bool device_ptrs,
+               const StreamSet* borrow)
     : p((int)p_), npan((int)npan_), ichart((int)ichart_) {
   if (p < 2 || p > QOL_MAXP) throw std::runtime_error("patch order out of range for the device kernels");
   if (npan <= 0) throw std::runtime_error("npan must be positive");
-  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  // QOL_ONE_STREAM=1 runs every stage on the main stream (kernel times without
-  // overlap, for profiling); by default the stages overlap on four streams
-  const char* one = std::getenv("QOL_ONE_STREAM");
-  if (one && one[0] == '1') {
-    s1 = s2 = s3 = stream;
+  if (borrow) {
+    borrowed = true;
+    stream = borrow->stream; s1 = borrow->s1; s2 = borrow->s2; s3 = borrow->s3;
+    for (int i = 0; i < 10; ++i)
+      for (int j = 0; j < 2; ++j) tev[i][j] = borrow->tev[i][j];
+    ev_start = borrow->ev_start; ev_lists = borrow->ev_lists; ev_direct = borrow->ev_direct;
+    ev_setup = borrow->ev_setup; ev_system = borrow->ev_system; ev_solve = borrow->ev_solve; ev_edge = borrow->ev_edge;
   } else {
-    HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // QOL_ONE_STREAM=1 runs every stage on the main stream (kernel times without
+    // overlap, for profiling); by default the stages overlap on four streams
+    const char* one = std::getenv("QOL_ONE_STREAM");
+    if (one && one[0] == '1') {
+      s1 = s2 = s3 = stream;
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+      HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+      HIPCHK(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+    }
+    for (auto& st : tev)
+      for (auto& e : st) HIPCHK(hipEventCreate(&e));
+    for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge})
+      HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
-  for (auto& st : tev)
-    for (auto& e : st) HIPCHK(hipEventCreate(&e));
-  for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge})
-    HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   tab = std::make_unique<DeviceTables>(p, stream);
   h = tab->h;
   nq = p + 2;
   ncoef = (p + 1) * (p + 2) / 2;
   nsrc = (size_t)npan * h;
+  set_surface(sx, snx, sw, rts, rps, sxbd, rv, device_ptrs);
+}
+
+void Engine::set_surface(const double* sx, const double* snx, const double* sw, const double* rts, const double* rps,
+                         const double* sxbd, const double* rv, bool device_ptrs) {
   const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   auto put = [&](DevBuf& d, const double* a, size_t n) {
     d.alloc(n * sizeof(double));
@@ -161,9 +241,18 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
     put(d_rv, rv, (size_t)npan * 9);
   }
   HIPCHK(hipStreamSynchronize(stream));
+  // everything derived from the old surface is stale
+  levels_built = false;
+  evaluated = false;
+  have_omega = false;
+  density_set = false;
+  val_current = false;
+  nnz = -1;
+  tlists_nnz = -1;
 }
 
 Engine::~Engine() {
+  if (borrowed) return;
   for (auto& st : tev)
     for (auto& e : st) (void)hipEventDestroy(e);
   for (hipEvent_t e : {ev_start, ev_lists, ev_direct, ev_setup, ev_system, ev_solve, ev_edge}) (void)hipEventDestroy(e);
@@ -171,6 +260,42 @@ Engine::~Engine() {
   if (s2 && s2 != stream) (void)hipStreamDestroy(s2);
   if (s3 && s3 != stream) (void)hipStreamDestroy(s3);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+size_t Engine::one_panel_bytes(i64 p, i64 m, i64 ichart) {
+  const OrderTables& T = order_tables(p);
+  LevelArgs LA{};
+  std::vector<int> suboff, subtyp;
+  level_layout(T, LA, suboff, subtyp);
+  const size_t h = (size_t)T.h, ntot = (size_t)T.ntot, nsubtot = (size_t)T.nsubtot, nq = (size_t)p + 2;
+  const size_t nb = 3 * nq, ncoef = (size_t)(p + 1) * (p + 2) / 2, pad = 16 * ((h + 15) / 16);
+  const size_t mm = (size_t)(m > 0 ? m : 1), D = sizeof(double), I = sizeof(int);
+  size_t tot = 0;
+  auto add = [&](size_t b) { tot += (b + 255) & ~(size_t)255; };
+  // per-order tables (DeviceTables)
+  for (size_t b : {pad * ntot * D, pad * ntot * sizeof(float), h * ntot * D, ntot * D, suboff.size() * I, subtyp.size() * I,
+                   level_chunks(LA, suboff.data()).size() * I, T.tgl.size() * D, T.wgl.size() * D,
+                   T.Legmat.size() * D, T.umatr.size() * D, T.uvn.size() * D, koorn_eval_table((int)p).size() * D,
+                   3 * h * D})
+    add(b);
+  // the panel (constructor), targets and pairs
+  for (size_t b : {3 * h * D, 3 * h * D, h * D, 3 * h * D, 3 * h * D}) add(b);
+  if (ichart == 1) { add(9 * nq * D); add(9 * D); }
+  add(3 * mm * D);                  // tx
+  add(2 * I); add(mm * I);          // offs, tidx
+  // alloc_pairs
+  for (size_t b : {mm, mm * I, 2 * h * D * mm, mm * D, (size_t)I, 5 * (size_t)I, 5 * (size_t)I, 8 * (size_t)I, mm * I})
+    add(b);
+  // alloc_rrq
+  for (size_t b : {16 * D, 9 * nq * D, 6 * nb * D, 3 * h * D, 12 * nb * ncoef * D, 3 * h * h * D, h * h * D,
+                   h * h * D, 10 * h * h * D, h * h * D})
+    add(b);
+  if (!rrq_solve_g1_in_lds((int)h)) add(4 * h * h * D);
+  // eval: Omega, Ias, the per-pair RRQ data, the level geometry
+  for (size_t b : {4 * h * mm * D, mm * D, 4 * mm * D, 3 * nq * mm * D, 3 * mm * D, 9 * mm * D,
+                   rrq_edge_order_ints((int64_t)mm) * I, 5 * h * mm * D, 8 * ntot * D, 4 * nsubtot * D})
+    add(b);
+  return tot;
 }
 
 void Engine::set_targets(i64 nt_, const double* tx, bool device_ptr) {
@@ -200,10 +325,21 @@ void Engine::alloc_pairs() {
 
 // classification and per-label pair lists (main stream)
 void Engine::classify_lists() {
-  const size_t lds_b = sizeof(double) * 4 * (size_t)tab->nsubtot;
-  hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, (const int*)d_offs.ptr,
-                     (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_balls.ptr,
+  SelectorArgs S = sel;
+  S.corner = (const double*)tab->d_corner.ptr;
+  const size_t lds_b = classify_smem(tab->LA, S);
+  set_lds((const void*)classify_kernel, lds_b);
+  hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, S, (const int*)d_offs.ptr,
+                     (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr,
+                     (const double*)d_snx.ptr, (const double*)d_sw.ptr, (const double*)d_balls.ptr,
                      (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr, (int*)d_lcount.ptr);
+  // pairs the selector serves with the panel's own nodes: naive weights, and a
+  // zero correction when eval makes the corrections itself
+  if (S.mode == 2 || (S.mode == 1 && S.forced == 0))
+    hipLaunchKernelGGL(naive_fill_kernel, dim3(blocks(nnz, 256)), dim3(256), 0, stream, (int)nnz, h,
+                       (const int8_t*)d_label.ptr, (const int*)d_tidx.ptr, (const int*)d_ppanel.ptr,
+                       (const double*)d_tx.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
+                       (const double*)d_sw.ptr, (double*)d_wout.ptr, density_set ? (double*)d_val.ptr : nullptr);
   hipLaunchKernelGGL(level_scan_kernel, dim3(5), dim3(1024), 0, stream, (const int*)d_lcount.ptr, npan,
                      (int*)d_lstart.ptr, (int*)d_ltot.ptr);
   hipLaunchKernelGGL(compact_kernel, dim3(npan), dim3(256), 0, stream, (const int*)d_offs.ptr,
@@ -211,11 +347,16 @@ void Engine::classify_lists() {
                      (int*)d_lpairs.ptr);
 }
 
-// direct S/D of every level-served pair
+// direct S/D of every level-served pair: levels 1..split_levels on the FP32
+// matrix cores (split precision), the rest on the FP64 matrix cores
 void Engine::direct(hipStream_t s) {
+  const int ns = std::max(0, std::min(4, split_levels));
+  launch_direct_split(tab->LA, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr,
+                      nnz, s, 0, ns - 1);
   launch_direct_mfma(tab->LA, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr,
-                     nnz, s);
+                     nnz, s, ns, 3);
 }
 
 i64 Engine::nearlist() {
@@ -274,9 +415,13 @@ void Engine::get_pairs(i64* offs, i64* idx1) {
   for (i64 j = 0; j < nnz; ++j) idx1[j] = (i64)t[j] + 1;
 }
 
-void Engine::build_levels() {
+void Engine::alloc_levels() {
   d_lgeo.alloc(sizeof(double) * 8 * (size_t)tab->ntot * npan);
   d_balls.alloc(sizeof(double) * 4 * (size_t)tab->nsubtot * npan);
+}
+
+void Engine::build_levels() {
+  alloc_levels();
   launch_level_geom(tab->LA, npan, (const double*)d_sx.ptr, (const double*)d_rts.ptr, (const double*)d_rps.ptr,
                     (double*)d_lgeo.ptr, (double*)d_balls.ptr, stream);
   HIPCHK(hipGetLastError());
@@ -356,6 +501,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   d_root.alloc(sizeof(double) * 9 * (size_t)nnz);
   d_eorder.alloc(sizeof(int) * rrq_edge_order_ints(nnz));
   d_vt.alloc(sizeof(double) * 5 * (size_t)h * nnz);
+  alloc_levels();  // every buffer is in place before the first launch
   HIPCHK(hipEventRecord(ev_start, stream));
   // s2: RRQ set-up and solve for every panel
   HIPCHK(hipStreamWaitEvent(s2, ev_start, 0));
@@ -451,6 +597,22 @@ void Engine::stage_ms(double* ms) {
   } else {
     ms[8] = 0.0;
   }
+}
+
+void Engine::pair_features(float* feat) {
+  if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
+  build_levels();
+  if (nnz > 0) {
+    DevBuf d_feat;
+    d_feat.alloc(sizeof(float) * kSelIn * (size_t)nnz);
+    hipLaunchKernelGGL(pair_features_kernel, dim3(npan), dim3(256), sizeof(double) * 3 * (size_t)h, stream, tab->LA,
+                       (const double*)tab->d_corner.ptr, (const int*)d_offs.ptr, (const int*)d_tidx.ptr,
+                       (const double*)d_tx.ptr, (const double*)d_sx.ptr, (const double*)d_snx.ptr,
+                       (const double*)d_sw.ptr, (const double*)d_balls.ptr, (float*)d_feat.ptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(feat, d_feat.ptr, sizeof(float) * kSelIn * (size_t)nnz, hipMemcpyDeviceToHost, stream));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
 }
 
 void Engine::get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias) {

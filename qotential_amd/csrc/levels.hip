@@ -301,44 +301,6 @@ void launch_level_geom(const LevelArgs& A, int npan, const double* sx, const dou
   hipLaunchKernelGGL(level_geom_kernel, grid, blk, lds, s, A, npan, sx, rts, rps, lgeo, balls);
 }
 
-// Classify every (panel, target) pair: the first level whose sub-patch balls all
-// exclude the target, else RRQ (label 5).  One workgroup per panel.  Also writes
-// the pair's panel id and the per-panel count of RRQ pairs.
-__global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, const int* __restrict__ offs,
-                                                       const int* __restrict__ tidx, const double* __restrict__ tx,
-                                                       const double* __restrict__ balls, int8_t* __restrict__ label,
-                                                       int* __restrict__ ppanel, int* __restrict__ nrrq,
-                                                       int* __restrict__ lcount) {
-  extern __shared__ double sb[];
-  __shared__ int cnt[5];
-  const int k = blockIdx.x;
-  const double* B = balls + (size_t)k * 4 * A.nsubtot;
-  for (int i = threadIdx.x; i < 4 * A.nsubtot; i += blockDim.x) sb[i] = B[i];
-  if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int j0 = offs[k], j1 = offs[k + 1];
-  for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
-    const int t = tidx[j];
-    const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
-    int lab = 5;
-    for (int l = 0; l < 4 && lab == 5; ++l) {
-      bool outside = true;
-      for (int s = A.sub_base[l]; s < A.sub_base[l] + A.lev_nsub[l]; ++s) {
-        const double d0 = sb[4 * s] - x0, d1 = sb[4 * s + 1] - x1, d2 = sb[4 * s + 2] - x2;
-        if (d0 * d0 + d1 * d1 + d2 * d2 < sb[4 * s + 3]) { outside = false; break; }
-      }
-      if (outside) lab = l + 1;
-    }
-    label[j] = (int8_t)lab;
-    ppanel[j] = k;
-    atomicAdd(&cnt[lab - 1], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) nrrq[k] = cnt[4];
-  if (threadIdx.x < 5) lcount[5 * k + threadIdx.x] = cnt[threadIdx.x];
-}
-
-
 // Per-label pair lists, stable (panel order, then pair order inside a panel):
 // lstart[l][k] = first slot of panel k's label-(l+1) pairs inside list l,
 // ltot[l] = length of list l.  One block per label.
@@ -421,14 +383,14 @@ __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int*
                                                          const int* __restrict__ tidx,
                                                          const double* __restrict__ tx,
                                                          const double* __restrict__ lgeo,
-                                                         double* __restrict__ wout) {
+                                                         double* __restrict__ wout, int lo, int hi) {
   int b = blockIdx.x, l = -1;
   int start[4];
   {
     int s = 0;
     for (int q = 0; q < 4; ++q) { start[q] = s; s += ltot[q]; }
   }
-  for (int q = 3; q >= 0; --q) {
+  for (int q = hi; q >= lo; --q) {
     const int ntile = (ltot[q] + 15) >> 4;
     if (b < ntile) { l = q; break; }
     b -= ntile;
@@ -514,18 +476,20 @@ __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int*
 }
 
 void launch_direct_mfma(LevelArgs A, const int* lpairs, const int* ltot, const int* ppanel, const int* tidx,
-                        const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s) {
+                        const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s, int lo,
+                        int hi) {
+  if (lo > hi) return;
   const int grid = (int)((nnz + 15) / 16) + 4;
   switch ((A.h + 15) / 16) {
-    case 1: hipLaunchKernelGGL(direct_mfma_kernel<1>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 2: hipLaunchKernelGGL(direct_mfma_kernel<2>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 3: hipLaunchKernelGGL(direct_mfma_kernel<3>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 4: hipLaunchKernelGGL(direct_mfma_kernel<4>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 5: hipLaunchKernelGGL(direct_mfma_kernel<5>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 6: hipLaunchKernelGGL(direct_mfma_kernel<6>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 7: hipLaunchKernelGGL(direct_mfma_kernel<7>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 8: hipLaunchKernelGGL(direct_mfma_kernel<8>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
-    case 9: hipLaunchKernelGGL(direct_mfma_kernel<9>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout); break;
+    case 1: hipLaunchKernelGGL(direct_mfma_kernel<1>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 2: hipLaunchKernelGGL(direct_mfma_kernel<2>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 3: hipLaunchKernelGGL(direct_mfma_kernel<3>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 4: hipLaunchKernelGGL(direct_mfma_kernel<4>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 5: hipLaunchKernelGGL(direct_mfma_kernel<5>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 6: hipLaunchKernelGGL(direct_mfma_kernel<6>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 7: hipLaunchKernelGGL(direct_mfma_kernel<7>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 8: hipLaunchKernelGGL(direct_mfma_kernel<8>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
+    case 9: hipLaunchKernelGGL(direct_mfma_kernel<9>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
     default: throw std::runtime_error("direct level kernel: unsupported patch order");
   }
 }

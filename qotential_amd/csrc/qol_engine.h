@@ -19,9 +19,19 @@
 
 namespace qol {
 
+// Where device buffers come from: hipMalloc by default; while the calling
+// thread has an arena set, DevBuf::alloc takes 256-byte aligned slices of it
+// instead and the buffer is never freed (the arena's owner keeps the memory).
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+Arena*& thread_arena();
+
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  bool owned = false;  // from hipMalloc (freed here) or from an arena (not)
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -33,8 +43,19 @@ struct DeviceTables {
   int h = 0, ntot = 0, nsubtot = 0;
   LevelArgs LA{};
   RRQTables RT{};
-  DevBuf d_interp, d_imp, d_wt, d_suboff, d_subtyp, d_chunks, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn, d_ktab;
+  DevBuf d_interp, d_imp, d_impf, d_wt, d_suboff, d_subtyp, d_chunks, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn, d_ktab;
+  DevBuf d_corner;  // 3 x h rows interpolating a panel at its corners (selector features)
   DeviceTables(i64 p, hipStream_t s);
+};
+
+// Streams and events an engine borrows instead of creating (and destroying) its
+// own; all four stream roles may be the same stream.
+struct StreamSet {
+  hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
+  hipEvent_t tev[10][2] = {};
+  hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_system = nullptr,
+             ev_solve = nullptr, ev_edge = nullptr;
+  void create_one_stream();  // one non-blocking stream in every role, events for every stage
 };
 
 struct Engine {
@@ -43,9 +64,12 @@ struct Engine {
   int nt = -1;
   i64 nnz = -1;
   bool levels_built = false, evaluated = false, have_omega = false, rrq_alloc = false;
+  bool borrowed = false;      // streams and events belong to a StreamSet
   bool density_set = false;   // ub, ubn resident (eval then also makes the corrections)
   bool val_current = false;   // d_val holds the corrections of the last eval
   i64 tlists_nnz = -1;
+  int split_levels = 0;       // levels 1..split_levels use the split-precision (FP32) direct kernel
+  SelectorArgs sel{};         // how pairs are labelled: ball rule, forced label, or learned selector
   hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
   hipEvent_t tev[10][2] = {};
   hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_system = nullptr, ev_solve = nullptr,
@@ -60,13 +84,23 @@ struct Engine {
   RRQPanels RP{};
 
   Engine(i64 p, i64 npan, const double* sx, const double* snx, const double* sw, const double* rts,
-         const double* rps, i64 ichart, const double* sxbd, const double* rv, bool device_ptrs);
+         const double* rps, i64 ichart, const double* sxbd, const double* rv, bool device_ptrs,
+         const StreamSet* borrow = nullptr);
+  // Device bytes one engine of one panel takes for m targets (constructor,
+  // set_targets, set_pairs and eval with Omega), each buffer rounded up to 256
+  // bytes as an arena slices it.
+  static size_t one_panel_bytes(i64 p, i64 m, i64 ichart);
+  // new panel data of the same order, panel count and ichart; targets, pairs
+  // and everything evaluated are dropped
+  void set_surface(const double* sx, const double* snx, const double* sw, const double* rts, const double* rps,
+                   const double* sxbd, const double* rv, bool device_ptrs);
   ~Engine();
   void set_targets(i64 nt, const double* tx, bool device_ptr);
   i64 nearlist();
   void set_pairs(const i64* offs, const i64* idx1);
   void get_pairs(i64* offs, i64* idx1);
   void build_levels();
+  void alloc_levels();
   void alloc_pairs();
   void alloc_rrq();
   void eval(int exterior, double side_tol, bool want_omega);
@@ -75,6 +109,7 @@ struct Engine {
   void direct(hipStream_t s);
   void stage_ms(double* ms);
   void get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias);
+  void pair_features(float* feat);
   void apply(const double* ub, const double* ubn, double* u, double* val, bool device_ptrs);
   void apply_device();
   void launch_correction(int first, int last, hipStream_t s);
@@ -84,5 +119,16 @@ struct Engine {
   void level_data(double* lgeo, double* balls);
   void triplets(const i64* trow, i64* si, i64* sj, double* vs, double* vd);
 };
+
+// engine stage times (ms: levels, classify, direct, rrq setup, rrq solve,
+// projection, root + SSQ, moments+recursion, solid angle, weights) added into
+// the reference's timeinfo slots (seconds, accumulated; lap3d_close_mod.f90:
+// 714-871): 1 levels, 2 boundary precompute, 3 level direct, 4 SSQ, 5 solid
+// angle (with the projection that orients it), 6 moments and weights, 8 solve
+void add_timeinfo(const double* ms, double* timeinfo);
+
+// the selector arguments of a model made by qol_selector_create (mode 2), or
+// the ball rule (mode 0) for a null model
+SelectorArgs selector_args(void* mlp, double threshold);
 
 }  // namespace qol

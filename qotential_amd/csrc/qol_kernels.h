@@ -14,6 +14,7 @@ struct LevelArgs {
   int lev_n[4], lev_off[4], lev_nsub[4], sub_base[4];
   const double* interp;  // level l at interp + h*lev_off[l], column-major lev_n[l] x h
   const double* imp;     // level l at imp + pad*lev_off[l], row-major lev_n[l] x pad, zero-padded
+  const float* impf;     // imp rounded to FP32 (the split-precision direct kernel's B operand)
   int pad;               // 16 * ceil(h / 16)
   const double* wt;      // ntot
   const int* suboff;     // per level (nsub+1) node offsets, concatenated
@@ -57,14 +58,53 @@ std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host);
 int level_geom_blocks(int npan);
 void launch_level_geom(const LevelArgs& A, int npan, const double* sx, const double* rts, const double* rps,
                        double* lgeo, double* balls, hipStream_t s);
-__global__ void classify_kernel(LevelArgs A, const int* offs, const int* tidx, const double* tx,
-                                const double* balls, int8_t* label, int* ppanel, int* nrrq, int* lcount);
+// Learned level selector: an MLP 10 -> 64 -> 64 -> 7 (ReLU) on per-pair geometry
+// features, BF16 weights with FP32 biases and accumulation.  Classes: 0 the
+// panel's own nodes suffice (no correction), 1..4 direct on level 1..4,
+// 5 RRQ, 6 direct on level 4 although the ball rule asks for RRQ.
+constexpr int kSelIn = 10, kSelHid = 64, kSelOut = 7;
+constexpr int kSelW1 = kSelHid * kSelIn, kSelW2 = kSelHid * kSelHid, kSelW3 = kSelOut * kSelHid;
+constexpr int kSelNW = kSelW1 + kSelW2 + kSelW3;  // weights: W1 (64x10), W2 (64x64), W3 (7x64), row-major
+constexpr int kSelNB = 2 * kSelHid + kSelOut;     // biases b1, b2, b3
+constexpr int kSelNP = kSelNW + kSelNB;
+struct SelectorArgs {
+  int mode = 0;             // 0 ball rule, 1 every pair gets `forced`, 2 learned (rule below threshold),
+                            // 3 ball rule with level pairs served `forced` levels finer (at most level 4)
+  int forced = 0;           // label 0..5 for mode 1, level shift for mode 3
+  float threshold = 0.9f;   // softmax confidence below which the ball rule decides
+  const uint16_t* w = nullptr;   // kSelNW BF16 weights
+  const float* b = nullptr;      // kSelNB biases
+  const float* norm = nullptr;   // feature mean (10) then 1/std (10)
+  const double* corner = nullptr;  // 3 x h rows interpolating the panel at its corners
+};
+// the label a selector class serves: 0 none, 1..4 levels, 5 RRQ (class 6 -> level 4)
+__host__ __device__ inline int selector_label(int cls) { return cls == 6 ? 4 : cls; }
+
+__global__ void classify_kernel(LevelArgs A, SelectorArgs S, const int* offs, const int* tidx, const double* tx,
+                                const double* sx, const double* snx, const double* sw, const double* balls,
+                                int8_t* label, int* ppanel, int* nrrq, int* lcount);
+size_t classify_smem(const LevelArgs& A, const SelectorArgs& S);
+// per-pair selector features (raw, before normalisation), feat[nnz][10]
+__global__ void pair_features_kernel(LevelArgs A, const double* corner, const int* offs, const int* tidx,
+                                     const double* tx, const double* sx, const double* snx, const double* sw,
+                                     const double* balls, float* feat);
+// naive weights for the pairs labelled 0 (their correction is then exactly 0)
+__global__ void naive_fill_kernel(int nnz, int h, const int8_t* label, const int* tidx, const int* ppanel,
+                                  const double* tx, const double* sx, const double* snx, const double* sw,
+                                  double* wout, double* val);
 
 __global__ void level_scan_kernel(const int* lcount, int npan, int* lstart, int* ltot);
 __global__ void compact_kernel(const int* offs, const int8_t* label, const int* lstart, const int* ltot, int npan,
                                int* lpairs);
+// direct S/D of the pairs of levels lo..hi (0-based, inclusive) on FP64 matrix cores
 void launch_direct_mfma(LevelArgs A, const int* lpairs, const int* ltot, const int* ppanel, const int* tidx,
-                        const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s);
+                        const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s, int lo = 0,
+                        int hi = 3);
+// the same on FP32 matrix cores: x - y in FP64, kernel entries in FP32, FP32
+// accumulation over chunks of 16 level nodes added into FP64
+void launch_direct_split(LevelArgs A, const int* lpairs, const int* ltot, const int* ppanel, const int* tidx,
+                         const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s, int lo,
+                         int hi);
 
 __global__ void nearlist_prep_kernel(int npan, int h, const double* sx, const double* sw, double* ball);
 __global__ void nearlist_count_kernel(int npan, int nt, const double* tx, const double* ball, int* count);

@@ -155,6 +155,27 @@ int64_t qol_geo_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t n
   }
 }
 
+int64_t qol_geo_plan_charts(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
+                            int64_t iterate_aspect, double restol, int64_t cap, double* charts) {
+  if (restol > 0.0) return qol_geo_charts(mp, np, p, nfp, nmode, mnrz, restol, cap, charts);
+  try {
+    Surface s = make_surface(nfp, nmode, mnrz);
+    GeometryPlan P;
+    plan_geometry(s, mp, np, p, iterate_aspect != 0, P);
+    if (cap > 0 && (int64_t)P.charts.size() > cap) return -1;
+    if (charts)
+      for (size_t k = 0; k < P.charts.size(); ++k) {
+        const Chart& c = P.charts[k];
+        const double v[6] = {c.a1, c.b1, c.a2, c.b2, c.jj, c.ii};
+        std::memcpy(charts + 6 * k, v, sizeof(v));
+      }
+    return (int64_t)P.charts.size();
+  } catch (const std::exception& e) {
+    qol::set_error(e.what());
+    return -2;
+  }
+}
+
 double qol_geo_curvature_tail(int64_t mp, int64_t np, int64_t p, int64_t nfp, int64_t nmode, const double* mnrz,
                               const double* chart) {
   try {

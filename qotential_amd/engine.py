@@ -99,6 +99,30 @@ class Engine:
         check(lib().qol_engine_eval(self._h, i64(1 if exterior else 0), dbl(side_tol), i64(1 if want_omega else 0)),
               "qol_engine_eval")
 
+    def set_precision(self, split_levels):
+        """levels 1..split_levels on the split-precision (FP32 matrix core) direct kernel; 0 = all FP64"""
+        check(lib().qol_engine_set_precision(self._h, i64(split_levels)), "qol_engine_set_precision")
+
+    def set_selector(self, selector, threshold=0.9):
+        """label pairs with a learned selector (qotential_amd.selector.Selector; None: the ball rule)"""
+        self._sel = selector
+        check(lib().qol_engine_set_selector(self._h, selector.handle if selector is not None else None,
+                                            dbl(threshold)), "qol_engine_set_selector")
+
+    def force_label(self, label):
+        """every pair served one way: 0 own nodes, 1..4 level, 5 RRQ; -1 the ball rule"""
+        check(lib().qol_engine_force_label(self._h, i64(label)), "qol_engine_force_label")
+
+    def refine_levels(self, shift):
+        """the ball rule with level pairs served `shift` levels finer (at most level 4)"""
+        check(lib().qol_engine_refine_levels(self._h, i64(shift)), "qol_engine_refine_levels")
+
+    def pair_features(self):
+        """the selector's 10 features of every pair, (nnz, 10) float32"""
+        f = np.zeros((max(self.nnz, 1), 10), dtype=np.float32)
+        check(lib().qol_engine_pair_features(self._h, ptr(f)), "qol_engine_pair_features")
+        return f[: self.nnz]
+
     def sync(self):
         check(lib().qol_engine_sync(self._h), "qol_engine_sync")
 

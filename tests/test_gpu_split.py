@@ -1,0 +1,87 @@
+"""Split precision (FP32 matrix cores for the direct level weights, x - y in
+FP64, chunked FP32 accumulation added into FP64) against the FP64 path and
+the oracle.  The error is stated per potential: the single-layer part S[du/dn]
+and the double-layer part D[u] of the near correction, relative to the largest
+exact potential."""
+import numpy as np
+import pytest
+
+from qotential_amd import workload as W
+from qotential_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def case():
+    geo = W.geometry(16, 40, 8, modes=W.VMEC_BENCH)
+    tx, tri, dist = W.targets(geo, 20000, -6.0, -1.0, sides=0, seed=5)
+    pos, q = W.point_charges(geo, nchg=8, offset=0.4, seed=2)
+    ub, gu = W.charge_field(geo["sx"], pos, q)
+    ubn = (geo["snx"] * gu).sum(1)
+    uex, _ = W.charge_field(tx, pos, q)
+    return geo, tx, dist, ub, ubn, uex
+
+
+def _run(geo, tx, ub, ubn, split):
+    e = Engine(geo)
+    e.set_targets(tx)
+    e.nearlist()
+    e.set_precision(split)
+    e.eval(exterior=True, side_tol=1e-10)
+    w = e.weights()
+    z = np.zeros_like(ub)
+    uS, _ = e.apply(z, ubn, np.zeros(len(tx)))   # S[ubn] corrections
+    uD, _ = e.apply(ub, z, np.zeros(len(tx)))    # -D[ub] corrections
+    e.close()
+    return w, uS, uD
+
+
+def test_split_levels_vs_fp64_and_oracle(case, oracle):
+    geo, tx, dist, ub, ubn, uex = case
+    w64, uS64, uD64 = _run(geo, tx, ub, ubn, 0)
+    w32, uS32, uD32 = _run(geo, tx, ub, ubn, 4)
+    lab = w64["label"]
+    assert np.array_equal(lab, w32["label"])
+    lv = lab <= 4
+    rr = lab == 5
+    # RRQ pairs are untouched; level pairs agree to FP32 accuracy, row by row
+    for key in ("As", "Ad"):
+        assert np.array_equal(w64[key][rr], w32[key][rr])
+        g, r = w32[key][lv], w64[key][lv]
+        rel = np.abs(g - r).max(1) / np.abs(r).max(1)
+        say(f"{key}: level-pair weights, max rel diff per row {rel.max():.2e}, median {np.median(rel):.2e}")
+        assert rel.max() < 5e-5 and np.median(rel) < 5e-7
+    sc = np.abs(uex).max()
+    eS = np.abs(uS32 - uS64).max() / sc
+    eD = np.abs(uD32 - uD64).max() / sc
+    say(f"per potential vs FP64: S[du/dn] {eS:.2e}, D[u] {eD:.2e} (of max |u|)")
+    assert eS < 2e-8 and eD < 2e-8
+    # the FP64 path against the oracle on sampled panels, so both errors are
+    # also errors against the oracle
+    e = Engine(geo)
+    e.set_targets(tx)
+    e.nearlist()
+    offs, idx = e.pairs()
+    e.close()
+    h = geo["h"]
+    oracle.set_side_tol(1e-10)
+    try:
+        for k in (10, 500, 1200):
+            a, b = offs[k], offs[k + 1]
+            if a == b:
+                continue
+            sl = slice(k * h, (k + 1) * h)
+            r = oracle.rrq(8, tx[idx[a:b] - 1], geo["sx"][sl], geo["snx"][sl], geo["sw"][sl], geo["rts"][sl],
+                           geo["rps"][sl], exterior=True)
+            m = r["label"] <= 4
+            for key in ("As", "Ad"):
+                g = w32[key][a:b][m]
+                rel = np.abs(g - r[key][m]).max(1) / np.abs(r[key][m]).max(1)
+                assert rel.max() < 5e-5, (k, key, rel.max())
+    finally:
+        oracle.set_side_tol(1e300)
