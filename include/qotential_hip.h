@@ -125,6 +125,23 @@ void qol_rrq_mex_(const int64_t* m, const double* tx, const int64_t* n, const do
                   const double* tgl, const double* wgl, const double* Dgl, const double* w_bclag,
                   const double* Legmat, const double* umatr, const double* vmatr, double* As,
                   double* Ad, double* Omega, double* IalphaAsvestas, double* timeinfo);
+/* qol_rrq_noalloc_: no reference counterpart.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the arguments, outputs and error reporting of
+ * qol_rrq_mex_, with every device buffer of the call taken from a fixed 64 MiB
+ * device array per GPU (set up on the first call on that device) and no
+ * hipMalloc / hipFree per call.  Calls needing more than the array -- about
+ * 16,000 targets at order 8, 7,900 at order 12 -- fail with qol_last_status() = 1
+ * and a message, before any device work.  Calls on one GPU share the array and
+ * are serialised by the library. */
+void qol_rrq_noalloc_(const int64_t* m, const double* tx, const int64_t* n, const double* sx,
+                      const double* snx, const double* sw, const double* rts, const double* rps,
+                      const int64_t* order, const int64_t* nquad, const int64_t* orderff,
+                      const double* distff, const int64_t* exterior, const int64_t* isimd,
+                      const int64_t* ichart, const double* sxbd_chart, const double* rv_chart,
+                      const double* tgl, const double* wgl, const double* Dgl, const double* w_bclag,
+                      const double* Legmat, const double* umatr, const double* vmatr, double* As,
+                      double* Ad, double* Omega, double* IalphaAsvestas, double* timeinfo);
 /* qol_getnearquad_lap_rrq_mex_ replaces qol_getnearquad_lap_rrq_mex ->
  * getnearquad_lap_rrq_r64 (src/lap3d_close_mex.f90:62-81,
  * src/lap3d_close_mod.f90:884-960): COO triplets of (S - S_naive, K - K_naive),
@@ -138,6 +155,17 @@ void qol_getnearquad_lap_rrq_mex_(const int64_t* npan, const int64_t* nterms, co
                                   const int64_t* iside, const int64_t* isimd, const int64_t* nnz,
                                   int64_t* sparsei, int64_t* sparsej, double* sparsevs,
                                   double* sparsevd, double* timeinfo);
+/* qol_lap3ddlp_closepanel_mex_ replaces qol_lap3ddlp_closepanel_mex ->
+ * Lap3dDLP_closepanel_r64 (src/lap3d_close_mex.f90:2-15, src/lap3d_close_mod.f90:
+ * 135-270): the close double-layer matrix Ac(m_tgt, npat) of one tensor-product
+ * patch, npat = order^2 nodes on the Gauss-Legendre tensor grid (node b*order+a,
+ * a the fast first-parameter index): sum_ip Ac(k, ip) sigma_ip is the DLP of the
+ * interpolated density at t_x(:, k), kernel (x - y).n/(4 pi |x - y|^3), n along
+ * y_u x y_v.  ref raises the leaf quadrature order (2 order + 2 ref); if_adapt_d
+ * must be nonzero (the reference stops for the non-adaptive form).  order <= 16. */
+void qol_lap3ddlp_closepanel_mex_(const int64_t* m_tgt, const double* t_x, const int64_t* npat,
+                                  const double* s_x, const int64_t* order, const int64_t* ref,
+                                  const double* if_adapt_d, double* Ac);
 /* handle-based refinement levels, replacing src/patch_refine_mex.f90:198-242
  * (qol_build_patch_levels_mex, qol_lap3dsdlpmat_levels_mex,
  * qol_clear_patch_levels_mex, qol_clear_patch_levels_all_mex).  One slot,
@@ -188,6 +216,8 @@ int qol_engine_apply(void* engine, const double* ub, const double* ubn, double* 
 int qol_engine_apply_resident(void* engine);
 /* copy back the per-pair corrections val(nnz) and the resident u(nt) (no compute) */
 int qol_engine_get_values(void* engine, double* val, double* u);
+/* the resident potential u(nt) that the per-target sums add into (no compute) */
+int qol_engine_set_potential(void* engine, const double* u);
 /* far field by direct summation (stellarator_grf_core_mod.f90:560-611) */
 int qol_engine_far(void* engine, const double* ub, const double* ubn, double* u);
 int qol_engine_level_data(void* engine, double* lgeo, double* balls);
@@ -238,6 +268,27 @@ int qol_engine_force_label(void* engine, int64_t label);
 int qol_engine_refine_levels(void* engine, int64_t shift);
 /* the selector's 10 features of every pair, feat(10, nnz) */
 int qol_engine_pair_features(void* engine, float* feat);
+
+/* ---------------- mixed-order batches ----------------
+ * No single reference counterpart: the reference evaluates a mixed-order
+ * surface panel by panel with each panel's own order (lap3d_close_mod.f90:
+ * 622-649 in the loop of stellarator_grf_core_mod.f90:759-888).  A batch holds
+ * one engine per order (each already given targets, pairs and a resident
+ * density, with one eager eval + apply done); qol_batch_capture records one
+ * step of all of them -- eval, the near corrections and the per-target sums --
+ * into one HIP graph, and qol_batch_launch replays it as one launch.  The
+ * engines must outlive the batch and keep their sizes.
+ * qol_batch_autotune times, per engine, the four- and one-stream schedules and
+ * (orders up to 8) the three RRQ solve variants, keeps the fastest, then the
+ * engines' order in the graph; ms(6 n + 2) and choice(2 n + 1) report the
+ * candidates' ms per launch (-1 where not applicable) and the choices
+ * (one-stream flag, solve variant per engine; 1 = descending order). */
+int qol_batch_create(int64_t n, void** engines, void** batch);
+int qol_batch_destroy(void* batch);
+int qol_batch_capture(void* batch, int64_t exterior, double side_tol);
+int qol_batch_launch(void* batch);
+int qol_batch_sync(void* batch);
+int qol_batch_autotune(void* batch, int64_t reps, double* ms, int64_t* choice);
 
 /* No reference counterpart: one plain device-to-device copy of nbytes (a
  * multiple of 16) through hbm_copy_kernel, on buffers it allocates and frees,

@@ -43,8 +43,11 @@ def simplex_precomp(nquad, korder):
 
 
 def rrq(tx, sx, snx, sw, rts, rps, order, exterior=True, ichart=0, sxbd_chart=None, rv_chart=None, pre=None,
-        timeinfo=None):
-    """As, Ad (m x n), Omega (4h x m), IalphaAsvestas (m) for m targets against one panel."""
+        timeinfo=None, noalloc=False):
+    """As, Ad (m x n), Omega (4h x m), IalphaAsvestas (m) for m targets against one panel.
+
+    noalloc=True calls qol_rrq_noalloc_ (no device allocation per call: the
+    library's fixed device scratch, which bounds m)."""
     tx = _f(tx)
     m = tx.shape[0]
     n = order * (order + 1) // 2
@@ -55,12 +58,13 @@ def rrq(tx, sx, snx, sw, rts, rps, order, exterior=True, ichart=0, sxbd_chart=No
     ti = np.zeros(20) if timeinfo is None else timeinfo
     sxbd = np.zeros((3 * nq, 3)) if sxbd_chart is None else _f(sxbd_chart)
     rv = np.zeros((3, 3)) if rv_chart is None else _f(rv_chart)
-    lib().qol_rrq_mex_(_r(m), ptr(tx), _r(n), ptr(_f(sx)), ptr(_f(snx)), ptr(_f(sw)), ptr(_f(rts)), ptr(_f(rps)),
+    fn = lib().qol_rrq_noalloc_ if noalloc else lib().qol_rrq_mex_
+    fn(_r(m), ptr(tx), _r(n), ptr(_f(sx)), ptr(_f(snx)), ptr(_f(sw)), ptr(_f(rts)), ptr(_f(rps)),
                        _r(order), _r(nq), _r(order + 2), C.byref(dbl(1.4)), _r(1 if exterior else 0), _r(0),
                        _r(ichart), ptr(sxbd), ptr(rv), *[ptr(pre[k]) for k in
                        ("tgl", "wgl", "Dgl", "w_bclag", "Legmat", "umatr", "vmatr")],
                        ptr(As), ptr(Ad), ptr(Om), ptr(Ia), ptr(ti))
-    _status("qol_rrq_mex")
+    _status("qol_rrq_noalloc" if noalloc else "qol_rrq_mex")
     return dict(As=As, Ad=Ad, Omega=Om, Ias=Ia, timeinfo=ti)
 
 
@@ -80,6 +84,17 @@ def getnearquad_lap_rrq(geo, tcx, tcxrow, tcxi, iside=30):
                                        ptr(sj), ptr(vs), ptr(vd), ptr(ti))
     _status("qol_getnearquad_lap_rrq_mex")
     return si[:nnz], sj[:nnz], vs[:nnz], vd[:nnz]
+
+
+def lap3ddlp_closepanel(tx, sx, order, ref=1, if_adapt=True):
+    """close DLP matrix Ac (m x order^2) of a tensor-product patch (qol_lap3ddlp_closepanel_mex)"""
+    tx = _f(tx)
+    m = tx.shape[0]
+    Ac = np.zeros((m, order * order), order="F")
+    lib().qol_lap3ddlp_closepanel_mex_(_r(m), ptr(tx), _r(order * order), ptr(_f(sx)), _r(order), _r(ref),
+                                       C.byref(dbl(1.0 if if_adapt else 0.0)), ptr(Ac))
+    _status("qol_lap3ddlp_closepanel_mex")
+    return Ac
 
 
 def build_patch_levels(sx, rts, rps, korder, dist=1.4, nlevel=4):
