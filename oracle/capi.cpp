@@ -277,6 +277,10 @@ void orc_close_apply(i64 npan, i64 nterms, i64 hdim, i64 nquad, const double* sx
       for (i64 j = offs[k]; j < offs[k + 1]; ++j) u[idx[j] - 1] = u[idx[j] - 1] + csr_val[j];
 }
 
+void orc_closepanel_dlp(i64 m, const double* tx, i64 p, const double* sx, i64 ref, double* Ac) {
+  closepanel_dlp(m, tx, p, sx, ref, Ac);
+}
+
 /* far field by direct summation with the coincident term dropped:
    u(t) = sum_j As(t,j) ubn_j - Ad(t,j) ub_j */
 void orc_far_direct(i64 nt, const double* tx, i64 ns, const double* sx, const double* snx,

@@ -197,3 +197,11 @@ def getnearquad(order, geo, tcx, tcxrow, tcxi, iside=30):
                           _p(np.ascontiguousarray(tcxi, dtype=np.int64)), i64(order + 2), dbl(1.4), i64(iside),
                           i64(nnz), _p(si), _p(sj), _p(vs), _p(vd), _p(ti))
     return si, sj, vs, vd
+
+
+def closepanel_dlp(tx, sx, order, ref=1):
+    """close DLP matrix Ac (m, order^2) of a tensor-product patch (closepanel.cpp)"""
+    tx = _f(tx); m = tx.shape[0]
+    Ac = np.zeros((m, order * order), order="F")
+    lib().orc_closepanel_dlp(i64(m), _p(tx), i64(order), _p(_f(sx)), i64(ref), _p(Ac))
+    return Ac

@@ -83,6 +83,9 @@ void lap3dsdlpmat(i64 m, const double* tx, i64 n, const double* sx, const double
 /* the ball test of one level: x inside some sub-patch ball (strict <) */
 bool inside_any_ball(const PanelLevel& L, const double* x);
 
+/* ---- tensor-patch close DLP (closepanel.cpp): Ac(m, p^2), column-major ---- */
+void closepanel_dlp(i64 m, const double* tx, i64 p, const double* sx, i64 ref, double* Ac);
+
 /* ---- RRQ (rrq.cpp) ---- */
 struct RRQStats {
   double t[20] = {0};

@@ -149,6 +149,16 @@ int qol_engine_get_values(void* h, double* val, double* u) {
   return guarded([&] { E(h)->get_values(val, u); });
 }
 
+int qol_engine_set_potential(void* h, const double* u) {
+  return guarded([&] {
+    Engine* e = E(h);
+    if (e->nt <= 0) return;
+    e->d_u.alloc(sizeof(double) * (size_t)e->nt);
+    HIPCHK(hipMemcpyAsync(e->d_u.ptr, u, sizeof(double) * e->nt, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+  });
+}
+
 int qol_engine_far(void* h, const double* ub, const double* ubn, double* u) {
   return guarded([&] { E(h)->far(ub, ubn, u); });
 }

@@ -73,7 +73,7 @@ void StreamSet::create_one_stream() {
   s1 = s2 = s3 = stream;
   for (auto& st : tev)
     for (auto& e : st) HIPCHK(hipEventCreate(&e));
-  for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge})
+  for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge, &ev_proj})
     HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
 }
 
@@ -198,6 +198,7 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
       for (int j = 0; j < 2; ++j) tev[i][j] = borrow->tev[i][j];
     ev_start = borrow->ev_start; ev_lists = borrow->ev_lists; ev_direct = borrow->ev_direct;
     ev_setup = borrow->ev_setup; ev_system = borrow->ev_system; ev_solve = borrow->ev_solve; ev_edge = borrow->ev_edge;
+    ev_proj = borrow->ev_proj;
   } else {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // QOL_ONE_STREAM=1 runs every stage on the main stream (kernel times without
@@ -212,7 +213,7 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
     }
     for (auto& st : tev)
       for (auto& e : st) HIPCHK(hipEventCreate(&e));
-    for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge})
+    for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge, &ev_proj})
       HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   tab = std::make_unique<DeviceTables>(p, stream);
@@ -255,7 +256,8 @@ Engine::~Engine() {
   if (borrowed) return;
   for (auto& st : tev)
     for (auto& e : st) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {ev_start, ev_lists, ev_direct, ev_setup, ev_system, ev_solve, ev_edge}) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ev_start, ev_lists, ev_direct, ev_setup, ev_system, ev_solve, ev_edge, ev_proj})
+    (void)hipEventDestroy(e);
   if (s1 && s1 != stream) (void)hipStreamDestroy(s1);
   if (s2 && s2 != stream) (void)hipStreamDestroy(s2);
   if (s3 && s3 != stream) (void)hipStreamDestroy(s3);
@@ -474,7 +476,13 @@ void Engine::eval_levels_only() {
 void Engine::eval(int exterior, double side_tol, bool want_omega) {
   if (nnz < 0) throw std::runtime_error("pairs not set (call nearlist or set_pairs)");
   if (p > 12) throw std::runtime_error("the RRQ kernels support patch orders up to 12");
-  auto rec = [&](int i, int j, hipStream_t s) { HIPCHK(hipEventRecord(tev[i][j], s)); };
+  // stage timing events are left out of a captured graph (timing = false)
+  auto rec = [&](int i, int j, hipStream_t s) {
+    if (timing) HIPCHK(hipEventRecord(tev[i][j], s));
+  };
+  // the four stream roles, or the main stream for all of them
+  hipStream_t s1 = one_stream_run ? stream : this->s1, s2 = one_stream_run ? stream : this->s2,
+              s3 = one_stream_run ? stream : this->s3;
   if (nnz <= 0) {
     rec(0, 0, stream);
     build_levels();
@@ -519,7 +527,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   rec(3, 1, s2);
   HIPCHK(hipEventRecord(ev_system, s2));
   rec(4, 0, s2);
-  launch_rrq_panel_solve(tab->RT, RP, npan, nullptr, s2);
+  launch_rrq_panel_solve(tab->RT, RP, npan, nullptr, s2, solve_variant);
   rec(4, 1, s2);
   HIPCHK(hipEventRecord(ev_solve, s2));
   // main: levels, classification, pair lists
@@ -538,9 +546,10 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol,
                      (double*)d_pq.ptr, (int*)d_eorder.ptr, nnz, s3);
   rec(5, 1, s3);
+  HIPCHK(hipEventRecord(ev_proj, s3));
   // s1: direct levels, after the projection: the projection heads the chain the
   // moments wait for, and the direct kernel's 121k waves would crowd it out
-  HIPCHK(hipStreamWaitEvent(s1, tev[5][1], 0));
+  HIPCHK(hipStreamWaitEvent(s1, ev_proj, 0));
   rec(2, 0, s1);
   direct(s1);
   rec(2, 1, s1);
@@ -551,7 +560,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   rec(6, 0, s3);
   launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                   (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
-                  (double*)d_root.ptr, (double*)d_oe.ptr, (int*)d_eorder.ptr, nnz, s3, tev[8][0]);
+                  (double*)d_root.ptr, (double*)d_oe.ptr, (int*)d_eorder.ptr, nnz, s3, timing ? tev[8][0] : nullptr);
   rec(6, 1, s3);
   HIPCHK(hipEventRecord(ev_edge, s3));
   // main: moments once the edges and the basis data are in, weights once the

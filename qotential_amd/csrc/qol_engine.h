@@ -54,7 +54,7 @@ struct StreamSet {
   hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
   hipEvent_t tev[10][2] = {};
   hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_system = nullptr,
-             ev_solve = nullptr, ev_edge = nullptr;
+             ev_solve = nullptr, ev_edge = nullptr, ev_proj = nullptr;
   void create_one_stream();  // one non-blocking stream in every role, events for every stage
 };
 
@@ -70,10 +70,13 @@ struct Engine {
   i64 tlists_nnz = -1;
   int split_levels = 0;       // levels 1..split_levels use the split-precision (FP32) direct kernel
   SelectorArgs sel{};         // how pairs are labelled: ball rule, forced label, or learned selector
+  bool timing = true;         // record the stage timing events (off while a step is captured into a graph)
+  bool one_stream_run = false;  // run every stage on the main stream (a scheduling choice; same results)
+  int solve_variant = 0;      // RRQ solve kernels: 0 default, 1 fused elimination, 2 matrix-core elimination
   hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
   hipEvent_t tev[10][2] = {};
   hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_system = nullptr, ev_solve = nullptr,
-            ev_edge = nullptr;
+            ev_edge = nullptr, ev_proj = nullptr;
   std::unique_ptr<DeviceTables> tab;
   DevBuf d_sx, d_snx, d_sw, d_rts, d_rps, d_sxbd, d_rv;
   DevBuf d_tx, d_nball, d_cnt, d_offs, d_tidx;

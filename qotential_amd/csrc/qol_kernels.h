@@ -136,7 +136,10 @@ __global__ void rrq_panel_frame_kernel(RRQTables T, RRQPanels P, int npan, const
                                        const double* rv_chart);
 __global__ void rrq_panel_system_kernel(RRQTables T, RRQPanels P, int npan, const int* nrrq, const double* sx,
                                         const double* snx);
-void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s);
+// variant 0: register-resident inverses up to order 8 (QOL_SOLVE_FUSED=1 / QOL_SOLVE_MFMA=1
+// select the others), 1: the fused vector-unit elimination, 2: the matrix-core
+// quaternion elimination (orders up to 8; the default above)
+void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s, int variant = 0);
 bool rrq_solve_g1_in_lds(int h);
 void launch_rrq_project(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
                         const int* tidx, const double* tx, const double* sx, int exterior, double side_tol,

@@ -1347,7 +1347,7 @@ __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPa
     }
 }
 
-void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s) {
+void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s, int variant) {
   const int h = T.h;
   const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
   const size_t lds = rrq_solve_smem(h);
@@ -1357,7 +1357,7 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
   // accumulator tiles, B from a separate real inverse), which is slower on the
   // order-8 workload (5.3 vs 4.9 ms for 14,250 panels)
   const char* mf = std::getenv("QOL_SOLVE_MFMA");
-  if (mf && mf[0] == '1') switch (T.p) {
+  if (variant == 2 || (variant == 0 && mf && mf[0] == '1')) switch (T.p) {
 #define QOL_SOLVE_MFMA(P_, RT_, CT_, RPT_)                                                                  \
   case P_:                                                                                                \
     hipLaunchKernelGGL((rrq_msl_inverse_kernel<RPT_, CT_>), dim3(npan), dim3(256), 0, s, T, P, npan, nrrq);  \
@@ -1378,7 +1378,8 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
   // registers (the latter also forms G); QOL_SOLVE_FUSED=1 keeps the single fused
   // kernel
   const char* fu = std::getenv("QOL_SOLVE_FUSED");
-  if (h <= 36 && !(fu && fu[0] == '1')) {
+  const bool fused = variant == 1 || (variant == 0 && fu && fu[0] == '1');
+  if (h <= 36 && !fused) {
     // lane grid per order (rows RL x RA, slots CL x CB)
 #define QOL_QINV(RL_, RA_, CL_, CB_, NW_)                                                                \
   hipLaunchKernelGGL((rrq_real_inverse_kernel<RL_, RA_, CL_, CB_>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); \

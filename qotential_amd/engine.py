@@ -162,6 +162,10 @@ class Engine:
         check(lib().qol_engine_get_values(self._h, ptr(val), ptr(u)), "qol_engine_get_values")
         return val[:self.nnz], u[:self.nt]
 
+    def set_potential(self, u):
+        """the resident potential the per-target sums add into"""
+        check(lib().qol_engine_set_potential(self._h, ptr(_f(u))), "qol_engine_set_potential")
+
     def far(self, ub, ubn):
         u = np.zeros(max(self.nt, 1))
         check(lib().qol_engine_far(self._h, ptr(_f(ub)), ptr(_f(ubn)), ptr(u)), "qol_engine_far")

@@ -50,6 +50,12 @@ module qotential_hip_c
       integer(c_int64_t) :: sparsei(*), sparsej(*)
       real(c_double) :: sx(*), snx(*), sw(*), rts(*), rps(*), tcx(*), distff, sparsevs(*), sparsevd(*), timeinfo(*)
     end subroutine
+    subroutine qol_lap3ddlp_closepanel_mex(m_tgt, t_x, npat, s_x, order, ref, if_adapt_d, Ac) &
+        bind(C, name='qol_lap3ddlp_closepanel_mex_')
+      import :: c_int64_t, c_double
+      integer(c_int64_t) :: m_tgt, npat, order, ref
+      real(c_double) :: t_x(*), s_x(*), if_adapt_d, Ac(*)
+    end subroutine
     function qol_geo_plan_charts(mp, np, p, nfp, nmode, mnrz, iterate_aspect, restol, cap, charts) &
         bind(C, name='qol_geo_plan_charts') result(n)
       import :: c_int64_t, c_double
@@ -97,7 +103,7 @@ module lap3d_close_mod
   use qotential_hip_c
   implicit none
   private
-  public :: rrq_r64, simplex_precomp_r64, getnearquad_lap_rrq_r64
+  public :: rrq_r64, simplex_precomp_r64, getnearquad_lap_rrq_r64, Lap3dDLP_closepanel_r64
 contains
 
   subroutine require_default_levels()
@@ -170,6 +176,21 @@ contains
                                      a8, a9, a10, sparsei, sparsej, sparsevs, sparsevd, timeinfo)
     call qol_check('getnearquad_lap_rrq_r64')
   end subroutine getnearquad_lap_rrq_r64
+
+  ! src/lap3d_close_mod.f90:135-141
+  subroutine Lap3dDLP_closepanel_r64(m_tgt, t_x, npat, s_x, order, ref, if_adapt, Ac)
+    integer(8), intent(in)  :: m_tgt, npat, order, ref
+    real(r64),  intent(in)  :: t_x(3, m_tgt)
+    real(r64),  intent(in)  :: s_x(3, npat)
+    logical,    intent(in)  :: if_adapt
+    real(r64),  intent(out) :: Ac(m_tgt, npat)
+    integer(c_int64_t) :: a1, a2, a3, a4
+    real(c_double) :: fa
+    a1 = m_tgt; a2 = npat; a3 = order; a4 = ref
+    fa = merge(1.0_c_double, 0.0_c_double, if_adapt)
+    call qol_lap3ddlp_closepanel_mex(a1, t_x, a2, s_x, a3, a4, fa, Ac)
+    call qol_check('Lap3dDLP_closepanel_r64')
+  end subroutine Lap3dDLP_closepanel_r64
 
 end module lap3d_close_mod
 

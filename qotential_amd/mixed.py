@@ -11,9 +11,12 @@ panels of several orders gets its pairs from each. The layer potential at a
 target is the sum over the orders, in ascending order (a fixed order, so the result
 is deterministic), of that order's far field plus its close corrections.
 """
+import ctypes as C
+
 import numpy as np
 
 from . import workload as W
+from ._lib import check, dbl, i64, lib
 from .engine import Engine
 
 
@@ -88,3 +91,88 @@ def layer_potential(surf, tx, density, exterior=True, side_tol=1e-10):
         for e in engines.values():
             e.close()
     return u, stats
+
+
+class MixedBatch:
+    """One step of every order of a mixed-order surface as one HIP graph launch.
+
+    Builds one engine per order (targets, near list, far field, resident
+    density, one eager evaluation), then captures eval + corrections + the
+    per-target sums of all orders into a graph (qol_batch_capture).  step()
+    launches it; autotune() picks each order's schedule and solve kernels and
+    the orders' sequence in the graph by timing (qol_batch_autotune).
+    """
+
+    def __init__(self, surf, tx, density, exterior=True, side_tol=1e-10):
+        self.orders = sorted(surf["parts"])
+        self.engines, self.far, self.dens = {}, {}, {}
+        for p in self.orders:
+            g = surf["parts"][p]
+            ub, gu = density(g["sx"])
+            self.dens[p] = (ub, (g["snx"] * gu).sum(1))
+            e = Engine(g)
+            e.set_targets(tx)
+            e.nearlist()
+            self.far[p] = e.far(*self.dens[p])
+            e.eval(exterior=exterior, side_tol=side_tol)
+            e.apply(*self.dens[p], self.far[p])
+            self.engines[p] = e
+        arr = (C.c_void_p * len(self.orders))(*[self.engines[p]._h for p in self.orders])
+        h = C.c_void_p()
+        check(lib().qol_batch_create(i64(len(self.orders)), arr, C.byref(h)), "qol_batch_create")
+        self._h = h
+        check(lib().qol_batch_capture(h, i64(1 if exterior else 0), dbl(side_tol)), "qol_batch_capture")
+
+    def autotune(self, reps=3):
+        n = len(self.orders)
+        ms = np.zeros(6 * n + 2)
+        ch = np.zeros(2 * n + 1, dtype=np.int64)
+        check(lib().qol_batch_autotune(self._h, i64(reps), ms.ctypes.data_as(C.c_void_p),
+                                       ch.ctypes.data_as(C.c_void_p)), "qol_batch_autotune")
+        table = {p: dict(ms=ms[6 * i:6 * i + 6].reshape(2, 3).tolist(), one_stream=int(ch[2 * i]),
+                         solve_variant=int(ch[2 * i + 1])) for i, p in enumerate(self.orders)}
+        table["graph_order_ms"] = dict(ascending=float(ms[6 * n]), descending=float(ms[6 * n + 1]))
+        table["descending"] = bool(ch[2 * n])
+        return table
+
+    def step(self):
+        check(lib().qol_batch_launch(self._h), "qol_batch_launch")
+
+    def sync(self):
+        check(lib().qol_batch_sync(self._h), "qol_batch_sync")
+
+    def values(self):
+        """each order's per-pair corrections of the last step, {p: val}"""
+        self.sync()
+        return {p: self.engines[p].values()[0] for p in self.orders}
+
+    def potential(self):
+        """S[du/dn] - D[u] at the targets from one graph launch: each order's
+        resident potential is reset to its far field, the step adds the
+        corrections, and the orders are summed in ascending order"""
+        self.sync()
+        for p in self.orders:
+            self.engines[p].set_potential(self.far[p])
+        self.step()
+        self.sync()
+        u = np.zeros(self.engines[self.orders[0]].nt)
+        for p in self.orders:
+            u += self.engines[p].values()[1]
+        return u
+
+    def pairs(self):
+        return sum(e.nnz for e in self.engines.values())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qol_batch_destroy(self._h)
+            self._h = None
+        for e in self.engines.values():
+            e.close()
+        self.engines = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

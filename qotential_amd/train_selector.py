@@ -74,6 +74,11 @@ def main(argv=None):
     sel.save(a.out, loss=loss, holdout_accuracy=acc, class_counts=np.bincount(lab, minlength=S.N_CLASS),
              label_tol=S.LABEL_TOL, surfaces=a.surfaces, pairs=len(lab))
     log(f"[selector] wrote {a.out}")
+    # an unseen surface: the ball rule, the rule one level finer, the selector
+    modes, nfp = S.random_stellarator(np.random.default_rng(a.seed + 12345))
+    geo = W.geometry(16, 16 * nfp, 8, modes=modes, nfp=nfp)
+    for name, r in S.ablation(sel, geo).items():
+        log(f"[selector] unseen surface ({geo['ntri']} triangles, p=8) {name}: {r}")
     return 0
 
 

@@ -1,0 +1,45 @@
+"""qol_lap3ddlp_closepanel_mex_ (tensor-patch close DLP, a15) on the GPU
+against the oracle, the flat-square solid angle, and the reference's own
+smoke test test/test_lap3d_close.f90 compiled unchanged against the Fortran
+modules (oracle/_ref/test_lap3d_close, built by oracle/fortran_bench.mk)."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from qotential_amd import rrq as R
+from test_oracle_closepanel import _solid_square, flat_square, torus_patch
+
+pytestmark = pytest.mark.gpu
+BUILT = Path(__file__).resolve().parents[1] / "oracle" / "_ref" / "test_lap3d_close"
+
+
+def test_closepanel_gpu_vs_oracle(oracle):
+    print("tensor-patch close DLP: torus patch order 8 and flat square order 6", flush=True)
+    p = 8
+    sx = torus_patch(oracle, p)
+    rng = np.random.default_rng(2)
+    n = np.array([0.0, 0.0, 1.0])
+    xs = np.vstack([[[0.05 * k, 0.03 * k - 0.05, 0.02 * k + 0.01] for k in range(1, 6)],
+                    sx[rng.integers(0, p * p, 6)] + (10.0 ** rng.uniform(-7, -1, 6) * rng.choice([-1, 1], 6))[:, None] * n])
+    g = R.lap3ddlp_closepanel(xs, sx, p)
+    o = oracle.closepanel_dlp(xs, sx, p)
+    rel = np.abs(g - o).max(1) / np.abs(o).max(1)
+    print("max rel diff per row", rel.max(), flush=True)
+    assert rel.max() < 1e-11
+    ps, a = 6, 0.1
+    sq = flat_square(oracle, ps, a)
+    xq = np.array([[0.01, 0.02, 0.05], [0.0, 0.0, 1e-6], [0.099, -0.02, 2e-5]])
+    gq = R.lap3ddlp_closepanel(xq, sq, ps)
+    for i, x in enumerate(xq):
+        assert abs(gq[i].sum() + _solid_square(x, a) / (4 * np.pi)) < 2e-11
+    with pytest.raises(Exception, match="if_adapt"):
+        R.lap3ddlp_closepanel(xq, sq, ps, if_adapt=False)
+
+
+@pytest.mark.skipif(not BUILT.exists(), reason="oracle/_ref/test_lap3d_close not built (no reference checkout)")
+def test_reference_smoke_test_runs():
+    r = subprocess.run([str(BUILT)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "PASS" in r.stdout

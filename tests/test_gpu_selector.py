@@ -76,8 +76,9 @@ def test_forced_servings_and_labels(small_set):
     geo, feat, cls, rule = small_set
     say("pairs", len(cls), "classes", np.bincount(cls, minlength=S.N_CLASS).tolist())
     assert np.all(np.isfinite(feat)) and feat.shape == (len(cls), S.N_FEAT)
-    # a class is never finer than the rule (the rule's own serving is within tol)
-    assert np.all(S.serve_rank(cls) <= S.serve_rank(rule))
+    # RRQ pairs are never served by levels 1..3, and level 4 for them is class 6
+    assert not np.isin(cls[rule == 5], (1, 2, 3, 4)).any()
+    assert not (cls[rule <= 4] == 6).any()
     tx, _, _ = W.targets(geo, 3000, -7.0, -0.5, sides=0, seed=21)
     pos, q = W.point_charges(geo, seed=4)
     ub, gu = W.charge_field(geo["sx"], pos, q)
