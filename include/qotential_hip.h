@@ -194,6 +194,11 @@ int qol_engine_destroy(void* engine);
 int qol_engine_set_targets(void* engine, int64_t nt, const double* tx, int64_t device_ptr);
 /* near list by the ball rule (stellarator_grf_core_mod.f90:652-695) */
 int qol_engine_nearlist(void* engine, int64_t* nnz);
+/* the same CSR, bit for bit, through a uniform grid of the targets: each panel
+ * tests only the cells its ball covers (the spatial search of the MATLAB
+ * caller's k-d tree, test/torus_fmm3dbie_rrq_comparison.m:81-99), then sorts
+ * its list into ascending target order */
+int qol_engine_nearlist_grid(void* engine, int64_t* nnz);
 /* caller's CSR by panel: offs(npan+1) 0-based, idx(nnz) 1-based target indices */
 int qol_engine_set_pairs(void* engine, const int64_t* offs, const int64_t* idx);
 int qol_engine_get_pairs(void* engine, int64_t* offs, int64_t* idx);
@@ -276,7 +281,10 @@ int qol_engine_pair_features(void* engine, float* feat);
  * one engine per order (each already given targets, pairs and a resident
  * density, with one eager eval + apply done); qol_batch_capture records one
  * step of all of them -- eval, the near corrections and the per-target sums --
- * into one HIP graph, and qol_batch_launch replays it as one launch.  The
+ * into one HIP graph, and qol_batch_launch replays it as one launch; capture
+ * is switched off for now (its first run ended the process with a host-side
+ * segmentation fault whose cause is not yet found), so qol_batch_capture
+ * returns an error and qol_batch_launch launches the step directly.  The
  * engines must outlive the batch and keep their sizes.
  * qol_batch_autotune times, per engine, the four- and one-stream schedules and
  * (orders up to 8) the three RRQ solve variants, keeps the fastest, then the

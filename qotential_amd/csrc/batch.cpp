@@ -1,7 +1,10 @@
 // Mixed-order batches: the close-evaluation step (eval + corrections + the
 // per-target sums) of several engines -- one per patch order of a mixed-order
-// surface -- captured into one HIP graph, so a step is a single graph launch,
-// and a per-order autotuner over the step's schedule.
+// surface -- launched together from one call, and a per-order autotuner over
+// the step's schedule.  Capturing the step into one HIP graph is built but
+// switched off: the one capture run so far ended its process with a host-side
+// segmentation fault whose cause has not been found, so qol_batch_capture
+// refuses until it is.
 //
 // The reference evaluates a mixed-order surface panel by panel, each rrq_r64
 // call taking that panel's own order (src/lap3d_close_mod.f90:622-649 inside the
@@ -12,8 +15,8 @@
 //
 // Autotuner: for each engine, the candidates are the four-stream and the
 // one-stream schedule and, up to order 8, the three RRQ solve variants; each
-// candidate's single-engine graph is timed over a few launches and the fastest
-// kept; then the order in which the engines enter the graph (ascending or
+// candidate's single-engine step is timed over a few steps and the fastest
+// kept; then the order in which the engines are launched (ascending or
 // descending patch order) is timed the same way.  The schedule choice does
 // not change any result; the solve variants agree to the conditioning of the
 // quaternion system.
@@ -76,7 +79,22 @@ struct Batch {
   void sync_engines() {
     for (Engine* x : eng) HIPCHK(hipStreamSynchronize(x->stream));
   }
+  // one step of the given engines, launched directly: each engine's stages on
+  // its own streams, all joined into the origin stream
+  void step(const std::vector<int>& which) {
+    HIPCHK(hipEventRecord(fork, origin));
+    for (int i : which) {
+      Engine* e = eng[i];
+      HIPCHK(hipStreamWaitEvent(e->stream, fork, 0));
+      e->eval(exterior, side_tol, false);
+      e->apply_device();
+      HIPCHK(hipEventRecord(join[i], e->stream));
+      HIPCHK(hipStreamWaitEvent(origin, join[i], 0));
+    }
+  }
   void capture(const std::vector<int>& which) {
+    throw std::runtime_error("batch graph capture is switched off: its first run ended the process with a host-side "
+                             "segmentation fault whose cause is not yet found");
     release();
     sync_engines();
     HIPCHK(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
@@ -103,14 +121,14 @@ struct Batch {
     HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
   void launch() {
-    if (!exec) throw std::runtime_error("batch: nothing captured");
-    HIPCHK(hipGraphLaunch(exec, origin));
+    if (exec) HIPCHK(hipGraphLaunch(exec, origin));
+    else step(order);
   }
-  // mean ms per launch over reps launches (after one untimed launch)
-  double timed(int reps) {
-    launch();
+  // mean ms per step of the given engines over reps steps (after one untimed step)
+  double timed(const std::vector<int>& which, int reps) {
+    step(which);
     HIPCHK(hipEventRecord(t0, origin));
-    for (int r = 0; r < reps; ++r) launch();
+    for (int r = 0; r < reps; ++r) step(which);
     HIPCHK(hipEventRecord(t1, origin));
     HIPCHK(hipEventSynchronize(t1));
     float ms = 0.0f;
@@ -189,8 +207,7 @@ int qol_batch_autotune(void* h, int64_t reps, double* ms, int64_t* choice) {
           if (v == 0 || e->p <= 8) {
             e->one_stream_run = one != 0;
             e->solve_variant = v;
-            b->capture({i});
-            t = b->timed(r);
+            t = b->timed({i}, r);
             if (t < best) { best = t; bs = one; bv = v; }
           }
           if (ms) ms[(size_t)i * 6 + one * 3 + v] = t;
@@ -204,14 +221,11 @@ int qol_batch_autotune(void* h, int64_t reps, double* ms, int64_t* choice) {
     for (int i = 0; i < n; ++i) up[i] = i;
     std::sort(up.begin(), up.end(), [&](int a, int c) { return b->eng[a]->p < b->eng[c]->p; });
     std::vector<int> down(up.rbegin(), up.rend());
-    b->capture(up);
-    const double tu = b->timed(r);
-    b->capture(down);
-    const double td = b->timed(r);
+    const double tu = b->timed(up, r);
+    const double td = b->timed(down, r);
     b->order = td < tu ? down : up;
     if (choice) choice[2 * n] = td < tu ? 1 : 0;
     if (ms) { ms[(size_t)n * 6] = tu; ms[(size_t)n * 6 + 1] = td; }
-    b->capture(b->order);
   });
 }
 

@@ -111,6 +111,10 @@ int qol_engine_nearlist(void* h, int64_t* nnz) {
   return guarded([&] { *nnz = E(h)->nearlist(); });
 }
 
+int qol_engine_nearlist_grid(void* h, int64_t* nnz) {
+  return guarded([&] { *nnz = E(h)->nearlist_grid(); });
+}
+
 int qol_engine_set_pairs(void* h, const int64_t* offs, const int64_t* idx) {
   return guarded([&] { E(h)->set_pairs(offs, idx); });
 }

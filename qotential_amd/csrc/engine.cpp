@@ -389,6 +389,82 @@ i64 Engine::nearlist() {
   return nnz;
 }
 
+// The same near list through a uniform grid of targets (cells at least as
+// large as the largest panel ball, at most 2^24 cells): bit-identical CSR.
+i64 Engine::nearlist_grid() {
+  if (nt < 0) throw std::runtime_error("targets not set");
+  d_nball.alloc(sizeof(double) * 4 * npan);
+  d_cnt.alloc(sizeof(int) * npan);
+  d_offs.alloc(sizeof(int) * (npan + 1));
+  hipLaunchKernelGGL(nearlist_prep_kernel, dim3(blocks(npan, 256)), dim3(256), 0, stream, npan, h,
+                     (const double*)d_sx.ptr, (const double*)d_sw.ptr, (double*)d_nball.ptr);
+  std::vector<double> bl(4 * (size_t)npan);
+  HIPCHK(hipMemcpyAsync(bl.data(), d_nball.ptr, sizeof(double) * bl.size(), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rmax = 0.0;
+  for (int k = 0; k < npan; ++k) {
+    const double r = std::sqrt(bl[4 * k + 3]);
+    rmax = std::max(rmax, r);
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], bl[4 * k + d] - r);
+      hi[d] = std::max(hi[d], bl[4 * k + d] + r);
+    }
+  }
+  double c = rmax > 0.0 ? rmax : 1.0;
+  GridArgs G{};
+  for (;;) {
+    double cells = 1.0;
+    for (int d = 0; d < 3; ++d) {
+      G.n[d] = std::max(1, (int)std::ceil((hi[d] - lo[d]) / c));
+      cells *= G.n[d];
+    }
+    if (cells <= (double)(1 << 24)) break;
+    c *= 1.26;
+  }
+  for (int d = 0; d < 3; ++d) G.lo[d] = lo[d];
+  G.inv_c = 1.0 / c;
+  const int ncell = G.n[0] * G.n[1] * G.n[2];
+  DevBuf d_cell_of, d_ccount, d_cstart, d_ccur, d_ctarg, d_pcur;
+  d_cell_of.alloc(sizeof(int) * (size_t)std::max(nt, 1));
+  d_ccount.alloc(sizeof(int) * ncell);
+  d_cstart.alloc(sizeof(int) * (ncell + 1));
+  d_ccur.alloc(sizeof(int) * ncell);
+  d_ctarg.alloc(sizeof(int) * (size_t)std::max(nt, 1));
+  d_pcur.alloc(sizeof(int) * npan);
+  HIPCHK(hipMemsetAsync(d_ccount.ptr, 0, sizeof(int) * ncell, stream));
+  HIPCHK(hipMemsetAsync(d_ccur.ptr, 0, sizeof(int) * ncell, stream));
+  HIPCHK(hipMemsetAsync(d_pcur.ptr, 0, sizeof(int) * npan, stream));
+  if (nt > 0) {
+    hipLaunchKernelGGL(grid_bin_kernel, dim3(blocks(nt, 256)), dim3(256), 0, stream, nt, (const double*)d_tx.ptr, G,
+                       (int*)d_cell_of.ptr, (int*)d_ccount.ptr);
+  }
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, (const int*)d_ccount.ptr, ncell, (int*)d_cstart.ptr);
+  if (nt > 0)
+    hipLaunchKernelGGL(grid_scatter_kernel, dim3(blocks(nt, 256)), dim3(256), 0, stream, nt, (const int*)d_cell_of.ptr,
+                       (const int*)d_cstart.ptr, (int*)d_ccur.ptr, (int*)d_ctarg.ptr);
+  hipLaunchKernelGGL(grid_panel_kernel, dim3(npan), dim3(64), 0, stream, npan, (const double*)d_nball.ptr, G,
+                     (const int*)d_cstart.ptr, (const int*)d_ctarg.ptr, (const double*)d_tx.ptr, 0, (int*)d_cnt.ptr,
+                     (const int*)nullptr, (int*)nullptr, (int*)nullptr);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, (const int*)d_cnt.ptr, npan, (int*)d_offs.ptr);
+  int tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, (int*)d_offs.ptr + npan, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  nnz = tot;
+  d_tidx.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(grid_panel_kernel, dim3(npan), dim3(64), 0, stream, npan, (const double*)d_nball.ptr, G,
+                       (const int*)d_cstart.ptr, (const int*)d_ctarg.ptr, (const double*)d_tx.ptr, 1, (int*)nullptr,
+                       (const int*)d_offs.ptr, (int*)d_pcur.ptr, (int*)d_tidx.ptr);
+    hipLaunchKernelGGL(segment_sort_kernel, dim3(npan), dim3(256), 0, stream, npan, (const int*)d_offs.ptr,
+                       (int*)d_tidx.ptr);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  alloc_pairs();
+  evaluated = false;
+  return nnz;
+}
+
 void Engine::set_pairs(const i64* offs, const i64* idx1) {
   std::vector<int> o(npan + 1);
   for (int k = 0; k <= npan; ++k) o[k] = (int)offs[k];

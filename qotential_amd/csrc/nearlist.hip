@@ -120,4 +120,110 @@ __global__ void __launch_bounds__(1024) scan_kernel(const int* __restrict__ in, 
   for (int i = a; i < b; ++i) { out[i] = acc; acc += in[i]; }
 }
 
+
+// ---------------------------------------------------------------------------
+// The same near list through a uniform grid of cells no smaller than the
+// largest panel ball (the spatial search the reference's MATLAB caller does
+// with a k-d tree, test/torus_fmm3dbie_rrq_comparison.m:81-99): targets are
+// binned by cell, each panel tests only the targets of the cells its ball's
+// bounding box covers (at most 3 x 3 x 3), and each panel's list is sorted, so
+// the CSR equals the brute-force one bit for bit (the same in_ball test).
+
+__device__ __forceinline__ int grid_coord(double x, double lo, double inv_c, int n) {
+  const double f = floor((x - lo) * inv_c);
+  return f < 0.0 ? 0 : (f > (double)(n - 1) ? n - 1 : (int)f);
+}
+
+__global__ void grid_bin_kernel(int nt, const double* __restrict__ tx, GridArgs G, int* __restrict__ cell_of,
+                                int* __restrict__ cell_count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int i = grid_coord(tx[3 * (size_t)t], G.lo[0], G.inv_c, G.n[0]);
+  const int j = grid_coord(tx[3 * (size_t)t + 1], G.lo[1], G.inv_c, G.n[1]);
+  const int k = grid_coord(tx[3 * (size_t)t + 2], G.lo[2], G.inv_c, G.n[2]);
+  const int c = (k * G.n[1] + j) * G.n[0] + i;
+  cell_of[t] = c;
+  atomicAdd(&cell_count[c], 1);
+}
+
+__global__ void grid_scatter_kernel(int nt, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                                    int* __restrict__ cursor, int* __restrict__ cell_targets) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int c = cell_of[t];
+  cell_targets[cell_start[c] + atomicAdd(&cursor[c], 1)] = t;
+}
+
+// one wave per panel; fill == 0 counts, fill == 1 writes the hits (unordered)
+__global__ void __launch_bounds__(64) grid_panel_kernel(int npan, const double* __restrict__ ball, GridArgs G,
+                                                        const int* __restrict__ cell_start,
+                                                        const int* __restrict__ cell_targets,
+                                                        const double* __restrict__ tx, int fill,
+                                                        int* __restrict__ count, const int* __restrict__ offs,
+                                                        int* __restrict__ cursor, int* __restrict__ idx) {
+  const int k = blockIdx.x;
+  if (k >= npan) return;
+  const double* b = ball + 4 * (size_t)k;
+  const double r = sqrt(b[3]);
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = grid_coord(b[d] - r, G.lo[d], G.inv_c, G.n[d]);
+    hi[d] = grid_coord(b[d] + r, G.lo[d], G.inv_c, G.n[d]);
+  }
+  int cnt = 0;
+  for (int kk = lo[2]; kk <= hi[2]; ++kk)
+    for (int jj = lo[1]; jj <= hi[1]; ++jj)
+      for (int ii = lo[0]; ii <= hi[0]; ++ii) {
+        const int c = (kk * G.n[1] + jj) * G.n[0] + ii;
+        for (int s = cell_start[c] + threadIdx.x; s < cell_start[c + 1]; s += 64) {
+          const int t = cell_targets[s];
+          if (in_ball(b, tx[3 * (size_t)t], tx[3 * (size_t)t + 1], tx[3 * (size_t)t + 2])) {
+            if (fill) idx[offs[k] + atomicAdd(&cursor[k], 1)] = t;
+            else ++cnt;
+          }
+        }
+      }
+  if (!fill) {
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (threadIdx.x == 0) count[k] = cnt;
+  }
+}
+
+// each panel's list in ascending target order: bitonic sort in LDS up to
+// kSortMax entries, one thread's insertion sort beyond
+constexpr int kSortMax = 4096;
+__global__ void __launch_bounds__(256) segment_sort_kernel(int npan, const int* __restrict__ offs, int* __restrict__ idx) {
+  __shared__ int s[kSortMax];
+  const int k = blockIdx.x;
+  const int a = offs[k], n = offs[k + 1] - a;
+  if (n <= 1) return;
+  if (n > kSortMax) {
+    if (threadIdx.x == 0)
+      for (int i = a + 1; i < a + n; ++i) {
+        const int v = idx[i];
+        int j = i - 1;
+        while (j >= a && idx[j] > v) { idx[j + 1] = idx[j]; --j; }
+        idx[j + 1] = v;
+      }
+    return;
+  }
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) s[i] = i < n ? idx[a + i] : 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= m; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const int x = s[i], y = s[j];
+          if ((x > y) == up) { s[i] = y; s[j] = x; }
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) idx[a + i] = s[i];
+}
+
 }  // namespace qol

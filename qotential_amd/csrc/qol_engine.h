@@ -100,6 +100,7 @@ struct Engine {
   ~Engine();
   void set_targets(i64 nt, const double* tx, bool device_ptr);
   i64 nearlist();
+  i64 nearlist_grid();
   void set_pairs(const i64* offs, const i64* idx1);
   void get_pairs(i64* offs, i64* idx1);
   void build_levels();

@@ -58,6 +58,19 @@ std::vector<int> level_chunks(const LevelArgs& A, const int* suboff_host);
 int level_geom_blocks(int npan);
 void launch_level_geom(const LevelArgs& A, int npan, const double* sx, const double* rts, const double* rps,
                        double* lgeo, double* balls, hipStream_t s);
+// uniform grid over the panel balls for the near-list search
+struct GridArgs {
+  double lo[3];   // grid corner
+  double inv_c;   // 1 / cell side
+  int n[3];       // cells per dimension
+};
+__global__ void grid_bin_kernel(int nt, const double* tx, GridArgs G, int* cell_of, int* cell_count);
+__global__ void grid_scatter_kernel(int nt, const int* cell_of, const int* cell_start, int* cursor, int* cell_targets);
+__global__ void grid_panel_kernel(int npan, const double* ball, GridArgs G, const int* cell_start,
+                                  const int* cell_targets, const double* tx, int fill, int* count, const int* offs,
+                                  int* cursor, int* idx);
+__global__ void segment_sort_kernel(int npan, const int* offs, int* idx);
+
 // Learned level selector: an MLP 10 -> 64 -> 64 -> 7 (ReLU) on per-pair geometry
 // features, BF16 weights with FP32 biases and accumulation.  Classes: 0 the
 // panel's own nodes suffice (no correction), 1..4 direct on level 1..4,

@@ -77,9 +77,12 @@ class Engine:
         self.nt = tx.shape[0]
         check(lib().qol_engine_set_targets(self._h, i64(self.nt), ptr(tx), i64(0)), "qol_engine_set_targets")
 
-    def nearlist(self):
+    def nearlist(self, grid=False):
+        """near list by the reference's ball rule (brute force, or through a
+        uniform grid of the targets: the same CSR)"""
         n = i64(0)
-        check(lib().qol_engine_nearlist(self._h, C.byref(n)), "qol_engine_nearlist")
+        fn = lib().qol_engine_nearlist_grid if grid else lib().qol_engine_nearlist
+        check(fn(self._h, C.byref(n)), "qol_engine_nearlist")
         self.nnz = n.value
         return self.nnz
 

@@ -94,16 +94,19 @@ def layer_potential(surf, tx, density, exterior=True, side_tol=1e-10):
 
 
 class MixedBatch:
-    """One step of every order of a mixed-order surface as one HIP graph launch.
+    """One step of every order of a mixed-order surface from one call.
 
     Builds one engine per order (targets, near list, far field, resident
-    density, one eager evaluation), then captures eval + corrections + the
-    per-target sums of all orders into a graph (qol_batch_capture).  step()
-    launches it; autotune() picks each order's schedule and solve kernels and
-    the orders' sequence in the graph by timing (qol_batch_autotune).
+    density, one eager evaluation) and a batch over them (qol_batch_create).
+    step() launches eval + corrections + the per-target sums of all orders
+    (qol_batch_launch); autotune() picks each order's schedule and solve
+    kernels and the orders' launch sequence by timing (qol_batch_autotune).
+    capture=True would record the step into one HIP graph (qol_batch_capture);
+    the library refuses that for now (its first run ended the process with a
+    host-side segmentation fault whose cause is not yet found).
     """
 
-    def __init__(self, surf, tx, density, exterior=True, side_tol=1e-10):
+    def __init__(self, surf, tx, density, exterior=True, side_tol=1e-10, capture=False):
         self.orders = sorted(surf["parts"])
         self.engines, self.far, self.dens = {}, {}, {}
         for p in self.orders:
@@ -121,7 +124,8 @@ class MixedBatch:
         h = C.c_void_p()
         check(lib().qol_batch_create(i64(len(self.orders)), arr, C.byref(h)), "qol_batch_create")
         self._h = h
-        check(lib().qol_batch_capture(h, i64(1 if exterior else 0), dbl(side_tol)), "qol_batch_capture")
+        if capture:
+            check(lib().qol_batch_capture(h, i64(1 if exterior else 0), dbl(side_tol)), "qol_batch_capture")
 
     def autotune(self, reps=3):
         n = len(self.orders)
@@ -147,7 +151,7 @@ class MixedBatch:
         return {p: self.engines[p].values()[0] for p in self.orders}
 
     def potential(self):
-        """S[du/dn] - D[u] at the targets from one graph launch: each order's
+        """S[du/dn] - D[u] at the targets from one batch step: each order's
         resident potential is reset to its far field, the step adds the
         corrections, and the orders are summed in ascending order"""
         self.sync()

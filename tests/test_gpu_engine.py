@@ -30,6 +30,10 @@ def test_nearlist_bit_exact(eng4, geo4, oracle):
     offs, idx = eng4.pairs()
     o_offs, o_idx = oracle.nearlist(geo4["h"], geo4["sx"], geo4["sw"], geo4["sx"])
     assert np.array_equal(offs, o_offs) and np.array_equal(idx, o_idx)
+    # the uniform-grid search gives the same CSR
+    assert eng4.nearlist(grid=True) == 137259
+    g_offs, g_idx = eng4.pairs()
+    assert np.array_equal(g_offs, o_offs) and np.array_equal(g_idx, o_idx)
 
 
 def test_level_geometry_bit_exact(eng4, geo4, oracle):

@@ -44,6 +44,19 @@ def test_bench_workload_lists_labels_levels_vs_oracle(bench_case, oracle):
     say(f"near list on the GPU: {nnz} pairs; oracle with {NTHREADS} threads")
     o_offs, o_idx = oracle.nearlist(h, geo["sx"], geo["sw"], tx, nthreads=NTHREADS)
     assert np.array_equal(offs, o_offs) and np.array_equal(idx, o_idx)
+    # the uniform-grid search: the same CSR
+    import time
+    e.sync()
+    t0 = time.perf_counter()
+    assert e.nearlist(grid=True) == nnz
+    t_grid = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e.nearlist()
+    t_brute = time.perf_counter() - t0
+    say(f"near list: brute force {t_brute * 1e3:.1f} ms, uniform grid {t_grid * 1e3:.1f} ms (incl. host sync)")
+    e.nearlist(grid=True)
+    g_offs, g_idx = e.pairs()
+    assert np.array_equal(g_offs, o_offs) and np.array_equal(g_idx, o_idx)
     e.eval(exterior=True, side_tol=1e-10)
     lab = e.weights()["label"]
     o_lab = oracle.labels(8, geo, tx, offs, idx, nthreads=NTHREADS)
