@@ -10,16 +10,18 @@ A step is one pass of the close-evaluation path over the resident batch:
 pair labelling, direct S/D on the refinement levels, RRQ for the rest, the
 naive-block subtraction and the fixed-order scatter into the potential (the
 reference's "rrq+fill" region).  Geometry, targets, near list and density are
-in HBM before timing starts.  Two paths are timed on the same workload:
+in HBM before timing starts.  Three paths are timed on the same workload:
 
-  plain  FP64 throughout, every pair labelled by the reference's ball rule;
-  fast   the learned level selector (qotential_amd/selector_model.npz) with the
-         ball rule below confidence 0.9, and split precision (FP32 matrix cores)
-         for the direct weights of levels 1..--split-levels.
+  plain     FP64 throughout, every pair labelled by the reference's ball rule;
+  selector  the learned level selector (qotential_amd/selector_model.npz) with
+            the ball rule below confidence 0.9, FP64;
+  split     the ball rule, with the direct weights of levels 1..--split-levels
+            from the FP32 matrix cores (x - y and (x - y).n in FP64).
 
-`value` is the path named by --path (default fast).  Both paths' Green's
-identity errors (S[du/dn] - D[u] = u inside, 0 outside, on every target) and
-the fast path's potential difference from the plain path are reported.
+`value` is the path named by --path: plain by default, the path that meets the
+metric's condition (GRF error within 1e-9 of plain FP64 -- it is plain FP64).
+Every path's Green's identity error (S[du/dn] - D[u] = u inside, 0 outside, on
+every target) and its potential's difference from the plain path are reported.
 """
 import argparse
 import json
@@ -49,10 +51,11 @@ def parse():
     ap.add_argument("--np", type=int, default=120)
     ap.add_argument("--order", type=int, default=8)
     ap.add_argument("--targets", type=int, default=200000)
-    ap.add_argument("--path", choices=("fast", "plain"), default="fast")
+    ap.add_argument("--path", choices=("plain", "selector", "split"), default="plain")
+    ap.add_argument("--no-compare", action="store_true", help="time the --path path only")
     ap.add_argument("--split-levels", type=int, default=2,
-                    help="fast path: levels 1..N get their direct weights from the FP32 matrix cores")
-    ap.add_argument("--threshold", type=float, default=0.9, help="fast path: selector confidence threshold")
+                    help="split path: levels 1..N get their direct weights from the FP32 matrix cores")
+    ap.add_argument("--threshold", type=float, default=0.9, help="selector path: confidence threshold")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-pairs", type=int, default=300000)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -260,15 +263,19 @@ def main():
 
     sel = Selector.load(MODEL_FILE)
     runs = {}
-    for path in ("plain", "fast"):
+    paths = [a.path] if a.no_compare else ["plain", "selector", "split"]
+    if "plain" not in paths:
+        paths = ["plain"] + paths
+    for path in paths:
         eng = Engine(geo)
         eng.set_targets(tx)
         t0 = time.perf_counter()
         nnz = eng.nearlist()
         t_near = time.perf_counter() - t0
         far = eng.far(ub, ubn)
-        if path == "fast":
+        if path == "selector":
             eng.set_selector(sel, a.threshold)
+        elif path == "split":
             eng.set_precision(a.split_levels)
         eng.eval(exterior=True, side_tol=1e-10)
         u0, _ = eng.apply(ub, ubn, far)  # uploads ub, ubn, u (= far) and checks the path once
@@ -285,16 +292,19 @@ def main():
         print(f"[bench] {path}: near list {nnz} pairs ({t_near * 1e3:.1f} ms incl. host sync); served by own nodes, "
               f"L1..L4, RRQ: {counts.tolist()}; {a.steps} steps {ms:.2f} ms/step, {nnz * a.steps / elapsed:.4g} "
               f"pairs/s", flush=True)
-        if path == "plain":
+        if path != a.path:
             eng.close()
+            r = runs[path]
+            r["eng"] = None
 
     nnz = runs["plain"]["nnz"]
     grf = {k: float(np.abs(r["u"] - uref).max() / scale) for k, r in runs.items()}
-    diff = float(np.abs(runs["fast"]["u"] - runs["plain"]["u"]).max() / scale)
+    diff = {k: float(np.abs(r["u"] - runs["plain"]["u"]).max() / scale) for k, r in runs.items()}
     repeat = {k: float(np.abs(r["u"] - r["u0"]).max() / np.abs(r["u0"]).max()) for k, r in runs.items()}
     value = {k: nnz * a.steps / r["elapsed"] for k, r in runs.items()}
-    print(f"[bench] GRF rel err plain {grf['plain']:.3e}, fast {grf['fast']:.3e}; fast vs plain: max |du| / max |u| "
-          f"= {diff:.2e}; speed-up {value['fast'] / value['plain']:.3f}", flush=True)
+    for k in runs:
+        print(f"[bench] {k}: GRF rel err {grf[k]:.3e}; vs plain max |du| / max |u| = {diff[k]:.2e}; "
+              f"pairs/s {value[k]:.4g} ({value[k] / value['plain']:.3f} of plain)", flush=True)
 
     if a.calib_copy_mib > 0:
         # plain copy kernel of known size for the HBM-traffic counter calibration (not timed)
@@ -308,7 +318,7 @@ def main():
 
     # roofline of the dominant kernel (most algorithmic work) of the measured path
     r = runs[a.path]
-    ns = a.split_levels if a.path == "fast" else 0
+    ns = a.split_levels if a.path == "split" else 0
     kf = kernel_flops(a.order, geo["ntri"], r["counts"][1:5], int(r["counts"][5]), r["rrq_panels"], ns)
     stage_of = {"levels": "levels", "rrq_solve": "rrq_solve", "rrq_moments": "rrq_moments",
                 "rrq_weights": "rrq_weights", "direct_split": "direct", "direct": "direct"}
@@ -336,25 +346,24 @@ def main():
         for k, v in base["reference_configs"].items():
             print(f"[bench] cpu {k}: {v['pairs']} pairs, {v['pairs_per_s']:.4g} pairs/s (1 thread)", flush=True)
 
-    runs["fast"]["eng"].close()
-    fr = runs["fast"]["counts"]
+    runs[a.path]["eng"].close()
     line = {
         "metric": "close-eval target-panel pairs/sec at GRF rel err <=1e-9 vs plain FP64 on GPU",
         "value": value[a.path], "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": 1e3 * runs[a.path]["elapsed"] / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.path == "plain" else f"f64 (levels 1..{a.split_levels} direct weights f32)",
+        "dtype": f"f64 (levels 1..{a.split_levels} direct weights f32)" if a.path == "split" else "f64",
         "data": "synthetic",
         "config": {"workload": f"{a.path} RRQ close evaluation, stellarator {a.mp}x{a.np} charts, order {a.order}, "
                                f"{geo['ntri']} triangles, {a.targets} near-surface targets both sides "
                                f"U[-6,-1], reference ball near list",
                    "path": a.path, "pairs": int(nnz),
                    "served_own_L1_L4_RRQ": {k: r_["counts"].tolist() for k, r_ in runs.items()},
-                   "pairs_per_s": value, "speedup_vs_plain_fp64": value["fast"] / value["plain"],
+                   "pairs_per_s": value, "ms_per_step": {k: 1e3 * r_["elapsed"] / a.steps for k, r_ in runs.items()},
                    "selector": {"model": os.path.relpath(str(MODEL_FILE), ROOT), "threshold": a.threshold},
                    "split_levels": a.split_levels,
                    "stage_ms": {k: r_["stage"] for k, r_ in runs.items()}, "kernels": kernels,
-                   "grf_rel_err": grf, "rel_err_vs_plain_fp64": diff if a.path == "fast" else 0.0,
+                   "grf_rel_err": grf, "rel_err_vs_plain_fp64": diff[a.path], "paths_rel_err_vs_plain_fp64": diff,
                    "repeat_rel_diff": repeat},
         "roofline": roof,
         "cpu_baseline": base,

@@ -329,12 +329,10 @@ void Engine::alloc_pairs() {
 void Engine::classify_lists() {
   SelectorArgs S = sel;
   S.corner = (const double*)tab->d_corner.ptr;
-  const size_t lds_b = classify_smem(tab->LA, S);
-  set_lds((const void*)classify_kernel, lds_b);
-  hipLaunchKernelGGL(classify_kernel, dim3(npan), dim3(256), lds_b, stream, tab->LA, S, (const int*)d_offs.ptr,
-                     (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr,
-                     (const double*)d_snx.ptr, (const double*)d_sw.ptr, (const double*)d_balls.ptr,
-                     (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr, (int*)d_lcount.ptr);
+  launch_classify(tab->LA, S, npan, (const int*)d_offs.ptr, (const int*)d_tidx.ptr, (const double*)d_tx.ptr,
+                  (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,
+                  (const double*)d_balls.ptr, (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr,
+                  (int*)d_lcount.ptr, stream);
   // pairs the selector serves with the panel's own nodes: naive weights, and a
   // zero correction when eval makes the corrections itself
   if (S.mode == 2 || (S.mode == 1 && S.forced == 0))

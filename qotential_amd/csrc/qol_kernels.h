@@ -93,9 +93,9 @@ struct SelectorArgs {
 // the label a selector class serves: 0 none, 1..4 levels, 5 RRQ (class 6 -> level 4)
 __host__ __device__ inline int selector_label(int cls) { return cls == 6 ? 4 : cls; }
 
-__global__ void classify_kernel(LevelArgs A, SelectorArgs S, const int* offs, const int* tidx, const double* tx,
-                                const double* sx, const double* snx, const double* sw, const double* balls,
-                                int8_t* label, int* ppanel, int* nrrq, int* lcount);
+void launch_classify(const LevelArgs& A, const SelectorArgs& S, int npan, const int* offs, const int* tidx,
+                     const double* tx, const double* sx, const double* snx, const double* sw, const double* balls,
+                     int8_t* label, int* ppanel, int* nrrq, int* lcount, hipStream_t s);
 size_t classify_smem(const LevelArgs& A, const SelectorArgs& S);
 // per-pair selector features (raw, before normalisation), feat[nnz][10]
 __global__ void pair_features_kernel(LevelArgs A, const double* corner, const int* offs, const int* tidx,

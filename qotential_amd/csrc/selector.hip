@@ -141,19 +141,26 @@ __device__ __forceinline__ void selector_logits(const uint16_t* __restrict__ W, 
   selector_forward(W, B, xin, h1, h2, out);
 }
 
+void launch_classify(const LevelArgs& A, const SelectorArgs& S, int npan, const int* offs, const int* tidx,
+                     const double* tx, const double* sx, const double* snx, const double* sw, const double* balls,
+                     int8_t* label, int* ppanel, int* nrrq, int* lcount, hipStream_t s);
+
 size_t classify_smem(const LevelArgs& A, const SelectorArgs& S) {
   size_t b = sizeof(double) * 4 * (size_t)A.nsubtot;
   if (S.mode == 2) {
     b += sizeof(double) * 3 * (size_t)A.h;                       // panel nodes
     b += sizeof(float) * (kSelNB + 2 * kSelIn);                    // biases, normalisation
     b += sizeof(uint16_t) * kSelNW;                                // weights
+    b += 64 + sizeof(PanelFeat);                                   // panel features (8-byte aligned)
   }
   return b;
 }
 
 // One workgroup per panel: every pair's label (ball rule, forced, or learned),
 // the pair's panel id, the per-panel counts of labels 1..5 and of RRQ pairs.
-// Label-0 pairs are counted in no list.
+// Label-0 pairs are counted in no list.  LEARNED instantiates the selector's
+// features and network; the ball-rule instantiation carries none of it.
+template <bool LEARNED>
 __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs S, const int* __restrict__ offs,
                                                        const int* __restrict__ tidx, const double* __restrict__ tx,
                                                        const double* __restrict__ sx, const double* __restrict__ snx,
@@ -163,14 +170,18 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
                                                        int* __restrict__ lcount) {
   extern __shared__ double sb[];
   __shared__ int cnt[5];
-  __shared__ PanelFeat pf;
+  // the panel's feature data lives past the network's parameters in dynamic LDS
+  PanelFeat* pf = nullptr;
   const int k = blockIdx.x, h = A.h;
   const double* Bk = balls + (size_t)k * 4 * A.nsubtot;
   for (int i = threadIdx.x; i < 4 * A.nsubtot; i += blockDim.x) sb[i] = Bk[i];
   double* ynod = sb + 4 * A.nsubtot;
   float* fb = (float*)(ynod + 3 * h);
   uint16_t* wl = (uint16_t*)(fb + kSelNB + 2 * kSelIn);
-  if (S.mode == 2) {
+  // past the biases, normalisation and weights, rounded up to whole doubles
+  constexpr int kParamDoubles = (int)((sizeof(float) * (kSelNB + 2 * kSelIn) + sizeof(uint16_t) * kSelNW + 7) / 8);
+  if (LEARNED) pf = (PanelFeat*)(ynod + 3 * h + kParamDoubles);
+  if (LEARNED) {
     for (int i = threadIdx.x; i < 3 * h; i += blockDim.x) ynod[i] = sx[3 * (size_t)k * h + i];
     for (int i = threadIdx.x; i < kSelNB; i += blockDim.x) fb[i] = S.b[i];
     for (int i = threadIdx.x; i < 2 * kSelIn; i += blockDim.x) fb[kSelNB + i] = S.norm[i];
@@ -178,8 +189,8 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
   }
   if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
   __syncthreads();
-  if (S.mode == 2 && threadIdx.x == 0)
-    panel_features(h, ynod, snx + 3 * (size_t)k * h, sw + (size_t)k * h, S.corner, sb + 4 * A.sub_base[0], pf);
+  if (LEARNED && threadIdx.x == 0)
+    panel_features(h, ynod, snx + 3 * (size_t)k * h, sw + (size_t)k * h, S.corner, sb + 4 * A.sub_base[0], pf[0]);
   __syncthreads();
   const int j0 = offs[k], j1 = offs[k + 1];
   for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
@@ -198,9 +209,9 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
       lab = S.forced;
     } else if (S.mode == 3) {
       if (lab <= 4) lab = min(4, lab + S.forced);
-    } else if (S.mode == 2) {
+    } else if (LEARNED) {
       float f[kSelIn], lg[kSelOut];
-      pair_feature_vector(h, pf, ynod, x0, x1, x2, f);
+      pair_feature_vector(h, pf[0], ynod, x0, x1, x2, f);
 #pragma unroll
       for (int i = 0; i < kSelIn; ++i) f[i] = (f[i] - fb[kSelNB + i]) * fb[kSelNB + kSelIn + i];
       selector_logits(wl, fb, f, lg);
@@ -221,6 +232,21 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
   __syncthreads();
   if (threadIdx.x == 0) nrrq[k] = cnt[4];
   if (threadIdx.x < 5) lcount[5 * k + threadIdx.x] = cnt[threadIdx.x];
+}
+
+void launch_classify(const LevelArgs& A, const SelectorArgs& S, int npan, const int* offs, const int* tidx,
+                     const double* tx, const double* sx, const double* snx, const double* sw, const double* balls,
+                     int8_t* label, int* ppanel, int* nrrq, int* lcount, hipStream_t s) {
+  const size_t lds = classify_smem(A, S);
+  if (S.mode == 2) {
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)classify_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(classify_kernel<true>, dim3(npan), dim3(256), lds, s, A, S, offs, tidx, tx, sx, snx, sw, balls,
+                       label, ppanel, nrrq, lcount);
+  } else {
+    hipLaunchKernelGGL(classify_kernel<false>, dim3(npan), dim3(256), lds, s, A, S, offs, tidx, tx, sx, snx, sw,
+                       balls, label, ppanel, nrrq, lcount);
+  }
 }
 
 __global__ void __launch_bounds__(256) pair_features_kernel(LevelArgs A, const double* __restrict__ corner,
