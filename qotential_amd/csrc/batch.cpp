@@ -14,7 +14,8 @@
 // dependencies, so the buckets overlap on the device without host launches.
 //
 // Autotuner: for each engine, the candidates are the four-stream and the
-// one-stream schedule and, up to order 8, the three RRQ solve variants; each
+// one-stream schedule and the RRQ solve variants (register-resident inverses
+// and the fused kernel at every order, the matrix-core form up to order 8); each
 // candidate's single-engine step is timed over a few steps and the fastest
 // kept; then the order in which the engines are launched (ascending or
 // descending patch order) is timed the same way.  The schedule choice does
@@ -204,7 +205,7 @@ int qol_batch_autotune(void* h, int64_t reps, double* ms, int64_t* choice) {
       for (int one = 0; one < 2; ++one)
         for (int v = 0; v < 3; ++v) {
           double t = -1.0;
-          if (v == 0 || e->p <= 8) {
+          if (v < 2 || e->p <= 8) {
             e->one_stream_run = one != 0;
             e->solve_variant = v;
             t = b->timed({i}, r);

@@ -972,6 +972,102 @@ __global__ void __launch_bounds__(256) rrq_msl_inverse_kernel(RRQTables T, RRQPa
     }
 }
 
+// rows already pivoted by a register-resident Gauss-Jordan (h <= 128)
+struct RowMask {
+  unsigned long long lo = 0ull, hi = 0ull;
+  __device__ __forceinline__ bool has(int r) const { return ((r < 64 ? lo : hi) >> (r & 63)) & 1ull; }
+  __device__ __forceinline__ void add(int r) {
+    if (r < 64) lo |= 1ull << r;
+    else hi |= 1ull << (r & 63);
+  }
+  // lowest row not yet pivoted
+  __device__ __forceinline__ int first_free() const { return ~lo ? __builtin_ctzll(~lo) : 64 + __builtin_ctzll(~hi); }
+};
+
+// The epilogue of the quaternion solve, shared by rrq_quat_inverse_kernel with
+// EPI (X in registers) and rrq_quat_epilogue_kernel (X from the Ad rows of G):
+// with
+// B = (Msl^T)^{-1} from rrq_real_inverse_kernel and X = Fq^{-1},
+//   Ad rows  G(h+j, q*h+k) = (X_kj)_q fac
+//   As rows  G(j, q*h+k)   = (fac/alpha) sum_i C(j,i) (X_ki)_q,  C = B Fn^T
+//   W block  G(j, 4h+r)    = (fac/alpha) B(j,r),  G(h+j, 4h+r) = 0
+// G is stored transposed, G(row, c) at c*2h + row.  LDS (3h^2 doubles): C, then
+// Bt and Fn (for C) or one component of X at a time (Xq aliases Bt).
+struct SolveLds {
+  double* Ct;  // Ct[i*h + j] = C(j, i)
+  double* Bt;  // Bt[r*h + j] = B(j, r)
+  double* Fs;  // Fs[i + h*r] = Fn(i, r)
+  double* Xq;  // Xq[i*h + k] = (X_ki)_q
+  __device__ SolveLds(double* dyn, int h)
+      : Ct(dyn), Bt(dyn + (size_t)h * h), Fs(dyn + (size_t)2 * h * h), Xq(dyn + (size_t)h * h) {}
+};
+
+// C = B Fn^T into LDS and the W block of G; ends with C visible to every thread
+// once the caller's next barrier has passed
+__device__ __forceinline__ void solve_epilogue_c(const RRQPanels& P, int k, int h, const SolveLds& L, double fa,
+                                                 double* G, int tid, int nth) {
+  const int m = 2 * h;
+  {
+    const double* Bk = P.Binv + (size_t)k * h * h;
+    const double* FN = P.Fn + (size_t)k * h * h;
+    for (int e = tid; e < h * h; e += nth) {
+      L.Bt[(size_t)(e % h) * h + e / h] = Bk[e];
+      L.Fs[e] = FN[e];
+    }
+  }
+  __syncthreads();
+  const int nig = (h + 3) / 4;
+  for (int it = tid; it < h * nig; it += nth) {
+    const int j = it % h, i0 = 4 * (it / h);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r = 0; r < h; ++r) {
+      const double bv = L.Bt[(size_t)r * h + j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = fma(bv, L.Fs[(i0 + u < h ? i0 + u : h - 1) + (size_t)h * r], acc[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u < h) L.Ct[(size_t)(i0 + u) * h + j] = acc[u];
+  }
+  for (int e = tid; e < h * h; e += nth) {
+    const int j = e % h, r = e / h;
+    double* w = G + (size_t)(4 * h + r) * m;
+    w[j] = L.Bt[(size_t)r * h + j] * fa;
+    w[h + j] = 0.0;
+  }
+}
+
+// As rows of component q from C and Xq in LDS: tiles of 3 rows j x 4 columns k
+__device__ __forceinline__ void solve_epilogue_as(int h, int q, const SolveLds& L, double fa, double* G, int tid,
+                                                  int nth) {
+  const int m = 2 * h;
+  const int njg = (h + 2) / 3, nkg = (h + 3) / 4;
+  for (int it = tid; it < njg * nkg; it += nth) {
+    const int j0 = 3 * (it % njg), k0 = 4 * (it / njg);
+    double acc[3][4];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[v][u] = 0.0;
+    for (int i = 0; i < h; ++i) {
+      double cv[3], xv[4];
+#pragma unroll
+      for (int v = 0; v < 3; ++v) cv[v] = L.Ct[(size_t)i * h + (j0 + v < h ? j0 + v : h - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[u] = L.Xq[(size_t)i * h + (k0 + u < h ? k0 + u : h - 1)];
+#pragma unroll
+      for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[v][u] = fma(cv[v], xv[u], acc[v][u]);
+    }
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if ((j0 + v < h) & (k0 + u < h)) G[(size_t)(q * h + k0 + u) * m + j0 + v] = acc[v][u] * fa;
+  }
+}
+
 // X = Fq^{-1} for the h x h pure-quaternion matrix Fq by Gauss-Jordan, the
 // matrix in registers of NW waves (one panel per workgroup).  Lane (ly, lx) of
 // an RL x CL lane grid (RL*CL <= 64; lanes beyond it idle) owns rows ly + RL*a
@@ -987,11 +1083,14 @@ __global__ void __launch_bounds__(256) rrq_msl_inverse_kernel(RRQTables T, RRQPa
 // pivot afterwards).  Each wave already holds its part of the pivot row, so one
 // workgroup barrier per step suffices (LDS double-buffered by step parity).
 // Afterwards row piv[k] holds row k of X with slot s holding column piv[s].
-// Output: the Ad rows of the weights operator, G(h+j, q*h+k) = (X_kj)_q * fac.
-template <int RL, int RA, int CL, int CB, int NW>
+// EPI: the kernel also forms G (solve_epilogue_*, 3h^2 doubles of dynamic LDS);
+// otherwise it writes X itself into the Ad rows of G, G(h+j, q*h+k) = (X_kj)_q,
+// and rrq_quat_epilogue_kernel finishes G, so that the elimination needs no
+// dynamic LDS and more panels share a CU.
+template <int RL, int RA, int CL, int CB, int NW, bool EPI>
 __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, RRQPanels P, int npan,
                                                                    const int* __restrict__ nrrq) {
-  static_assert(RL * CL <= 64 && RL * RA <= 64 && CB % NW == 0, "lane grid");
+  static_assert(RL * CL <= 64 && RL * RA <= 128 && CB % NW == 0, "lane grid");
   constexpr int CW = CB / NW;  // slot groups per wave
   const int k = blockIdx.x;
   if (nrrq && nrrq[k] == 0) return;  // null: every panel
@@ -1016,7 +1115,7 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
         e[1] = q[0]; e[2] = q[1]; e[3] = q[2];
       }
     }
-  unsigned long long used = 0ull;  // rows already pivoted (h <= 64)
+  RowMask used;
   for (int kk = 0; kk < h; ++kk) {
     const int buf = kk & 1, kt = kk % CL, kbg = kk / CL;
     const int ow = kbg / CW, kb = kbg % CW;  // wave and slot group holding column kk
@@ -1034,7 +1133,7 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
           const int r = ly + RL * a;
           const double* e = A[a][b];
           const double v = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
-          const bool t = own & (r < h) & !((used >> (r & 63)) & 1ull) & (v > bv);
+          const bool t = own & (r < h) & !used.has(r) & (v > bv);
           bv = t ? v : bv;
           bi = t ? r : bi;
         }
@@ -1055,7 +1154,7 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
       p = __builtin_amdgcn_readfirstlane(p);
       // no candidate only with NaN entries: take the lowest unused row, so every
       // row is pivoted once and every index stays in range
-      if (p >= h) p = __builtin_ctzll(~used);
+      if (p >= h) p = used.first_free();
       const int pa = p / RL;
       const bool prow = act & (ly == p % RL);
       if (lane == 0) pivs[buf] = p;
@@ -1084,7 +1183,7 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
     }
     __syncthreads();
     const int p = __builtin_amdgcn_readfirstlane(pivs[buf]);
-    used |= 1ull << p;
+    used.add(p);
     if (threadIdx.x == 0) piv[kk] = p;
     const int pa = p / RL;
     const bool prow = act & (ly == p % RL);
@@ -1117,7 +1216,7 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
     for (int b = 0; b < CW; ++b) qmul_fma(pinv, rq + 4 * (lx + CL * (wv * CW + b)), nq[b]);
 #pragma unroll
     for (int a = 0; a < RA; ++a) {
-      const double* c = cq + 4 * (ly + RL * a);
+      const double* c = cq + 4 * (act ? ly + RL * a : 0);  // lanes past the grid idle
       const double cv[4] = {c[0], c[1], c[2], c[3]};
 #pragma unroll
       for (int b = 0; b < CW; ++b) qsub_mul(A[a][b], cv, nq[b]);
@@ -1139,53 +1238,9 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
   }
   __syncthreads();
   for (int i = threadIdx.x; i < h; i += 64 * NW) pos[piv[i]] = i;
-  // the rest of the weights operator G (stored transposed, G(row, c) at c*2h + row):
-  //   Ad rows  G(h+j, q*h+k) = (X_kj)_q fac
-  //   As rows  G(j, q*h+k)   = (fac/alpha) sum_i C(j,i) (X_ki)_q,  C = B Fn^T
-  //   W block  G(j, 4h+r)    = (fac/alpha) B(j,r),  G(h+j, 4h+r) = 0
-  // with B = (Msl^T)^{-1} from rrq_real_inverse_kernel.  LDS: C, then Bt and Fn
-  // (for C) or one component of X at a time.
-  extern __shared__ double dyn[];
-  double* Ct = dyn;                    // Ct[i*h + j] = C(j, i)
-  double* Bt = dyn + (size_t)h * h;    // Bt[r*h + j] = B(j, r)
-  double* Fs = Bt + (size_t)h * h;     // Fs[i + h*r] = Fn(i, r)
-  double* Xq = Bt;                     // Xq[i*h + k] = (X_ki)_q
-  const double fac = 1.0 / (16.0 * atan(1.0));
-  const double fa = fac / P.frame[(size_t)k * 16 + 12];
   double* G = P.G + (size_t)k * m * 5 * h;
-  {
-    const double* Bk = P.Binv + (size_t)k * h * h;
-    const double* FN = P.Fn + (size_t)k * h * h;
-    for (int e = threadIdx.x; e < h * h; e += 64 * NW) {
-      Bt[(size_t)(e % h) * h + e / h] = Bk[e];
-      Fs[e] = FN[e];
-    }
-  }
-  __syncthreads();
-  {
-    const int nig = (h + 3) / 4;
-    for (int it = threadIdx.x; it < h * nig; it += 64 * NW) {
-      const int j = it % h, i0 = 4 * (it / h);
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      for (int r = 0; r < h; ++r) {
-        const double bv = Bt[(size_t)r * h + j];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = fma(bv, Fs[(i0 + u < h ? i0 + u : h - 1) + (size_t)h * r], acc[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (i0 + u < h) Ct[(size_t)(i0 + u) * h + j] = acc[u];
-    }
-    for (int e = threadIdx.x; e < h * h; e += 64 * NW) {
-      const int j = e % h, r = e / h;
-      double* w = G + (size_t)(4 * h + r) * m;
-      w[j] = Bt[(size_t)r * h + j] * fa;
-      w[h + j] = 0.0;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    __syncthreads();  // Bt/Fn (q = 0) or the previous component no longer read
+  if constexpr (!EPI) {
+    __syncthreads();
 #pragma unroll
     for (int a = 0; a < RA; ++a)
 #pragma unroll
@@ -1193,37 +1248,62 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
         const int r = ly + RL * a, c = lx + CL * (wv * CW + b);
         if (act & (r < h) & (c < h)) {
           const int kx = pos[r], j = piv[c];
-          Xq[(size_t)j * h + kx] = A[a][b][q];
-          G[(size_t)(q * h + kx) * m + h + j] = A[a][b][q] * fac;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) G[(size_t)(q * h + kx) * m + h + j] = A[a][b][q];
         }
       }
-    __syncthreads();
-    // As rows of component q: tiles of 3 rows j x 4 columns k
-    const int njg = (h + 2) / 3, nkg = (h + 3) / 4;
-    for (int it = threadIdx.x; it < njg * nkg; it += 64 * NW) {
-      const int j0 = 3 * (it % njg), k0 = 4 * (it / njg);
-      double acc[3][4];
+  } else {
+    extern __shared__ double dyn[];
+    const SolveLds L(dyn, h);
+    const double fac = 1.0 / (16.0 * atan(1.0));
+    const double fa = fac / P.frame[(size_t)k * 16 + 12];
+    solve_epilogue_c(P, k, h, L, fa, G, threadIdx.x, 64 * NW);
 #pragma unroll
-      for (int v = 0; v < 3; ++v)
+    for (int q = 0; q < 4; ++q) {
+      __syncthreads();  // Bt/Fn (q = 0) or the previous component no longer read
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[v][u] = 0.0;
-      for (int i = 0; i < h; ++i) {
-        double cv[3], xv[4];
+      for (int a = 0; a < RA; ++a)
 #pragma unroll
-        for (int v = 0; v < 3; ++v) cv[v] = Ct[(size_t)i * h + (j0 + v < h ? j0 + v : h - 1)];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xv[u] = Xq[(size_t)i * h + (k0 + u < h ? k0 + u : h - 1)];
-#pragma unroll
-        for (int v = 0; v < 3; ++v)
-#pragma unroll
-          for (int u = 0; u < 4; ++u) acc[v][u] = fma(cv[v], xv[u], acc[v][u]);
-      }
-#pragma unroll
-      for (int v = 0; v < 3; ++v)
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if ((j0 + v < h) & (k0 + u < h)) G[(size_t)(q * h + k0 + u) * m + j0 + v] = acc[v][u] * fa;
+        for (int b = 0; b < CW; ++b) {
+          const int r = ly + RL * a, c = lx + CL * (wv * CW + b);
+          if (act & (r < h) & (c < h)) {
+            const int kx = pos[r], j = piv[c];
+            L.Xq[(size_t)j * h + kx] = A[a][b][q];
+            G[(size_t)(q * h + kx) * m + h + j] = A[a][b][q] * fac;
+          }
+        }
+      __syncthreads();
+      solve_epilogue_as(h, q, L, fa, G, threadIdx.x, 64 * NW);
     }
+  }
+}
+
+// The rest of G after rrq_quat_inverse_kernel<..., false> has left X in its Ad
+// rows: the same epilogue as the fused form (so the same results), one
+// workgroup per panel, X read back one component at a time and the Ad rows
+// scaled in place.
+__global__ void __launch_bounds__(256) rrq_quat_epilogue_kernel(RRQTables T, RRQPanels P, int npan,
+                                                                const int* __restrict__ nrrq) {
+  const int k = blockIdx.x;
+  if (nrrq && nrrq[k] == 0) return;
+  const int h = T.h, m = 2 * h;
+  extern __shared__ double dyn[];
+  const SolveLds L(dyn, h);
+  const double fac = 1.0 / (16.0 * atan(1.0));
+  const double fa = fac / P.frame[(size_t)k * 16 + 12];
+  double* G = P.G + (size_t)k * m * 5 * h;
+  solve_epilogue_c(P, k, h, L, fa, G, threadIdx.x, blockDim.x);
+  for (int q = 0; q < 4; ++q) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+      const int j = e % h, kx = e / h;
+      double* g = G + (size_t)(q * h + kx) * m + h + j;
+      const double x = *g;
+      L.Xq[(size_t)j * h + kx] = x;
+      *g = x * fac;
+    }
+    __syncthreads();
+    solve_epilogue_as(h, q, L, fa, G, threadIdx.x, blockDim.x);
   }
 }
 
@@ -1238,7 +1318,7 @@ size_t rrq_quat_inverse_smem(int h) { return sizeof(double) * (size_t)3 * h * h;
 template <int RL, int RA, int CL, int CB>
 __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPanels P, int npan,
                                                               const int* __restrict__ nrrq) {
-  static_assert(RL * CL <= 64 && RL * RA <= 64, "lane grid");
+  static_assert(RL * CL <= 64 && RL * RA <= 128, "lane grid");
   const int k = blockIdx.x;
   if (nrrq && nrrq[k] == 0) return;  // null: every panel
   const int h = T.h;
@@ -1257,7 +1337,7 @@ __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPa
       const int r = ly + RL * a, c = lx + CL * b;
       A[a][b] = (act & (r < h) & (c < h)) ? MS[r + (size_t)h * c] : 0.0;
     }
-  unsigned long long used = 0ull;
+  RowMask used;
   for (int kk = 0; kk < h; ++kk) {
     const int buf = kk & 1, kt = kk % CL, kb = kk / CL;
     const bool own = act & (lx == kt);
@@ -1270,7 +1350,7 @@ __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPa
       for (int a = 0; a < RA; ++a) {
         const int r = ly + RL * a;
         const double v = fabs(A[a][b]);
-        const bool t = own & (r < h) & !((used >> (r & 63)) & 1ull) & (v > bv);
+        const bool t = own & (r < h) & !used.has(r) & (v > bv);
         bv = t ? v : bv;
         bi = t ? r : bi;
       }
@@ -1288,8 +1368,8 @@ __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPa
       }
     }
     p = __builtin_amdgcn_readfirstlane(p);
-    if (p >= h) p = __builtin_ctzll(~used);  // NaN entries only
-    used |= 1ull << p;
+    if (p >= h) p = used.first_free();  // NaN entries only
+    used.add(p);
     if (lane == 0) piv[kk] = p;
     const int pa = p / RL;
     const bool prow = act & (ly == p % RL);
@@ -1321,7 +1401,7 @@ __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPa
     for (int b = 0; b < CB; ++b) nr[b] = ip * rowr[buf][lx + CL * b];
 #pragma unroll
     for (int a = 0; a < RA; ++a) {
-      const double c = colr[buf][ly + RL * a];
+      const double c = colr[buf][act ? ly + RL * a : 0];  // lanes past the grid idle
 #pragma unroll
       for (int b = 0; b < CB; ++b) A[a][b] = fma(-c, nr[b], A[a][b]);
     }
@@ -1347,15 +1427,24 @@ __global__ void __launch_bounds__(64) rrq_real_inverse_kernel(RRQTables T, RRQPa
     }
 }
 
+// whether one workgroup on the current device may take this much dynamic LDS
+// (beside a few KB of static LDS)
+static bool lds_fits(size_t bytes) {
+  int dev = 0, mx = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  if (hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return false;
+  return bytes + 16 * 1024 <= (size_t)mx;
+}
+
 void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq, hipStream_t s, int variant) {
   const int h = T.h;
   const int rpt = (h + 15) / 16, cpt = (2 * h + 15) / 16;
   const size_t lds = rrq_solve_smem(h);
   if (!rrq_solve_g1_in_lds(h) && !P.scr) throw std::runtime_error("RRQ solve: scratch for G1 not allocated");
-  // the fused vector-unit elimination by default; QOL_SOLVE_MFMA=1 selects, for
-  // patch orders up to 8, the matrix-core quaternion elimination (A as 4-wave
-  // accumulator tiles, B from a separate real inverse), which is slower on the
-  // order-8 workload (5.3 vs 4.9 ms for 14,250 panels)
+  // variant 2 (or QOL_SOLVE_MFMA=1) selects, for patch orders up to 8, the
+  // matrix-core quaternion elimination (A as 4-wave accumulator tiles, B from a
+  // separate real inverse), which is slower on the order-8 workload (5.3 vs 4.9
+  // ms for 14,250 panels, against 2.4 ms for the register-resident inverses)
   const char* mf = std::getenv("QOL_SOLVE_MFMA");
   if (variant == 2 || (variant == 0 && mf && mf[0] == '1')) switch (T.p) {
 #define QOL_SOLVE_MFMA(P_, RT_, CT_, RPT_)                                                                  \
@@ -1374,16 +1463,16 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
     default: break;
   }
 #undef QOL_SOLVE_MFMA
-  // patch orders up to 8: the real and the quaternion inverse with the matrix in
-  // registers (the latter also forms G); QOL_SOLVE_FUSED=1 keeps the single fused
-  // kernel
+  // by default the real and the quaternion inverse with the matrix in registers;
+  // variant 1 (or QOL_SOLVE_FUSED=1) selects the single fused kernel, which also
+  // serves orders 9-12 where the epilogue's LDS does not fit
   const char* fu = std::getenv("QOL_SOLVE_FUSED");
   const bool fused = variant == 1 || (variant == 0 && fu && fu[0] == '1');
   if (h <= 36 && !fused) {
     // lane grid per order (rows RL x RA, slots CL x CB)
 #define QOL_QINV(RL_, RA_, CL_, CB_, NW_)                                                                \
   hipLaunchKernelGGL((rrq_real_inverse_kernel<RL_, RA_, CL_, CB_>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); \
-  hipLaunchKernelGGL((rrq_quat_inverse_kernel<RL_, RA_, CL_, CB_, NW_>), dim3(npan), dim3(64 * NW_),             \
+  hipLaunchKernelGGL((rrq_quat_inverse_kernel<RL_, RA_, CL_, CB_, NW_, true>), dim3(npan), dim3(64 * NW_),       \
                      rrq_quat_inverse_smem(h), s, T, P, npan, nrrq)
     if (h <= 3) { QOL_QINV(3, 1, 3, 1, 1); }
     else if (h <= 6) { QOL_QINV(6, 1, 6, 1, 1); }
@@ -1393,6 +1482,27 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
     else if (h <= 28) { QOL_QINV(4, 7, 16, 2, 2); }
     else { QOL_QINV(6, 6, 10, 4, 2); }
 #undef QOL_QINV
+    return;
+  }
+  // patch orders 9 to 12: the same register-resident inverses over more waves
+  // (about 48 quaternion entries per lane), the elimination without dynamic LDS
+  // and G finished by rrq_quat_epilogue_kernel, whose 3h^2 doubles (146 KB at
+  // h = 78) must fit one workgroup's LDS
+  if (h <= 78 && !fused && lds_fits(rrq_quat_inverse_smem(h))) {
+    const size_t el = rrq_quat_inverse_smem(h);
+    if (el > 64 * 1024)
+      (void)hipFuncSetAttribute((const void*)rrq_quat_epilogue_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)el);
+#define QOL_QINV_SPLIT(RL_, RA_, CL_, CB_, NW_)                                                                   \
+  hipLaunchKernelGGL((rrq_real_inverse_kernel<RL_, RA_, CL_, CB_>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); \
+  hipLaunchKernelGGL((rrq_quat_inverse_kernel<RL_, RA_, CL_, CB_, NW_, false>), dim3(npan), dim3(64 * NW_), 0, s, T, \
+                     P, npan, nrrq);                                                                              \
+  hipLaunchKernelGGL(rrq_quat_epilogue_kernel, dim3(npan), dim3(256), el, s, T, P, npan, nrrq)
+    if (h <= 45) { QOL_QINV_SPLIT(8, 6, 8, 6, 3); }
+    else if (h <= 55) { QOL_QINV_SPLIT(9, 7, 7, 8, 4); }
+    else if (h <= 66) { QOL_QINV_SPLIT(10, 7, 6, 12, 6); }
+    else { QOL_QINV_SPLIT(12, 7, 5, 16, 8); }
+#undef QOL_QINV_SPLIT
     return;
   }
   if (rpt <= 3 && cpt <= 5) {

@@ -52,12 +52,19 @@ def test_offsurface_p8_vs_oracle_and_grf(oracle):
     assert rel < 1e-5
 
 
-@pytest.mark.parametrize("p", [6, 8])
+# per-weight agreement of two FP64 solves, about 1e-16 cond(M^T): cond is about
+# 1.9e4, 1.7e6, 1.7e8 and 2.2e10 at p = 6, 8, 10, 12
+SOLVE_TOL = {6: 2e-7, 8: 2e-7, 10: 1e-6, 12: 2e-5}
+
+
+@pytest.mark.parametrize("p", [6, 8, 10, 12])
 def test_solve_variants_and_single_stream(p, monkeypatch):
-    """The fused vector-unit solve (QOL_SOLVE_FUSED=1) and the matrix-core
-    quaternion solve (QOL_SOLVE_MFMA=1) agree with the default one-wave solve to
-    the conditioning of the system, and the one-stream schedule (QOL_ONE_STREAM=1)
-    gives the four-stream result bit for bit."""
+    """The fused vector-unit solve (QOL_SOLVE_FUSED=1) and, up to order 8, the
+    matrix-core quaternion solve (QOL_SOLVE_MFMA=1) agree with the default
+    register-resident inverses (at orders 10 and 12 spread over 4-8 waves, with
+    G finished by a separate epilogue kernel) to the conditioning of the system,
+    and the one-stream schedule (QOL_ONE_STREAM=1) gives the four-stream result
+    bit for bit."""
     print(f"solve variants and one-stream schedule, VMEC surface 12x30, p={p}", flush=True)
     geo = W.geometry(12, 30, p, modes=W.VMEC_BENCH)
     tx, tri, dist = W.targets(geo, 2000, -6, -1, sides=0, seed=5)
@@ -85,4 +92,5 @@ def test_solve_variants_and_single_stream(p, monkeypatch):
         for key in ("As", "Ad"):
             assert np.array_equal(alt[key][~rq], base[key][~rq]), (var, key)
             err = np.abs(alt[key][rq] - base[key][rq]) / np.abs(base[key][rq]).max(axis=1, keepdims=True)
-            assert err.max() < 2e-7, (var, key, err.max())
+            assert err.max() < SOLVE_TOL[p], (var, key, err.max())
+            print(f"  {var} {key} max rel diff {err.max():.2e}", flush=True)

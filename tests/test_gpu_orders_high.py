@@ -1,7 +1,8 @@
 """Patch orders 9-12 on the GPU against the oracle.  These orders run kernels and
-tile shapes that orders up to 8 do not: the scalar level-geometry kernel, direct
-kernels with 4 and 5 column tiles, and the generic fused solve (4x7 and 5x10
-register tiles; G1 in scratch memory above h = 51).  The quaternion system's
+tile shapes that orders up to 8 do not: the matrix-core level geometry over 12-20
+blocks of 4 panel nodes, direct kernels with 4 and 5 column tiles, and the
+register-resident inverses spread over 3, 4, 6 and 8 waves with G finished by a
+separate epilogue kernel (rows beyond 64 in the pivot mask).  The quaternion system's
 condition number grows to about 2e10 at p = 12, so the per-weight bound is
 looser, and so is the bound on the action on a smooth density."""
 import numpy as np
