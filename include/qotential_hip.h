@@ -103,6 +103,18 @@ int qol_make_targets(int64_t p, int64_t ntri, const double* sx, const double* sn
 void qol_simplex_precomp_mex_(const int64_t* nquad, const int64_t* korder, const int64_t* kpols,
                               double* tgl, double* wgl, double* Dgl, double* w_bclag,
                               double* Legmat, double* umatr, double* vmatr);
+/* The level form every drop-in entry point uses: the reference's module
+ * variables PR_ADAPTIVE and PR_SQUARE (src/patch_refine_mod.f90:51-55), which
+ * its drivers set from iadap (test/stellarator_bench.f90:46-47).  square = 1
+ * (the default) regroups levels 2-4 into Gauss-Legendre squares; square = 0
+ * keeps triangle sub-patches (:229-240); adaptive = 1 (triangle levels only)
+ * serves pairs by the adaptive walk over levels 1-3, then level 4, then RRQ
+ * (:532-539, 617-842).  Both set is refused at the next evaluation, as the
+ * reference stops ("PR_SQUARE and PR_ADAPTIVE cannot be combined", :116-117).
+ * Process-wide, like the module variables; the order-table queries below
+ * follow square. */
+int qol_set_level_mode(int64_t adaptive, int64_t square);
+int qol_get_level_mode(int64_t* adaptive, int64_t* square);
 /* level sizes of patch order p: info[0..12] = h, nquad, ncoef, ntot, nsubtot,
  * nl(1..4), nsub(1..4).  No reference counterpart (the reference keeps them in
  * the module variable REF, src/patch_refine_mod.f90:51). */
@@ -225,10 +237,21 @@ int qol_engine_get_values(void* engine, double* val, double* u);
 int qol_engine_set_potential(void* engine, const double* u);
 /* far field by direct summation (stellarator_grf_core_mod.f90:560-611) */
 int qol_engine_far(void* engine, const double* ub, const double* ubn, double* u);
+/* this engine's level sizes: info[0..11] = ntot, nsubtot, nl(1..4), nsub(1..4),
+ * square, adaptive (they follow its level form); size qol_engine_level_data's
+ * buffers from them */
+int qol_engine_level_info(void* engine, int64_t* info);
+/* the level nodes lgeo(npan, 7, ntot) (x, y, z, nx, ny, nz, w) and balls
+ * (npan, nsubtot, 4) of every panel */
 int qol_engine_level_data(void* engine, double* lgeo, double* balls);
 int qol_engine_triplets(void* engine, const int64_t* trow, int64_t* si, int64_t* sj, double* vs,
                         double* vd);
 int qol_engine_stream(void* engine, void** stream);
+/* The level form of one engine (see qol_set_level_mode); new level tables are
+ * uploaded when square changes.  Adaptive levels take the ball-rule walk only
+ * (no selector, forced labels or split precision).  Labels then read: 1 level 1
+ * whole, 2 a tree cut of levels 2-3, 4 level 4, 5 RRQ. */
+int qol_engine_set_level_mode(void* engine, int64_t adaptive, int64_t square);
 /* Split precision (no reference counterpart; the reference computes every
  * entry in FP64, patch_refine_mod.f90:569-602): pairs served at levels
  * 1..split_levels get their direct S/D weights from the FP32 matrix cores

@@ -19,6 +19,9 @@ extern "C" {
 
 void orc_set_side_tol(double tol) { set_side_tol(tol); }
 
+/* PR_ADAPTIVE / PR_SQUARE; returns 0, or 1 (nothing changed) for both set */
+i64 orc_set_level_mode(i64 adaptive, i64 square) { return set_level_mode(adaptive != 0, square != 0) ? 0 : 1; }
+
 void orc_gauss(i64 n, double* t, double* w, double* D) { gauss(n, t, w, D); }
 
 void orc_koorn_pols(double u, double v, i64 nmax, double* pols, double* du, double* dv) {
@@ -156,9 +159,12 @@ void orc_nearlist_fill(i64 npan, i64 hdim, const double* sx, const double* sw, i
 
 /* Labels only (patch_refine_mod.f90:423-502 levels, :516-615 classification):
    for every pair of the CSR (offs 0-based, idx 1-based), the level 1..4 that
-   serves it or 5 for RRQ.  Panels in nthreads threads. */
+   serves it or 5 for RRQ; with PR_ADAPTIVE the walk's labels (1 level 1 whole,
+   2 a cut of levels 2-3, with cut = 1 + opened level-2 cells when cut is not
+   null), then 4 and 5.  Panels in nthreads threads. */
 void orc_labels(i64 npan, i64 order, const double* sx, const double* rts, const double* rps,
-                const double* tx, const i64* offs, const i64* idx, i64* label, i64 nthreads) {
+                const double* tx, const i64* offs, const i64* idx, i64* label, i64* cut, i64 nthreads) {
+  const bool adaptive = level_adaptive();
   const i64 h = order * (order + 1) / 2, korder = order - 1, orderff = order + 2;
   patch_ref(korder, h, orderff, 4);
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? (int)nthreads : 1)
@@ -167,12 +173,25 @@ void orc_labels(i64 npan, i64 order, const double* sx, const double* rts, const 
     if (m <= 0) continue;
     PanelLevels lv;
     build_patch_levels(korder, h, sx + 3 * k * h, rts + 3 * k * h, rps + 3 * k * h, orderff, 1.4, 4, lv);
+    std::vector<i64> cl, cs;
     for (i64 j = 0; j < m; ++j) {
       const double* x = tx + 3 * (idx[offs[k] + j] - 1);
-      i64 lab = 5;
-      for (i64 il = 0; il < 4; ++il)
-        if (!inside_any_ball(lv.lev[il], x)) { lab = il + 1; break; }
+      i64 lab = 5, c = 0;
+      bool ok = false;
+      if (adaptive) adaptive_walk(lv, 3, x, cl, cs, ok);
+      if (ok) {
+        const bool whole = cl.size() == 1 && cl[0] == 0;
+        lab = whole ? 1 : 2;
+        i64 open = 0;
+        for (size_t q = 0; q < cl.size(); ++q)
+          if (cl[q] == 2) open |= (i64)1 << (cs[q] / 4);
+        c = whole ? 0 : 1 + open;
+      } else {
+        for (i64 il = adaptive ? 3 : 0; il < 4; ++il)
+          if (!inside_any_ball(lv.lev[il], x)) { lab = il + 1; break; }
+      }
       label[offs[k] + j] = lab;
+      if (cut) cut[offs[k] + j] = c;
     }
   }
 }

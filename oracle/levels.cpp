@@ -2,7 +2,8 @@
  * near list.  TEST INFRASTRUCTURE ONLY (see qol_oracle.h).
  *
  * build_patch_levels   restates patch_refine_mod.f90:423-502
- * lap3dsdlpmat_levels  restates patch_refine_mod.f90:516-615 (PR_ADAPTIVE off)
+ * lap3dsdlpmat_levels  restates patch_refine_mod.f90:516-615, and with PR_ADAPTIVE
+ *                      adaptive_pass / cut_gI / walk / one_level (:617-842)
  * lap3dsdlpmat         restates the direct kernel (legacy scalar form
  *                      utils/f/ckernels_expts.c:52-124: zero when r^2 <= 1e-28)
  * nearlist             restates the ball rule of stellarator_grf_core_mod.f90:652-695
@@ -99,14 +100,84 @@ bool inside_any_ball(const PanelLevel& L, const double* x) {
   return false;
 }
 
+/* walk (patch_refine_mod.f90:740-771): from the level-1 cell, a cell is a leaf
+   when the target is outside its ball (strict >); otherwise its four children
+   (cells 4q..4q+3 of the next level, 0-based) are visited, and a visited cell
+   of level ntree holding the target ends the walk with ok = false */
+void adaptive_walk(const PanelLevels& lv, i64 ntree, const double* x, std::vector<i64>& cl,
+                   std::vector<i64>& cs, bool& ok) {
+  cl.clear(); cs.clear(); ok = true;
+  std::vector<i64> act = {0}, nxt;
+  for (i64 il = 0; il < ntree; ++il) {
+    nxt.clear();
+    const PanelLevel& L = lv.lev[il];
+    for (i64 q : act) {
+      double d0 = x[0] - L.qpoint[3 * q], d1 = x[1] - L.qpoint[3 * q + 1], d2 = x[2] - L.qpoint[3 * q + 2];
+      if (d0 * d0 + d1 * d1 + d2 * d2 > L.qradii2[q]) {
+        cl.push_back(il); cs.push_back(q);
+      } else if (il == ntree - 1) {
+        ok = false;
+        return;
+      } else {
+        for (i64 b = 0; b < 4; ++b) nxt.push_back(4 * q + b);
+      }
+    }
+    if (nxt.empty()) break;
+    act = nxt;
+  }
+}
+
+/* adaptive_pass + cut_gI (:617-738): every target the walk serves gets direct
+   S/D on its leaf cells' nodes contracted with those nodes' interpolation rows
+   (the reference's per-mask gI), leaves in walk order; the rest go on to
+   one_level at the last level (:773-842), then RRQ */
+static void adaptive_levels(i64 m, const double* tx, i64 npols, const PanelLevels& lv, const PatchRef& ref,
+                            double* As, double* Ad, std::vector<i64>& irem, i64& nrem, i64* label, i64* cut) {
+  const i64 nlevel = (i64)lv.lev.size(), ntree = nlevel - 1, n1 = ref.lev[0].n;
+  std::vector<i64> inext, cl, cs;
+  std::vector<double> a(n1), b(n1);
+  for (i64 jj = 0; jj < nrem; ++jj) {
+    const i64 j = irem[jj];
+    bool ok = false;
+    adaptive_walk(lv, ntree, tx + 3 * j, cl, cs, ok);
+    if (!ok) { inext.push_back(j); continue; }
+    std::vector<double> s(npols, 0.0), d(npols, 0.0);
+    i64 open = 0;
+    for (size_t c = 0; c < cl.size(); ++c) {
+      const PanelLevel& L = lv.lev[cl[c]];
+      const RefLevel& R = ref.lev[cl[c]];
+      const i64 i0 = cs[c] * n1;
+      lap3dsdlpmat(1, tx + 3 * j, n1, L.sx.data() + 3 * i0, L.snx.data() + 3 * i0, L.sw.data() + i0, a.data(),
+                   b.data());
+      for (i64 k = 0; k < n1; ++k)
+        for (i64 col = 0; col < npols; ++col) {
+          s[col] += a[k] * R.interpmat[i0 + k + R.n * col];
+          d[col] += b[k] * R.interpmat[i0 + k + R.n * col];
+        }
+      if (cl[c] == 2) open |= (i64)1 << (cs[c] / 4);
+    }
+    for (i64 col = 0; col < npols; ++col) { As[j + m * col] = s[col]; Ad[j + m * col] = d[col]; }
+    const bool whole = cl.size() == 1 && cl[0] == 0;
+    if (label) label[j] = whole ? 1 : 2;
+    if (cut) cut[j] = whole ? 0 : 1 + open;
+  }
+  nrem = (i64)inext.size();
+  irem.assign(inext.begin(), inext.end());
+}
+
 void lap3dsdlpmat_levels(i64 m, const double* tx, i64 npols, const PanelLevels& lv,
                          const PatchRef& ref, double* As, double* Ad, i64* idxs, i64* ms,
-                         i64* label) {
+                         i64* label, i64* cut) {
   std::vector<i64> irem(m), ibat, inext;
   for (i64 j = 0; j < m; ++j) irem[j] = j;
   i64 nrem = m;
   i64 nlevel = (i64)lv.lev.size();
-  for (i64 il = 0; il < nlevel && nrem > 0; ++il) {
+  i64 il0 = 0;
+  if (level_adaptive() && nlevel >= 3) {
+    adaptive_levels(m, tx, npols, lv, ref, As, Ad, irem, nrem, label, cut);
+    il0 = nlevel - 1;  /* one_level at the last level */
+  }
+  for (i64 il = il0; il < nlevel && nrem > 0; ++il) {
     const PanelLevel& L = lv.lev[il];
     ibat.clear(); inext.clear();
     for (i64 jj = 0; jj < nrem; ++jj) {

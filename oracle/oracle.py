@@ -46,6 +46,13 @@ def set_side_tol(tol):
     lib().orc_set_side_tol(dbl(tol))
 
 
+def set_level_mode(adaptive=False, square=True):
+    """PR_ADAPTIVE / PR_SQUARE for every later call (both true is refused)"""
+    lib().orc_set_level_mode.restype = i64
+    if lib().orc_set_level_mode(i64(1 if adaptive else 0), i64(1 if square else 0)) != 0:
+        raise ValueError("PR_SQUARE and PR_ADAPTIVE cannot be combined")
+
+
 def gauss(n):
     t = np.zeros(n); w = np.zeros(n); D = np.zeros((n, n), order="F")
     lib().orc_gauss(i64(n), _p(t), _p(w), _p(D))
@@ -150,15 +157,17 @@ def nearlist(hdim, sx, sw, tx, nthreads=1):
     return offs, idx
 
 
-def labels(order, geo, tx, offs, idx, nthreads=1):
-    """level label (1..4, or 5 for RRQ) of every pair of a CSR near list"""
+def labels(order, geo, tx, offs, idx, nthreads=1, cuts=False):
+    """level label (1..4, or 5 for RRQ) of every pair of a CSR near list; with
+    the adaptive level mode also the tree cut of each pair (cuts=True)"""
     h = order * (order + 1) // 2
     npan = geo["sw"].shape[0] // h
     lab = np.zeros(int(offs[-1]), dtype=np.int64)
+    cut = np.zeros(int(offs[-1]), dtype=np.int64) if cuts else None
     lib().orc_labels(i64(npan), i64(order), _p(_f(geo["sx"])), _p(_f(geo["rts"])), _p(_f(geo["rps"])), _p(_f(tx)),
                      _p(np.ascontiguousarray(offs, dtype=np.int64)), _p(np.ascontiguousarray(idx, dtype=np.int64)),
-                     _p(lab), i64(nthreads))
-    return lab
+                     _p(lab), _p(cut), i64(nthreads))
+    return (lab, cut) if cuts else lab
 
 
 def close_apply(order, geo, tx, offs, idx, ub, ubn, exterior=True, ichart=0, sxbd_all=None, rv_all=None,

@@ -58,8 +58,15 @@ struct RefLevel {
 };
 struct PatchRef {
   i64 korder = -1, npols = -1, order = -1, nlevel = -1;
+  bool square = true;
   std::vector<RefLevel> lev;
 };
+/* PR_ADAPTIVE / PR_SQUARE (patch_refine_mod.f90:51-55); both true is refused
+   (returns false, nothing changes), as the reference stops at :116-117 */
+bool set_level_mode(bool adaptive, bool square);
+bool level_adaptive();
+bool level_square();
+/* the tables of the current PR_SQUARE */
 const PatchRef& patch_ref(i64 korder, i64 npols, i64 order, i64 nlevel);
 
 /* ---- per-panel levels (build_patch_levels_r64) ---- */
@@ -74,10 +81,17 @@ void build_patch_levels(i64 korder, i64 npols, const double* sx, const double* r
                         const double* rps, i64 order, double dist, i64 nlevel,
                         PanelLevels& lv);
 /* levels classification + direct + contraction; label(j)=level 1..nlevel or
- * nlevel+1 for targets left to RRQ.  idxs are 1-based. */
+ * nlevel+1 for targets left to RRQ.  idxs are 1-based.  With PR_ADAPTIVE the
+ * adaptive walk serves first: label 1 when level 1 is the whole cut, 2 for a
+ * cut over levels 2-3 (cut(j) = 1 + the mask of opened level-2 cells), then
+ * level nlevel (label nlevel) and RRQ (nlevel + 1). */
 void lap3dsdlpmat_levels(i64 m, const double* tx, i64 npols, const PanelLevels& lv,
                          const PatchRef& ref, double* As, double* Ad, i64* idxs, i64* ms,
-                         i64* label);
+                         i64* label, i64* cut = nullptr);
+/* the walk of one target (patch_refine_mod.f90:740-771): leaf cells (level
+ * 0-based, cell) in walk order, ok = false when a level-ntree cell holds x */
+void adaptive_walk(const PanelLevels& lv, i64 ntree, const double* x, std::vector<i64>& cl,
+                   std::vector<i64>& cs, bool& ok);
 void lap3dsdlpmat(i64 m, const double* tx, i64 n, const double* sx, const double* snx,
                   const double* sw, double* As, double* Ad);
 /* the ball test of one level: x inside some sub-patch ball (strict <) */

@@ -11,6 +11,7 @@
  *   ref_square_regroup (patch_refine_mod.f90:100-421) line by line.
  */
 #include "qol_oracle.h"
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -321,9 +322,10 @@ static void split4(const RefLevel& P, std::vector<double>& uv, std::vector<doubl
   }
 }
 
-static PatchRef build_ref(i64 korder, i64 npols, i64 order, i64 nlevel) {
+static PatchRef build_ref(i64 korder, i64 npols, i64 order, i64 nlevel, bool square) {
   PatchRef R;
   R.korder = korder; R.npols = npols; R.order = order; R.nlevel = nlevel;
+  R.square = square;
   R.lev.resize(nlevel);
   std::vector<double> umatr(npols * npols), vmatr(npols * npols), uvv(2 * npols), pp(npols);
   vr_nodes(korder, uvv.data(), nullptr);
@@ -368,9 +370,10 @@ static PatchRef build_ref(i64 korder, i64 npols, i64 order, i64 nlevel) {
   }
   for (i64 ilev = 1; ilev <= nlevel; ++ilev) {
     RefLevel& L = R.lev[ilev - 1];
-    if (ilev >= 2) {
+    if (square && ilev >= 2) {
       square_regroup(R, ilev, n1, korder, ilev == nlevel && nlevel == 4);
     } else {
+      /* PR_SQUARE off (patch_refine_mod.f90:234-239): n1 nodes per sub-patch */
       L.suboff.resize(L.nsub + 1);
       L.subtyp.assign(L.nsub, 0);
       for (i64 k = 0; k <= L.nsub; ++k) L.suboff[k] = k * n1;
@@ -389,13 +392,26 @@ static PatchRef build_ref(i64 korder, i64 npols, i64 order, i64 nlevel) {
   return R;
 }
 
+/* PR_ADAPTIVE / PR_SQUARE (patch_refine_mod.f90:51-55): process-wide, as the
+   reference's module variables are */
+static std::atomic<int> g_adaptive{0}, g_square{1};
+bool set_level_mode(bool adaptive, bool square) {
+  if (adaptive && square) return false; /* patch_refine_mod.f90:116-117 stops */
+  g_adaptive.store(adaptive ? 1 : 0);
+  g_square.store(square ? 1 : 0);
+  return true;
+}
+bool level_adaptive() { return g_adaptive.load() != 0; }
+bool level_square() { return g_square.load() != 0; }
+
 const PatchRef& patch_ref(i64 korder, i64 npols, i64 order, i64 nlevel) {
   static std::mutex mu;
-  static std::map<std::tuple<i64, i64, i64, i64>, PatchRef> cache;
+  static std::map<std::tuple<i64, i64, i64, i64, bool>, PatchRef> cache;
+  const bool square = level_square();
   std::lock_guard<std::mutex> g(mu);
-  auto key = std::make_tuple(korder, npols, order, nlevel);
+  auto key = std::make_tuple(korder, npols, order, nlevel, square);
   auto it = cache.find(key);
-  if (it == cache.end()) it = cache.emplace(key, build_ref(korder, npols, order, nlevel)).first;
+  if (it == cache.end()) it = cache.emplace(key, build_ref(korder, npols, order, nlevel, square)).first;
   return it->second;
 }
 

@@ -13,6 +13,8 @@
 // (qol_engine_*) keeps geometry, targets and weights resident across calls.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -70,6 +72,10 @@ struct PanelEngineCache {
   int64_t p = -1, ichart = -1;
 };
 
+// the reference's PR_ADAPTIVE / PR_SQUARE module variables (process-wide, as
+// Fortran module variables are; patch_refine_mod.f90:51-55)
+std::atomic<int> g_adaptive{0}, g_square{1};
+
 Engine& panel_engine(int64_t p, int64_t ichart, const double* sx, const double* snx, const double* sw,
                      const double* rts, const double* rps, const double* sxbd, const double* rv) {
   static thread_local PanelEngineCache* cache = nullptr;
@@ -82,10 +88,20 @@ Engine& panel_engine(int64_t p, int64_t ichart, const double* sx, const double* 
   } else {
     cache->e->set_surface(sx, snx, sw, rts, rps, sxbd, rv, false);
   }
+  bool ad = false, sq = true;
+  level_mode(&ad, &sq);
+  cache->e->set_level_mode(ad, sq);
   return *cache->e;
 }
 
 }  // namespace
+
+namespace qol {
+void level_mode(bool* adaptive, bool* square) {
+  *adaptive = g_adaptive.load() != 0;
+  *square = g_square.load() != 0;
+}
+}  // namespace qol
 
 extern "C" {
 
@@ -96,6 +112,24 @@ int qol_engine_create(int64_t p, int64_t npan, const double* sx, const double* s
   return guarded([&] {
     *out = nullptr;
     *out = new Engine(p, npan, sx, snx, sw, rts, rps, ichart, sxbd, rv, device_ptrs != 0);
+  });
+}
+
+int qol_engine_set_level_mode(void* h, int64_t adaptive, int64_t square) {
+  return guarded([&] { E(h)->set_level_mode(adaptive != 0, square != 0); });
+}
+
+int qol_set_level_mode(int64_t adaptive, int64_t square) {
+  return guarded([&] {
+    g_adaptive.store(adaptive != 0 ? 1 : 0);
+    g_square.store(square != 0 ? 1 : 0);
+  });
+}
+
+int qol_get_level_mode(int64_t* adaptive, int64_t* square) {
+  return guarded([&] {
+    *adaptive = g_adaptive.load();
+    *square = g_square.load();
   });
 }
 
@@ -165,6 +199,16 @@ int qol_engine_set_potential(void* h, const double* u) {
 
 int qol_engine_far(void* h, const double* ub, const double* ubn, double* u) {
   return guarded([&] { E(h)->far(ub, ubn, u); });
+}
+
+int qol_engine_level_info(void* h, int64_t* info) {
+  return guarded([&] {
+    const LevelArgs& A = E(h)->tab->LA;
+    info[0] = A.ntot; info[1] = A.nsubtot;
+    for (int l = 0; l < 4; ++l) { info[2 + l] = A.lev_n[l]; info[6 + l] = A.lev_nsub[l]; }
+    info[10] = E(h)->square ? 1 : 0;
+    info[11] = E(h)->adaptive ? 1 : 0;
+  });
 }
 
 int qol_engine_level_data(void* h, double* lgeo, double* balls) {
@@ -338,7 +382,7 @@ int qol_hbm_copy(int64_t nbytes, double* ms) {
 
 int qol_order_info(int64_t p, int64_t* info) {
   return guarded([&] {
-    const OrderTables& T = order_tables(p);
+    const OrderTables& T = order_tables(p, g_square.load() != 0);
     info[0] = T.h; info[1] = T.nq; info[2] = T.ncoef; info[3] = T.ntot; info[4] = T.nsubtot;
     for (int l = 0; l < 4; ++l) { info[5 + l] = T.lev[l].n; info[9 + l] = T.lev[l].nsub; }
   });
@@ -347,7 +391,7 @@ int qol_order_info(int64_t p, int64_t* info) {
 int qol_order_level(int64_t p, int64_t lev, double* uv, double* wt, double* interp, int64_t* suboff,
                     int64_t* subtyp) {
   return guarded([&] {
-    const OrderTables& T = order_tables(p);
+    const OrderTables& T = order_tables(p, g_square.load() != 0);
     if (lev < 1 || lev > 4) throw std::runtime_error("level must be 1..4");
     const LevelTable& L = T.lev[lev - 1];
     std::memcpy(uv, L.uv.data(), sizeof(double) * 2 * L.n);
@@ -445,7 +489,10 @@ void qol_getnearquad_lap_rrq_mex_(const int64_t* npan, const int64_t* nterms, co
     const int64_t np_ = *npan, nt = *ntcx;
     if (tcxi[0] != 1) throw std::runtime_error("tcxi(1) must be 1");
     if ((tcxi[np_] - 1) * h != *nnz) throw std::runtime_error("nnz must equal (tcxi(npan+1)-1)*hdim");
-    Engine e(p, np_, sx, snx, sw, rts, rps, 0, nullptr, nullptr, false);
+    bool ad = false, sq = true;
+    level_mode(&ad, &sq);
+    Engine e(p, np_, sx, snx, sw, rts, rps, 0, nullptr, nullptr, false, nullptr, sq);
+    e.set_level_mode(ad, sq);
     e.set_targets(nt, tcx, false);
     std::vector<int64_t> offs(np_ + 1), idx(tcxi[np_] - 1);
     for (int64_t k = 0; k <= np_; ++k) offs[k] = tcxi[k] - 1;

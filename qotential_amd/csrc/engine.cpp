@@ -113,8 +113,8 @@ static void level_layout(const OrderTables& T, LevelArgs& LA, std::vector<int>& 
   }
 }
 
-DeviceTables::DeviceTables(i64 p, hipStream_t s) {
-  const OrderTables& T = order_tables(p);
+DeviceTables::DeviceTables(i64 p, bool square, hipStream_t s) {
+  const OrderTables& T = order_tables(p, square);
   h = (int)T.h;
   ntot = (int)T.ntot;
   nsubtot = (int)T.nsubtot;
@@ -154,6 +154,12 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
   upload(d_chunks, chunks.data(), chunks.size(), s);
   LA.chunks = (const int*)d_chunks.ptr;
   LA.nchunk = (int)chunks.size() / 5;
+  LA.n1 = LA.lev_n[0];
+  if (!square) {
+    const std::vector<int> seg = adaptive_segments(LA);
+    upload(d_aseg, seg.data(), seg.size(), s);
+    LA.aseg = (const int*)d_aseg.ptr;
+  }
   upload(d_tgl, T.tgl.data(), T.tgl.size(), s);
   upload(d_wgl, T.wgl.data(), T.wgl.size(), s);
   upload(d_Leg, T.Legmat.data(), T.Legmat.size(), s);
@@ -187,8 +193,8 @@ DeviceTables::DeviceTables(i64 p, hipStream_t s) {
 // ---------------------------------------------------------------------------
 Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const double* sw, const double* rts,
                const double* rps, i64 ichart_, const double* sxbd, const double* rv, bool device_ptrs,
-               const StreamSet* borrow)
-    : p((int)p_), npan((int)npan_), ichart((int)ichart_) {
+               const StreamSet* borrow, bool square_levels)
+    : p((int)p_), npan((int)npan_), ichart((int)ichart_), square(square_levels) {
   if (p < 2 || p > QOL_MAXP) throw std::runtime_error("patch order out of range for the device kernels");
   if (npan <= 0) throw std::runtime_error("npan must be positive");
   if (borrow) {
@@ -216,7 +222,7 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
     for (hipEvent_t* e : {&ev_start, &ev_lists, &ev_direct, &ev_setup, &ev_system, &ev_solve, &ev_edge, &ev_proj})
       HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
-  tab = std::make_unique<DeviceTables>(p, stream);
+  tab = std::make_unique<DeviceTables>(p, square, stream);
   h = tab->h;
   nq = p + 2;
   ncoef = (p + 1) * (p + 2) / 2;
@@ -264,8 +270,8 @@ Engine::~Engine() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-size_t Engine::one_panel_bytes(i64 p, i64 m, i64 ichart) {
-  const OrderTables& T = order_tables(p);
+size_t Engine::one_panel_bytes(i64 p, i64 m, i64 ichart, bool square) {
+  const OrderTables& T = order_tables(p, square);
   LevelArgs LA{};
   std::vector<int> suboff, subtyp;
   level_layout(T, LA, suboff, subtyp);
@@ -280,13 +286,15 @@ size_t Engine::one_panel_bytes(i64 p, i64 m, i64 ichart) {
                    T.Legmat.size() * D, T.umatr.size() * D, T.uvn.size() * D, koorn_eval_table((int)p).size() * D,
                    3 * h * D})
     add(b);
+  if (!square) add((size_t)kAdaptBuckets * (kAdaptSegs + 1) * I);
   // the panel (constructor), targets and pairs
   for (size_t b : {3 * h * D, 3 * h * D, h * D, 3 * h * D, 3 * h * D}) add(b);
   if (ichart == 1) { add(9 * nq * D); add(9 * D); }
   add(3 * mm * D);                  // tx
   add(2 * I); add(mm * I);          // offs, tidx
   // alloc_pairs
-  for (size_t b : {mm, mm * I, 2 * h * D * mm, mm * D, (size_t)I, 5 * (size_t)I, 5 * (size_t)I, 8 * (size_t)I, mm * I})
+  for (size_t b : {mm, mm * I, 2 * h * D * mm, mm * D, (size_t)I, 5 * (size_t)I, 5 * (size_t)I, 8 * (size_t)I, mm * I,
+                   mm, mm * I, 3 * (size_t)kAdaptBuckets * I})
     add(b);
   // alloc_rrq
   for (size_t b : {16 * D, 9 * nq * D, 6 * nb * D, 3 * h * D, 12 * nb * ncoef * D, 3 * h * h * D, h * h * D,
@@ -323,6 +331,26 @@ void Engine::alloc_pairs() {
   d_lstart.alloc(sizeof(int) * 5 * npan);
   d_ltot.alloc(sizeof(int) * 8);
   d_lpairs.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  d_abucket.alloc((size_t)(nnz > 0 ? nnz : 1));
+  d_apairs.alloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  d_acnt.alloc(sizeof(int) * 3 * kAdaptBuckets);
+}
+
+// The reference's PR_ADAPTIVE and PR_SQUARE (patch_refine_mod.f90:51-55, 109-117):
+// square levels (the default), triangle levels, or triangle levels served by the
+// adaptive walk.  New level tables are uploaded when the level form changes.
+void Engine::set_level_mode(bool adaptive_, bool square_) {
+  if (adaptive_ && square_) throw std::runtime_error("PR_SQUARE and PR_ADAPTIVE cannot be combined");
+  if (square_ != square) {
+    HIPCHK(hipStreamSynchronize(stream));
+    tab = std::make_unique<DeviceTables>(p, square_, stream);
+    square = square_;
+    levels_built = false;
+    evaluated = false;
+    val_current = false;
+  }
+  adaptive = adaptive_;
+  tab->LA.adaptive = adaptive ? 1 : 0;
 }
 
 // classification and per-label pair lists (main stream)
@@ -332,7 +360,7 @@ void Engine::classify_lists() {
   launch_classify(tab->LA, S, npan, (const int*)d_offs.ptr, (const int*)d_tidx.ptr, (const double*)d_tx.ptr,
                   (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr,
                   (const double*)d_balls.ptr, (int8_t*)d_label.ptr, (int*)d_ppanel.ptr, (int*)d_nrrq.ptr,
-                  (int*)d_lcount.ptr, stream);
+                  (int*)d_lcount.ptr, (int8_t*)d_abucket.ptr, stream);
   // pairs the selector serves with the panel's own nodes: naive weights, and a
   // zero correction when eval makes the corrections itself
   if (S.mode == 2 || (S.mode == 1 && S.forced == 0))
@@ -350,6 +378,20 @@ void Engine::classify_lists() {
 // direct S/D of every level-served pair: levels 1..split_levels on the FP32
 // matrix cores (split precision), the rest on the FP64 matrix cores
 void Engine::direct(hipStream_t s) {
+  if (adaptive) {
+    // level 1 whole, the tree cuts (list 1), level 4 for what the tree cannot serve
+    if (split_levels > 0) throw std::runtime_error("split precision is not built for adaptive levels");
+    launch_direct_mfma(tab->LA, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr,
+                       nnz, s, 0, 0);
+    launch_direct_adaptive(tab->LA, (const int*)d_ltot.ptr, (const int*)d_lpairs.ptr, (const int8_t*)d_abucket.ptr,
+                           (int*)d_acnt.ptr, (int*)d_apairs.ptr, (const int*)d_ppanel.ptr, (const int*)d_tidx.ptr,
+                           (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr, nnz, s);
+    launch_direct_mfma(tab->LA, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr,
+                       nnz, s, 3, 3);
+    return;
+  }
   const int ns = std::max(0, std::min(4, split_levels));
   launch_direct_split(tab->LA, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                       (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_lgeo.ptr, (double*)d_wout.ptr,

@@ -232,9 +232,10 @@ void regroup_squares(LevelTable& L, i64 n1, i64 p, bool cc, i64 ilev, double wsu
   }
 }
 
-std::unique_ptr<OrderTables> make_tables(i64 p) {
+std::unique_ptr<OrderTables> make_tables(i64 p, bool square) {
   auto T = std::make_unique<OrderTables>();
   T->p = p;
+  T->square = square;
   T->h = p * (p + 1) / 2;
   T->nq = p + 2;
   T->korder = p - 1;
@@ -303,7 +304,16 @@ std::unique_ptr<OrderTables> make_tables(i64 p) {
   for (double w : L[0].wt) wsum1 += w;
   L[0].suboff = {0, n1};
   L[0].subtyp = {0};
-  for (int l = 1; l < 4; ++l) regroup_squares(L[l], n1, p, l == 3, l + 1, wsum1);
+  for (int l = 1; l < 4; ++l) {
+    if (square) {
+      regroup_squares(L[l], n1, p, l == 3, l + 1, wsum1);
+    } else {
+      // PR_SQUARE off: every sub-patch keeps its n1 triangle nodes
+      L[l].suboff.assign(L[l].nsub + 1, 0);
+      for (i64 q = 0; q <= L[l].nsub; ++q) L[l].suboff[q] = q * n1;
+      L[l].subtyp.assign(L[l].nsub, 0);
+    }
+  }
   T->ntot = 0;
   T->nsubtot = 0;
   std::vector<double> pol(h);
@@ -326,13 +336,14 @@ std::unique_ptr<OrderTables> make_tables(i64 p) {
 
 }  // namespace
 
-const OrderTables& order_tables(i64 p) {
+const OrderTables& order_tables(i64 p, bool square) {
   static std::mutex mu;
-  static std::map<i64, std::unique_ptr<OrderTables>> cache;
+  static std::map<std::pair<i64, bool>, std::unique_ptr<OrderTables>> cache;
   if (p < 2 || p > 20) throw std::runtime_error("patch order must be in 2..20, got " + std::to_string(p));
   std::lock_guard<std::mutex> g(mu);
-  auto it = cache.find(p);
-  if (it == cache.end()) it = cache.emplace(p, make_tables(p)).first;
+  const auto key = std::make_pair(p, square);
+  auto it = cache.find(key);
+  if (it == cache.end()) it = cache.emplace(key, make_tables(p, square)).first;
   return *it->second;
 }
 

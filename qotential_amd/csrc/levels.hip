@@ -376,6 +376,45 @@ __global__ void __launch_bounds__(256) compact_kernel(const int* __restrict__ of
 // (row-major, padded to 16*NT columns).  Accumulators: 16 pairs x 16*NT
 // outputs for S and for D.
 
+// One step of the direct contraction: this lane's pair (A row m of the tile)
+// against node ic (the step's row kq), S and D entries on the vector ALUs, then
+// accS/D[q] += entry x row ic of the padded interpolation table B (columns
+// 16q + m) on the matrix cores.  use = false gives exactly zero entries (the
+// operands stay finite: clamped index, r^2 = 1), so no branch is needed.
+template <int NT>
+__device__ __forceinline__ void direct_step(const double* __restrict__ G, const double* __restrict__ B, int pad,
+                                            int m, int ic, bool use, double x0, double x1, double x2, double c4,
+                                            qd4* accS, qd4* accD) {
+  const double2* g2 = (const double2*)(G + 8 * ic);
+  const double2 ga = g2[0], gb = g2[1], gc = g2[2], gd = g2[3];
+  const double gx = ga.x, gy = ga.y, gz = gb.x, nx = gb.y, ny = gc.x, nz = gc.y, gw = gd.x;
+  double bq[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) bq[q] = B[(size_t)ic * pad + 16 * q + m];
+  const double dx = x0 - gx, dy = x1 - gy, dz = x2 - gz;
+  const double rr = dx * dx + dy * dy + dz * dz;
+  const bool ok = use && rr > 1.0e-28;
+  const double rs = ok ? rr : 1.0;
+  // 1/r from the hardware estimate and two Newton steps
+  double y = __builtin_amdgcn_rsq(rs);
+  double t = rs * y;
+  y = fma(0.5 * y, fma(-t, y, 1.0), y);
+  t = rs * y;
+  y = fma(0.5 * y, fma(-t, y, 1.0), y);
+  const double msk = ok ? 1.0 : 0.0;
+  const double cw = msk * (c4 * gw);
+  const double dn = dx * nx + dy * ny + dz * nz;
+  const double s = cw * y;
+  const double d = cw * dn * (y * y * y);
+  // past a segment's last node the A entries are exactly zero and the B row
+  // (the clamped node's) is finite, so B needs no mask
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    accS[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(s, bq[q], accS[q], 0, 0, 0);
+    accD[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(d, bq[q], accD[q], 0, 0, 0);
+  }
+}
+
 template <int NT>
 __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int* __restrict__ lpairs,
                                                          const int* __restrict__ ltot,
@@ -422,37 +461,7 @@ __global__ void __launch_bounds__(64) direct_mfma_kernel(LevelArgs A, const int*
   for (int k0 = 0; k0 < n; k0 += 4) {
     const int i = k0 + kq;
     const bool in = i < n;
-    const int ic = in ? i : n - 1;
-    const double2* g2 = (const double2*)(G + 8 * ic);
-    const double2 ga = g2[0], gb = g2[1], gc = g2[2], gd = g2[3];
-    const double gx = ga.x, gy = ga.y, gz = gb.x, nx = gb.y, ny = gc.x, nz = gc.y, gw = gd.x;
-    double bq[NT];
-#pragma unroll
-    for (int q = 0; q < NT; ++q) bq[q] = B[(size_t)ic * pad + 16 * q + m];
-    const double dx = x0 - gx, dy = x1 - gy, dz = x2 - gz;
-    const double rr = dx * dx + dy * dy + dz * dz;
-    const bool ok = live && in && rr > 1.0e-28;
-    const double rs = ok ? rr : 1.0;
-    // 1/r from the hardware estimate and two Newton steps
-    double y = __builtin_amdgcn_rsq(rs);
-    double t = rs * y;
-    y = fma(0.5 * y, fma(-t, y, 1.0), y);
-    t = rs * y;
-    y = fma(0.5 * y, fma(-t, y, 1.0), y);
-    // every operand is finite (clamped index, rs = 1 when masked), so a 0/1 mask
-    // zeroes the masked lanes without a branch
-    const double msk = ok ? 1.0 : 0.0;
-    const double cw = msk * (c4 * gw);
-    const double dn = dx * nx + dy * ny + dz * nz;
-    const double s = cw * y;
-    const double d = cw * dn * (y * y * y);
-    // past the level's last node the A entries are exactly zero and the B row
-    // (the clamped node's) is finite, so B needs no mask
-#pragma unroll
-    for (int q = 0; q < NT; ++q) {
-      accS[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(s, bq[q], accS[q], 0, 0, 0);
-      accD[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(d, bq[q], accD[q], 0, 0, 0);
-    }
+    direct_step<NT>(G, B, pad, m, in ? i : n - 1, live && in, x0, x1, x2, c4, accS, accD);
   }
   // v_mfma_f64_16x16x4 operands and result: lane l supplies A[l%16][l/16] and
   // B[l/16][l%16] and holds D[4r + l/16][l%16] in slot r, so this lane has
@@ -492,6 +501,175 @@ void launch_direct_mfma(LevelArgs A, const int* lpairs, const int* ltot, const i
     case 9: hipLaunchKernelGGL(direct_mfma_kernel<9>, dim3(grid), dim3(64), 0, s, A, lpairs, ltot, ppanel, tidx, tx, lgeo, wout, lo, hi); break;
     default: throw std::runtime_error("direct level kernel: unsupported patch order");
   }
+}
+
+// ---------------------------------------------------------------------------
+// The reference's adaptive quadtree (PR_ADAPTIVE: adaptive_pass, walk, cut_gI,
+// patch_refine_mod.f90:617-771) on triangle levels.  walk descends from the
+// level-1 cell: a cell whose ball the target is outside of (strict >) is a
+// leaf; otherwise its four children are tested, and a level-3 cell the target
+// is inside of sends the pair to one_level at level 4.  With three tree levels
+// a cut is fixed by which level-2 cells are opened, so the 17 possible cuts
+// are buckets; the reference's gI (cut_gI :721-738) is the leaf cells'
+// interpolation rows, which here stay in the levels' padded table.
+
+// per cut: the leaf cells' first nodes in the concatenated level arrays, in the
+// reference's walk order (level-2 leaves by cell, then the level-3 leaves)
+std::vector<int> adaptive_segments(const LevelArgs& A) {
+  std::vector<int> seg((size_t)kAdaptBuckets * (kAdaptSegs + 1), 0);
+  const int n1 = A.lev_n[0];
+  seg[0] = 1;
+  seg[1] = A.lev_off[0];
+  for (int m = 0; m < 16; ++m) {
+    int* g = seg.data() + (size_t)(1 + m) * (kAdaptSegs + 1);
+    int c = 0;
+    for (int q = 0; q < 4; ++q)
+      if (!((m >> q) & 1)) g[1 + c++] = A.lev_off[1] + q * n1;
+    for (int q = 0; q < 4; ++q)
+      if ((m >> q) & 1)
+        for (int b = 0; b < 4; ++b) g[1 + c++] = A.lev_off[2] + (4 * q + b) * n1;
+    g[0] = c;
+  }
+  return seg;
+}
+
+// acnt: [0, 17) pairs per cut, [17, 34) first slot of each cut, [34, 51) cursors
+__global__ void __launch_bounds__(256) abucket_count_kernel(const int* __restrict__ ltot,
+                                                            const int* __restrict__ lpairs,
+                                                            const int8_t* __restrict__ abucket,
+                                                            int* __restrict__ acnt) {
+  __shared__ int lh[kAdaptBuckets];
+  if (threadIdx.x < kAdaptBuckets) lh[threadIdx.x] = 0;
+  __syncthreads();
+  const int n = ltot[1], base = ltot[0];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(lh + abucket[lpairs[base + i]], 1);
+  __syncthreads();
+  if (threadIdx.x < kAdaptBuckets && lh[threadIdx.x]) atomicAdd(acnt + threadIdx.x, lh[threadIdx.x]);
+}
+
+__global__ void abucket_offsets_kernel(int* __restrict__ acnt) {
+  if (threadIdx.x != 0) return;
+  int s = 0;
+  for (int b = 0; b < kAdaptBuckets; ++b) {
+    acnt[kAdaptBuckets + b] = s;
+    acnt[2 * kAdaptBuckets + b] = s;
+    s += acnt[b];
+  }
+}
+
+// one global range per non-empty cut per workgroup, ranks inside it from LDS;
+// the order inside a cut does not matter (every pair's weights depend on that
+// pair alone)
+__global__ void __launch_bounds__(256) abucket_scatter_kernel(const int* __restrict__ ltot,
+                                                              const int* __restrict__ lpairs,
+                                                              const int8_t* __restrict__ abucket,
+                                                              int* __restrict__ acnt, int* __restrict__ apairs) {
+  __shared__ int lh[kAdaptBuckets], lb[kAdaptBuckets];
+  if (threadIdx.x < kAdaptBuckets) lh[threadIdx.x] = 0;
+  __syncthreads();
+  const int n = ltot[1], base = ltot[0];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int pair = -1, b = 0, r = 0;
+  if (i < n) {
+    pair = lpairs[base + i];
+    b = abucket[pair];
+    r = atomicAdd(lh + b, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kAdaptBuckets)
+    lb[threadIdx.x] = lh[threadIdx.x] ? atomicAdd(acnt + 2 * kAdaptBuckets + threadIdx.x, lh[threadIdx.x]) : 0;
+  __syncthreads();
+  if (pair >= 0) apairs[lb[b] + r] = pair;
+}
+
+// one wave per 16 pairs of one cut: direct_mfma_kernel's step over the cut's
+// leaf cells (n1 nodes each) instead of one level's nodes
+template <int NT>
+__global__ void __launch_bounds__(64) direct_adaptive_kernel(LevelArgs A, const int* __restrict__ acnt,
+                                                             const int* __restrict__ apairs,
+                                                             const int* __restrict__ ppanel,
+                                                             const int* __restrict__ tidx,
+                                                             const double* __restrict__ tx,
+                                                             const double* __restrict__ lgeo,
+                                                             double* __restrict__ wout) {
+  int b = blockIdx.x, bk = -1;
+  for (int q = 0; q < kAdaptBuckets; ++q) {
+    const int ntile = (acnt[q] + 15) / 16;
+    if (b < ntile) { bk = q; break; }
+    b -= ntile;
+  }
+  if (bk < 0) return;
+  const int h = A.h, n1 = A.n1, pad = A.pad;
+  const int first = acnt[kAdaptBuckets + bk] + 16 * b;
+  const int cnt = min(16, acnt[bk] - 16 * b);
+  const int lane = threadIdx.x, m = lane & 15, kq = lane >> 4;
+  const bool live = m < cnt;
+  const int pair = live ? apairs[first + m] : 0;
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+  const double* G = lgeo;
+  if (live) {
+    const size_t t = (size_t)tidx[pair];
+    x0 = tx[3 * t]; x1 = tx[3 * t + 1]; x2 = tx[3 * t + 2];
+    G = lgeo + 8 * (size_t)ppanel[pair] * A.ntot;
+  }
+  const int* sg = A.aseg + bk * (kAdaptSegs + 1);
+  const int nseg = sg[0];
+  const double c4 = 1.0 / (16.0 * atan(1.0));
+  qd4 accS[NT], accD[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    accS[q] = qd4{0.0, 0.0, 0.0, 0.0};
+    accD[q] = qd4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int c = 0; c < nseg; ++c) {
+    const int o = sg[1 + c];
+    for (int k0 = 0; k0 < n1; k0 += 4) {
+      const int i = k0 + kq;
+      const bool in = i < n1;
+      direct_step<NT>(G, A.imp, pad, m, o + (in ? i : n1 - 1), live && in, x0, x1, x2, c4, accS, accD);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = 4 * r + kq;
+    const int pr = __shfl(pair, row, 64);
+    if (row < cnt) {
+      double* o = wout + (size_t)pr * 2 * h;
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        const int col = 16 * q + m;
+        if (col < h) {
+          o[col] = accS[q][r];
+          o[h + col] = accD[q][r];
+        }
+      }
+    }
+  }
+}
+
+void launch_direct_adaptive(LevelArgs A, const int* ltot, const int* lpairs, const int8_t* abucket, int* acnt,
+                            int* apairs, const int* ppanel, const int* tidx, const double* tx, const double* lgeo,
+                            double* wout, int64_t nnz, hipStream_t s) {
+  if (nnz <= 0) return;
+  if (!A.adaptive || !A.aseg) throw std::runtime_error("adaptive direct kernel without adaptive level tables");
+  const dim3 gs((unsigned)((nnz + 255) / 256)), blk(256);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, acnt, 3 * kAdaptBuckets);
+  hipLaunchKernelGGL(abucket_count_kernel, gs, blk, 0, s, ltot, lpairs, abucket, acnt);
+  hipLaunchKernelGGL(abucket_offsets_kernel, dim3(1), dim3(64), 0, s, acnt);
+  hipLaunchKernelGGL(abucket_scatter_kernel, gs, blk, 0, s, ltot, lpairs, abucket, acnt, apairs);
+  const int grid = (int)((nnz + 15) / 16) + kAdaptBuckets;
+#define QOL_ADAPT(NT_)                                                                                       \
+  case NT_:                                                                                                  \
+    hipLaunchKernelGGL(direct_adaptive_kernel<NT_>, dim3(grid), dim3(64), 0, s, A, (const int*)acnt,         \
+                       (const int*)apairs, ppanel, tidx, tx, lgeo, wout);                                    \
+    break;
+  switch ((A.h + 15) / 16) {
+    QOL_ADAPT(1) QOL_ADAPT(2) QOL_ADAPT(3) QOL_ADAPT(4) QOL_ADAPT(5) QOL_ADAPT(6) QOL_ADAPT(7) QOL_ADAPT(8)
+    QOL_ADAPT(9)
+    default: throw std::runtime_error("adaptive direct kernel: unsupported patch order");
+  }
+#undef QOL_ADAPT
 }
 
 }  // namespace qol

@@ -91,7 +91,10 @@ extern "C" void qol_rrq_noalloc_(const int64_t* m, const double* tx, const int64
     if (*distff != 1.4) throw std::runtime_error("distff must be 1.4 (the reference's level radius factor)");
     const int64_t mm = *m;
     if (mm < 0) throw std::runtime_error("m must be non-negative");
-    const size_t need = Engine::one_panel_bytes(p, mm, *ichart);
+    bool ad = false, sq = true;
+    level_mode(&ad, &sq);
+    if (ad && sq) throw std::runtime_error("PR_SQUARE and PR_ADAPTIVE cannot be combined");
+    const size_t need = Engine::one_panel_bytes(p, mm, *ichart, sq);
     if (need > kScratchBytes)
       throw std::runtime_error("qol_rrq_noalloc_: " + std::to_string(mm) + " targets at order " + std::to_string(p) +
                                " need " + std::to_string(need) + " bytes of device scratch; the library holds " +
@@ -105,7 +108,8 @@ extern "C" void qol_rrq_noalloc_(const int64_t* m, const double* tx, const int64
       std::lock_guard<std::mutex> g(d.mu);
       Arena a{d.ptr, kScratchBytes, 0};
       ArenaScope scope(&a);
-      Engine e(p, 1, sx, snx, sw, rts, rps, *ichart, sxbd_chart, rv_chart, false, &d.ss);
+      Engine e(p, 1, sx, snx, sw, rts, rps, *ichart, sxbd_chart, rv_chart, false, &d.ss, sq);
+      e.set_level_mode(ad, sq);
       e.set_targets(mm, tx, false);
       std::vector<int64_t> offs = {0, mm}, idx(mm);
       for (int64_t j = 0; j < mm; ++j) idx[j] = j + 1;

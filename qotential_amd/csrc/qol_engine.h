@@ -45,7 +45,8 @@ struct DeviceTables {
   RRQTables RT{};
   DevBuf d_interp, d_imp, d_impf, d_wt, d_suboff, d_subtyp, d_chunks, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn, d_ktab;
   DevBuf d_corner;  // 3 x h rows interpolating a panel at its corners (selector features)
-  DeviceTables(i64 p, hipStream_t s);
+  DevBuf d_aseg;    // triangle levels: the adaptive tree cuts' leaf cells
+  DeviceTables(i64 p, bool square, hipStream_t s);
 };
 
 // Streams and events an engine borrows instead of creating (and destroying) its
@@ -73,6 +74,8 @@ struct Engine {
   bool timing = true;         // record the stage timing events (off while a step is captured into a graph)
   bool one_stream_run = false;  // run every stage on the main stream (a scheduling choice; same results)
   int solve_variant = 0;      // RRQ solve kernels: 0 default, 1 fused elimination, 2 matrix-core elimination
+  bool square = true;         // level form: square regroup (PR_SQUARE) or triangle sub-patches
+  bool adaptive = false;      // triangle levels served by the adaptive walk (PR_ADAPTIVE)
   hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
   hipEvent_t tev[10][2] = {};
   hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_system = nullptr, ev_solve = nullptr,
@@ -81,6 +84,7 @@ struct Engine {
   DevBuf d_sx, d_snx, d_sw, d_rts, d_rps, d_sxbd, d_rv;
   DevBuf d_tx, d_nball, d_cnt, d_offs, d_tidx;
   DevBuf d_label, d_ppanel, d_nrrq, d_lcount, d_lstart, d_ltot, d_lpairs, d_wout, d_val, d_omega, d_ias;
+  DevBuf d_abucket, d_apairs, d_acnt;  // adaptive levels: tree cut per pair, pairs sorted by cut, cut counts
   DevBuf d_lgeo, d_balls, d_pq, d_w1, d_oe, d_root, d_eorder, d_vt;
   DevBuf d_frame, d_edge, d_bX, d_cx, d_basis, d_Fq, d_Msl, d_Fn, d_G, d_Binv, d_scr;
   DevBuf d_ub, d_ubn, d_u, d_tcount, d_toffs, d_tcur, d_tpairs;
@@ -88,16 +92,17 @@ struct Engine {
 
   Engine(i64 p, i64 npan, const double* sx, const double* snx, const double* sw, const double* rts,
          const double* rps, i64 ichart, const double* sxbd, const double* rv, bool device_ptrs,
-         const StreamSet* borrow = nullptr);
+         const StreamSet* borrow = nullptr, bool square_levels = true);
   // Device bytes one engine of one panel takes for m targets (constructor,
   // set_targets, set_pairs and eval with Omega), each buffer rounded up to 256
   // bytes as an arena slices it.
-  static size_t one_panel_bytes(i64 p, i64 m, i64 ichart);
+  static size_t one_panel_bytes(i64 p, i64 m, i64 ichart, bool square = true);
   // new panel data of the same order, panel count and ichart; targets, pairs
   // and everything evaluated are dropped
   void set_surface(const double* sx, const double* snx, const double* sw, const double* rts, const double* rps,
                    const double* sxbd, const double* rv, bool device_ptrs);
   ~Engine();
+  void set_level_mode(bool adaptive, bool square);
   void set_targets(i64 nt, const double* tx, bool device_ptr);
   i64 nearlist();
   i64 nearlist_grid();
@@ -130,6 +135,10 @@ struct Engine {
 // 714-871): 1 levels, 2 boundary precompute, 3 level direct, 4 SSQ, 5 solid
 // angle (with the projection that orients it), 6 moments and weights, 8 solve
 void add_timeinfo(const double* ms, double* timeinfo);
+
+// the level form the drop-in entry points use (qol_set_level_mode: the
+// reference's module variables PR_ADAPTIVE and PR_SQUARE)
+void level_mode(bool* adaptive, bool* square);
 
 // the selector arguments of a model made by qol_selector_create (mode 2), or
 // the ball rule (mode 0) for a null model

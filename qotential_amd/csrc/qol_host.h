@@ -26,10 +26,13 @@ struct OrderTables {
   std::vector<double> umatr, vmatr;               // h*h
   LevelTable lev[4];
   i64 ntot = 0, nsubtot = 0;                       // sums over levels
+  bool square = true;                              // levels 2-4 regrouped into squares
 };
 
-// Thread-safe, cached per order.  Orders 2..20.
-const OrderTables& order_tables(i64 p);
+// Thread-safe, cached per order and level form.  Orders 2..20.  square = false
+// is the reference's PR_SQUARE = .false. (patch_refine_mod.f90:229-240): every
+// sub-patch of levels 2-4 keeps its triangle nodes.
+const OrderTables& order_tables(i64 p, bool square = true);
 
 // primitives (also used by the workload generator)
 void gl_nodes(i64 n, double* t, double* w, double* D);

@@ -22,7 +22,19 @@ struct LevelArgs {
   double dist2;
   const int* chunks;     // nchunk x {level, first node, end node, first sub-patch, end sub-patch}
   int nchunk;            // (node and sub-patch indices within the level)
+  // the reference's adaptive quadtree (PR_ADAPTIVE, triangle levels only): n1
+  // nodes per sub-patch and, per tree cut, the node offsets of its leaf cells
+  int adaptive = 0;
+  int n1 = 0;
+  const int* aseg = nullptr;  // kAdaptBuckets x (1 + kAdaptSegs): count, then offsets into the levels' nodes
 };
+
+// Tree cuts of the adaptive walk over levels 1-3: cut 0 is level 1 whole, cut
+// 1 + m (m a 4-bit mask) has the level-2 cells not in m as leaves and the four
+// level-3 children of each cell in m.
+constexpr int kAdaptBuckets = 17, kAdaptSegs = 16;
+std::vector<int> adaptive_segments(const LevelArgs& A);
+__global__ void zero_ints_kernel(int* p, int n);
 
 // per-order RRQ tables on the device
 struct RRQTables {
@@ -93,9 +105,10 @@ struct SelectorArgs {
 // the label a selector class serves: 0 none, 1..4 levels, 5 RRQ (class 6 -> level 4)
 __host__ __device__ inline int selector_label(int cls) { return cls == 6 ? 4 : cls; }
 
+// abucket (adaptive levels only): the tree cut of every pair the tree serves (label 2)
 void launch_classify(const LevelArgs& A, const SelectorArgs& S, int npan, const int* offs, const int* tidx,
                      const double* tx, const double* sx, const double* snx, const double* sw, const double* balls,
-                     int8_t* label, int* ppanel, int* nrrq, int* lcount, hipStream_t s);
+                     int8_t* label, int* ppanel, int* nrrq, int* lcount, int8_t* abucket, hipStream_t s);
 size_t classify_smem(const LevelArgs& A, const SelectorArgs& S);
 // per-pair selector features (raw, before normalisation), feat[nnz][10]
 __global__ void pair_features_kernel(LevelArgs A, const double* corner, const int* offs, const int* tidx,
@@ -110,6 +123,12 @@ __global__ void level_scan_kernel(const int* lcount, int npan, int* lstart, int*
 __global__ void compact_kernel(const int* offs, const int8_t* label, const int* lstart, const int* ltot, int npan,
                                int* lpairs);
 // direct S/D of the pairs of levels lo..hi (0-based, inclusive) on FP64 matrix cores
+// the pairs of list 1 (label 2) served by an adaptive tree cut: sorted by cut
+// (acnt: 3 x kAdaptBuckets ints of counts, starts and cursors; apairs: nnz),
+// then direct S/D on the cut's leaf nodes contracted with their interpolation rows
+void launch_direct_adaptive(LevelArgs A, const int* ltot, const int* lpairs, const int8_t* abucket, int* acnt,
+                            int* apairs, const int* ppanel, const int* tidx, const double* tx, const double* lgeo,
+                            double* wout, int64_t nnz, hipStream_t s);
 void launch_direct_mfma(LevelArgs A, const int* lpairs, const int* ltot, const int* ppanel, const int* tidx,
                         const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s, int lo = 0,
                         int hi = 3);

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <stdexcept>
 
 #include "qol_kernels.h"
 
@@ -141,10 +142,6 @@ __device__ __forceinline__ void selector_logits(const uint16_t* __restrict__ W, 
   selector_forward(W, B, xin, h1, h2, out);
 }
 
-void launch_classify(const LevelArgs& A, const SelectorArgs& S, int npan, const int* offs, const int* tidx,
-                     const double* tx, const double* sx, const double* snx, const double* sw, const double* balls,
-                     int8_t* label, int* ppanel, int* nrrq, int* lcount, hipStream_t s);
-
 size_t classify_smem(const LevelArgs& A, const SelectorArgs& S) {
   size_t b = sizeof(double) * 4 * (size_t)A.nsubtot;
   if (S.mode == 2) {
@@ -154,6 +151,39 @@ size_t classify_smem(const LevelArgs& A, const SelectorArgs& S) {
     b += 64 + sizeof(PanelFeat);                                   // panel features (8-byte aligned)
   }
   return b;
+}
+
+// The adaptive walk of one target (patch_refine_mod.f90:740-771, then
+// one_level at level 4, :773-842) over the balls in LDS (ball s at b[4s]):
+// label 1 when the level-1 cell is a leaf, 2 with the tree cut in `cut` when
+// every leaf is at levels 2-3, else 4 (outside every level-4 ball) or 5 (RRQ).
+// The walk's leaf test is d^2 > r^2 and one_level's "inside" is d^2 < r^2, as
+// in the reference, so a target exactly on a ball is opened by the walk and
+// served by level 4.
+__device__ __forceinline__ int adaptive_label(const LevelArgs& A, const double* __restrict__ b, double x0, double x1,
+                                              double x2, int& cut) {
+  auto d2 = [&](int s) {
+    const double e0 = b[4 * s] - x0, e1 = b[4 * s + 1] - x1, e2 = b[4 * s + 2] - x2;
+    return e0 * e0 + e1 * e1 + e2 * e2;
+  };
+  cut = 0;
+  const int s1 = A.sub_base[0];
+  if (d2(s1) > b[4 * s1 + 3]) return 1;
+  int open = 0;
+  bool ok = true;
+  for (int q = 0; q < 4 && ok; ++q) {
+    const int s2 = A.sub_base[1] + q;
+    if (d2(s2) > b[4 * s2 + 3]) continue;
+    open |= 1 << q;
+    for (int c = 0; c < 4; ++c) {
+      const int s3 = A.sub_base[2] + 4 * q + c;
+      if (!(d2(s3) > b[4 * s3 + 3])) { ok = false; break; }
+    }
+  }
+  if (ok) { cut = 1 + open; return 2; }
+  for (int s = A.sub_base[3]; s < A.sub_base[3] + A.lev_nsub[3]; ++s)
+    if (d2(s) < b[4 * s + 3]) return 5;
+  return 4;
 }
 
 // One workgroup per panel: every pair's label (ball rule, forced, or learned),
@@ -167,7 +197,7 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
                                                        const double* __restrict__ sw,
                                                        const double* __restrict__ balls, int8_t* __restrict__ label,
                                                        int* __restrict__ ppanel, int* __restrict__ nrrq,
-                                                       int* __restrict__ lcount) {
+                                                       int* __restrict__ lcount, int8_t* __restrict__ abucket) {
   extern __shared__ double sb[];
   __shared__ int cnt[5];
   // the panel's feature data lives past the network's parameters in dynamic LDS
@@ -197,13 +227,19 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
     const int t = tidx[j];
     const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
     int lab = 5;
-    for (int l = 0; l < 4 && lab == 5; ++l) {
-      bool outside = true;
-      for (int s = A.sub_base[l]; s < A.sub_base[l] + A.lev_nsub[l]; ++s) {
-        const double d0 = sb[4 * s] - x0, d1 = sb[4 * s + 1] - x1, d2 = sb[4 * s + 2] - x2;
-        if (d0 * d0 + d1 * d1 + d2 * d2 < sb[4 * s + 3]) { outside = false; break; }
+    if (!LEARNED && A.adaptive) {
+      int cut = 0;
+      lab = adaptive_label(A, sb, x0, x1, x2, cut);
+      abucket[j] = (int8_t)cut;
+    } else {
+      for (int l = 0; l < 4 && lab == 5; ++l) {
+        bool outside = true;
+        for (int s = A.sub_base[l]; s < A.sub_base[l] + A.lev_nsub[l]; ++s) {
+          const double d0 = sb[4 * s] - x0, d1 = sb[4 * s + 1] - x1, d2 = sb[4 * s + 2] - x2;
+          if (d0 * d0 + d1 * d1 + d2 * d2 < sb[4 * s + 3]) { outside = false; break; }
+        }
+        if (outside) lab = l + 1;
       }
-      if (outside) lab = l + 1;
     }
     if (S.mode == 1) {
       lab = S.forced;
@@ -236,16 +272,18 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
 
 void launch_classify(const LevelArgs& A, const SelectorArgs& S, int npan, const int* offs, const int* tidx,
                      const double* tx, const double* sx, const double* snx, const double* sw, const double* balls,
-                     int8_t* label, int* ppanel, int* nrrq, int* lcount, hipStream_t s) {
+                     int8_t* label, int* ppanel, int* nrrq, int* lcount, int8_t* abucket, hipStream_t s) {
+  if (A.adaptive && (S.mode != 0 || !abucket))
+    throw std::runtime_error("adaptive levels: the ball-rule walk only (no selector, forced or shifted labels)");
   const size_t lds = classify_smem(A, S);
   if (S.mode == 2) {
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void*)classify_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(classify_kernel<true>, dim3(npan), dim3(256), lds, s, A, S, offs, tidx, tx, sx, snx, sw, balls,
-                       label, ppanel, nrrq, lcount);
+                       label, ppanel, nrrq, lcount, abucket);
   } else {
     hipLaunchKernelGGL(classify_kernel<false>, dim3(npan), dim3(256), lds, s, A, S, offs, tidx, tx, sx, snx, sw,
-                       balls, label, ppanel, nrrq, lcount);
+                       balls, label, ppanel, nrrq, lcount, abucket);
   }
 }
 

@@ -102,6 +102,14 @@ class Engine:
         check(lib().qol_engine_eval(self._h, i64(1 if exterior else 0), dbl(side_tol), i64(1 if want_omega else 0)),
               "qol_engine_eval")
 
+    def set_level_mode(self, adaptive=False, square=True):
+        """the reference's PR_ADAPTIVE / PR_SQUARE for this engine: square levels
+        (default), triangle levels (square=False), or the adaptive walk over
+        triangle levels (adaptive=True, square=False); labels then read 1 level 1
+        whole, 2 a tree cut of levels 2-3, 4 level 4, 5 RRQ"""
+        check(lib().qol_engine_set_level_mode(self._h, i64(1 if adaptive else 0), i64(1 if square else 0)),
+              "qol_engine_set_level_mode")
+
     def set_precision(self, split_levels):
         """levels 1..split_levels on the split-precision (FP32 matrix core) direct kernel; 0 = all FP64"""
         check(lib().qol_engine_set_precision(self._h, i64(split_levels)), "qol_engine_set_precision")
@@ -174,8 +182,15 @@ class Engine:
         check(lib().qol_engine_far(self._h, ptr(_f(ub)), ptr(_f(ubn)), ptr(u)), "qol_engine_far")
         return u[: self.nt]
 
+    def level_info(self):
+        """this engine's level sizes (they follow its level form)"""
+        info = np.zeros(12, dtype=np.int64)
+        check(lib().qol_engine_level_info(self._h, ptr(info)), "qol_engine_level_info")
+        return dict(ntot=int(info[0]), nsubtot=int(info[1]), nl=info[2:6].copy(), nsub=info[6:10].copy(),
+                    square=bool(info[10]), adaptive=bool(info[11]))
+
     def level_data(self):
-        info = order_info(self.p)
+        info = self.level_info()
         lg = np.zeros((self.npan, 7, info["ntot"]))
         bl = np.zeros((self.npan, info["nsubtot"], 4))
         check(lib().qol_engine_level_data(self._h, ptr(lg), ptr(bl)), "qol_engine_level_data")

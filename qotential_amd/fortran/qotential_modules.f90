@@ -18,6 +18,11 @@ module qotential_hip_c
   use iso_c_binding
   implicit none
   interface
+    function qol_set_level_mode(adaptive, square) bind(C, name='qol_set_level_mode') result(rc)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: adaptive, square
+      integer(c_int) :: rc
+    end function
     function qol_last_status() bind(C, name='qol_last_status') result(s)
       import :: c_int64_t
       integer(c_int64_t) :: s
@@ -91,9 +96,10 @@ end module qotential_hip_c
 
 module patch_refine_mod
   implicit none
-  ! the reference's level-construction switches (src/patch_refine_mod.f90:51-55);
-  ! this library builds the default levels (square regrouping, ball rule), and
-  ! its drop-ins stop with a message when a caller selects anything else
+  ! the reference's level-construction switches (src/patch_refine_mod.f90:51-55):
+  ! square levels (the default), triangle levels (PR_SQUARE = .false.) and the
+  ! adaptive walk over triangle levels (PR_ADAPTIVE = .true.); every drop-in call
+  ! passes the current values to the library (qol_set_level_mode)
   logical, save :: PR_ADAPTIVE = .false.
   logical, save :: PR_SQUARE = .true.
 end module patch_refine_mod
@@ -106,13 +112,15 @@ module lap3d_close_mod
   public :: rrq_r64, simplex_precomp_r64, getnearquad_lap_rrq_r64, Lap3dDLP_closepanel_r64
 contains
 
-  subroutine require_default_levels()
+  ! the module variables to the library; both set stops, as the reference does
+  ! (src/patch_refine_mod.f90:116-117)
+  subroutine pass_level_mode()
     use patch_refine_mod, only: PR_ADAPTIVE, PR_SQUARE
-    if (PR_ADAPTIVE .or. .not. PR_SQUARE) then
-      write(*, '(a)') 'qotential HIP: PR_ADAPTIVE = .true. or PR_SQUARE = .false. is not built (default levels only)'
-      error stop 'unsupported patch_refine_mod setting'
-    end if
-  end subroutine require_default_levels
+    integer(c_int) :: rc
+    if (PR_ADAPTIVE .and. PR_SQUARE) error stop 'PR_SQUARE and PR_ADAPTIVE cannot be combined'
+    rc = qol_set_level_mode(merge(1_c_int64_t, 0_c_int64_t, PR_ADAPTIVE), merge(1_c_int64_t, 0_c_int64_t, PR_SQUARE))
+    call qol_check('qol_set_level_mode')
+  end subroutine pass_level_mode
 
   ! src/lap3d_close_mod.f90:115-133
   subroutine simplex_precomp_r64(nquad, korder, kpols, tgl, wgl, Dgl, w_bclag, Legmat, umatr, vmatr)
@@ -148,7 +156,7 @@ contains
                                  vmatr(order*(order+1)/2,order*(order+1)/2)
     integer(c_int64_t) :: cm, cn, co, cq, cf, ce, cs, ci
     real(c_double) :: cd
-    call require_default_levels()
+    call pass_level_mode()
     cm = m; cn = n; co = order; cq = nquad; cf = orderff; cs = isimd; ci = ichart; cd = distff
     ce = merge(1_c_int64_t, 0_c_int64_t, exterior)
     call qol_rrq_mex(cm, tx, cn, sx, snx, sw, rts, rps, co, cq, cf, cd, ce, cs, ci, sxbd_chart, rv_chart, &
@@ -169,7 +177,7 @@ contains
     real(r64),  intent(inout) :: timeinfo(20)
     integer(c_int64_t) :: a1, a2, a3, a4, a5, a6, a7, a8, a9, a10
     real(c_double) :: cd
-    call require_default_levels()
+    call pass_level_mode()
     a1 = npan; a2 = nterms; a3 = hdim; a4 = nquad; a5 = nsrc; a6 = ntcx; a7 = orderff; a8 = iside
     a9 = isimd; a10 = nnz; cd = distff
     call qol_getnearquad_lap_rrq_mex(a1, a2, a3, a4, a5, sx, snx, sw, rts, rps, a6, tcx, tcxrow, tcxi, a7, cd, &

@@ -31,6 +31,19 @@ def _status(what):
         raise QolError(f"{what}: {lib().qol_last_error().decode()}")
 
 
+def set_level_mode(adaptive=False, square=True):
+    """PR_ADAPTIVE / PR_SQUARE (src/patch_refine_mod.f90:51-55) for every later
+    drop-in call; both true is refused at the next evaluation, as the reference stops"""
+    lib().qol_set_level_mode(i64(1 if adaptive else 0), i64(1 if square else 0))
+    _status("qol_set_level_mode")
+
+
+def level_mode():
+    a, s = C.c_int64(), C.c_int64()
+    lib().qol_get_level_mode(C.byref(a), C.byref(s))
+    return bool(a.value), bool(s.value)
+
+
 def simplex_precomp(nquad, korder):
     kp = (korder + 1) * (korder + 2) // 2
     out = dict(tgl=np.zeros(nquad), wgl=np.zeros(nquad), Dgl=np.zeros((nquad, nquad), order="F"),

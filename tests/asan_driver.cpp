@@ -18,7 +18,8 @@ i64 orc_nearlist_count_mt(i64 npan, i64 hdim, const double* sx, const double* sw
 void orc_nearlist_fill_mt(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt, const double* tx,
                           const i64* offs, i64* idx, i64 nthreads);
 void orc_labels(i64 npan, i64 order, const double* sx, const double* rts, const double* rps, const double* tx,
-                const i64* offs, const i64* idx, i64* label, i64 nthreads);
+                const i64* offs, const i64* idx, i64* label, i64* cut, i64 nthreads);
+i64 orc_set_level_mode(i64 adaptive, i64 square);
 }
 
 namespace qol {
@@ -29,10 +30,11 @@ std::vector<double> curvature_tails_gpu(const Surface&, const std::vector<Chart>
 
 int main() {
   // per-order tables for every supported order
-  for (int p = 2; p <= 12; ++p) {
-    const qol::OrderTables& T = qol::order_tables(p);
-    if (T.h != p * (p + 1) / 2) return 1;
-  }
+  for (int p = 2; p <= 12; ++p)
+    for (bool square : {true, false}) {
+      const qol::OrderTables& T = qol::order_tables(p, square);
+      if (T.h != p * (p + 1) / 2) return 1;
+    }
   // geometry: built-in surface, VMEC surface, refined W7-X-like charts
   const double modes[3][4] = {{0, 0, 1.0, 0.0}, {1, 0, 0.3, 0.3}, {1, 1, 0.06, -0.06}};
   for (int p : {4, 8}) {
@@ -55,8 +57,11 @@ int main() {
     orc_nearlist_count_mt(n, h, sx.data(), sw.data(), nt, tx.data(), offs.data(), 2);
     std::vector<int64_t> idx(offs[n] > 0 ? offs[n] : 1);
     orc_nearlist_fill_mt(n, h, sx.data(), sw.data(), nt, tx.data(), offs.data(), idx.data(), 2);
-    std::vector<int64_t> lab(idx.size());
-    orc_labels(n, p, sx.data(), rts.data(), rps.data(), tx.data(), offs.data(), idx.data(), lab.data(), 2);
+    std::vector<int64_t> lab(idx.size()), cut(idx.size());
+    // square levels, then the adaptive walk on triangle levels
+    for (int mode = 0; mode < 2; ++mode) {
+    orc_set_level_mode(mode, 1 - mode);
+    orc_labels(n, p, sx.data(), rts.data(), rps.data(), tx.data(), offs.data(), idx.data(), lab.data(), cut.data(), 2);
     for (int64_t k = 0; k < n; k += n / 5) {
       const int64_t m = offs[k + 1] - offs[k];
       if (m == 0) continue;
@@ -74,6 +79,8 @@ int main() {
       for (double v : As)
         if (!std::isfinite(v)) return 6;
     }
+    }
+    orc_set_level_mode(0, 1);
   }
   // adaptive chart refinement on the host
   std::vector<double> cl(6 * 200000);

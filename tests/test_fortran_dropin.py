@@ -3,7 +3,7 @@ compiles unchanged against this library's Fortran modules
 (qotential_amd/fortran/qotential_modules.f90: quatapproximation_mod,
 stellarator_mesh_mod, lap3d_close_mod, patch_refine_mod) and links to
 libqotential_hip.so; on a GPU the driver built by oracle/fortran_bench.mk runs
-its first case through the drop-in."""
+its first case through the drop-in with each of its three level forms."""
 import os
 import shutil
 import subprocess
@@ -30,12 +30,17 @@ def test_reference_bench_compiles_unchanged(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not BUILT.exists(), reason="oracle/_ref/stellarator_bench not built (no reference checkout)")
-def test_reference_bench_runs_on_the_gpu():
-    print("running the reference's stellarator_bench (case 1: order 4, 24x72) through the drop-in", flush=True)
-    r = subprocess.run([str(BUILT), "1"], capture_output=True, text=True, timeout=600)
+@pytest.mark.parametrize("iadap", [0, 3, 1])
+def test_reference_bench_runs_on_the_gpu(iadap):
+    """case 1 (order 4, 24x72 charts) with the default levels (iadap = 0),
+    triangle levels (3) and the adaptive walk (1): the driver sets PR_SQUARE and
+    PR_ADAPTIVE from iadap (test/stellarator_bench.f90:46-47)"""
+    print(f"running the reference's stellarator_bench (case 1: order 4, 24x72, iadap {iadap}) through the drop-in",
+          flush=True)
+    r = subprocess.run([str(BUILT), "1", "0", str(iadap)], capture_output=True, text=True, timeout=600)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
     out = r.stdout
-    assert "nterms =   4" in out and "target pps" in out
+    assert "nterms =   4" in out and "target pps" in out and f"iadap = {iadap}" in out
     tt = float(out.split("target time")[1].split()[0])
     assert tt > 0.0
