@@ -10,10 +10,10 @@
 // call taking that panel's own order (src/lap3d_close_mod.f90:622-649 inside the
 // OpenMP loop of stellarator_grf_core_mod.f90:759-888).  Here each order's
 // panels form one engine (work bucketed by order, and inside an engine by
-// level label); the graph holds every bucket's kernels with their stream
-// dependencies, so the buckets overlap on the device without host launches.
+// level label); one call launches every bucket's kernels with their stream
+// dependencies, so the buckets overlap on the device with no host wait.
 //
-// Autotuner: for each engine, the candidates are the four-stream and the
+// Autotuner: for each engine, the candidates are the three-stream and the
 // one-stream schedule and the RRQ solve variants (register-resident inverses
 // and the fused kernel at every order, the matrix-core form up to order 8); each
 // candidate's single-engine step is timed over a few steps and the fastest

@@ -208,14 +208,19 @@ Engine::Engine(i64 p_, i64 npan_, const double* sx, const double* snx, const dou
   } else {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // QOL_ONE_STREAM=1 runs every stage on the main stream (kernel times without
-    // overlap, for profiling); by default the stages overlap on four streams
+    // overlap, for profiling); by default the stages overlap on three streams
     const char* one = std::getenv("QOL_ONE_STREAM");
     if (one && one[0] == '1') {
       s1 = s2 = s3 = stream;
     } else {
+      // three streams: the per-pair edge chain (s3 role) runs on the main
+      // stream, which has nothing else to do between the pair lists and the
+      // moments.  A fourth stream would share a hardware queue with one of the
+      // others (HIP spreads streams over the process's few queues in creation
+      // order), and the edge chain then waited behind the solve.
       HIPCHK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
       HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-      HIPCHK(hipStreamCreateWithFlags(&s3, hipStreamNonBlocking));
+      s3 = stream;
     }
     for (auto& st : tev)
       for (auto& e : st) HIPCHK(hipEventCreate(&e));
@@ -583,10 +588,11 @@ void Engine::eval_levels_only() {
 // (frame kernel, then system kernel) and solve need only the surface, so they
 // run for every panel on s2 from the start, beside the level geometry and
 // classification on the main stream.  After the pair lists: the per-pair
-// projection, root/SSQ and solid angle on s3 (after the frame kernel), and the
+// projection, root/SSQ and solid angle in the s3 role (the main stream itself
+// for an engine that makes its own streams; after the frame kernel), and the
 // direct levels on s1 (matrix cores) once the projection is done; the moments
-// join s3 and the system kernel, the weights the solve, and the main stream
-// joins s1 last.
+// join the edge chain and the system kernel, the weights the solve, and the
+// main stream joins s1 last.
 // tev[i] brackets stage i: levels, classify+lists, direct, rrq setup, rrq solve,
 // projection, root + SSQ (solid angle from tev[8][0]), moments, -, weights.
 void Engine::eval(int exterior, double side_tol, bool want_omega) {
@@ -596,7 +602,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   auto rec = [&](int i, int j, hipStream_t s) {
     if (timing) HIPCHK(hipEventRecord(tev[i][j], s));
   };
-  // the four stream roles, or the main stream for all of them
+  // the stream roles (s3 is the main stream unless borrowed), or the main stream for all of them
   hipStream_t s1 = one_stream_run ? stream : this->s1, s2 = one_stream_run ? stream : this->s2,
               s3 = one_stream_run ? stream : this->s3;
   if (nnz <= 0) {

@@ -5,7 +5,7 @@ Against the oracle at full size: the near-list CSR and every pair's label bit
 for bit, the order-8 level nodes and balls of a sample of panels bit for bit,
 and that sample's weights.  Size-independent properties: the Green's
 representation identity on every target, bit-identical repeated corrections,
-and the one-stream schedule giving the four-stream result bit for bit."""
+and the one-stream schedule giving the three-stream result bit for bit."""
 import os
 
 import numpy as np
@@ -125,7 +125,7 @@ def _run(geo, tx, ub, ubn):
 
 def test_bench_workload_identity_and_determinism(bench_case, monkeypatch):
     geo, tx, dist, pos, q, ub, ubn = bench_case
-    say(f"full workload: {geo['ntri']} triangles, {tx.shape[0]} targets, four streams")
+    say(f"full workload: {geo['ntri']} triangles, {tx.shape[0]} targets, three streams")
     nnz, lab, u, val, val2 = _run(geo, tx, ub, ubn)
     say("pairs", nnz, "labels L1..L4, RRQ", lab.tolist())
     assert nnz == 1939854 and lab.sum() == nnz and (lab > 0).all()
@@ -142,3 +142,48 @@ def test_bench_workload_identity_and_determinism(bench_case, monkeypatch):
     monkeypatch.delenv("QOL_ONE_STREAM")
     assert nnz1 == nnz and np.array_equal(lab1, lab)
     assert np.array_equal(val1, val) and np.array_equal(u1, u)
+
+
+def test_bench_workload_grf_decomposition(bench_case):
+    """What sets the Green's-identity error of the bench workload (5.4e-8): the
+    pairs the ball rule serves at level 1, farthest from the surface.  Serving
+    every level pair one level finer halves the maximum; adding every target
+    within 3 node radii of a panel to that panel's near list (so the smooth rule
+    covers less) changes nothing."""
+    from scipy.spatial import cKDTree
+    geo, tx, dist, pos, q, ub, ubn = bench_case
+    h, npan = geo["h"], geo["ntri"]
+    uex, _ = W.charge_field(tx, pos, q)
+    ref = np.where(dist < 0.0, uex, 0.0)
+    scale = np.abs(uex).max()
+    e = Engine(geo)
+    e.set_targets(tx)
+    e.nearlist()
+    offs, idx = e.pairs()
+    far = e.far(ub, ubn)
+
+    def grf():
+        e.eval(exterior=True, side_tol=1e-10)
+        u, _ = e.apply(ub, ubn, far)
+        return np.abs(u - ref) / scale
+
+    err = grf()
+    k = int(np.argmax(err))
+    e.refine_levels(1)
+    finer = grf().max()
+    e.refine_levels(0)
+    sx = geo["sx"].reshape(npan, h, 3)
+    c = sx.mean(1)
+    r = np.sqrt(((sx - c[:, None, :]) ** 2).sum(2).max(1))
+    extra = cKDTree(tx).query_ball_point(c, 3.0 * r)
+    lists = [np.union1d(idx[offs[j]:offs[j + 1]] - 1, np.asarray(extra[j], dtype=np.int64)) for j in range(npan)]
+    w_offs = np.concatenate([[0], np.cumsum([len(x) for x in lists])]).astype(np.int64)
+    e.set_pairs(w_offs, np.concatenate(lists) + 1)
+    wide = grf().max()
+    e.close()
+    say(f"GRF max {err.max():.3e} (worst target at |d| = {abs(dist[k]):.1e}); one level finer {finer:.3e}; "
+        f"near list to 3 node radii ({int(w_offs[-1])} pairs) {wide:.3e}")
+    assert err.max() < 1e-7
+    assert finer < 0.6 * err.max()
+    assert abs(wide - err.max()) < 0.01 * err.max()
+

@@ -63,7 +63,7 @@ def test_solve_variants_and_single_stream(p, monkeypatch):
     matrix-core quaternion solve (QOL_SOLVE_MFMA=1) agree with the default
     register-resident inverses (at orders 10 and 12 spread over 4-8 waves, with
     G finished by a separate epilogue kernel) to the conditioning of the system,
-    and the one-stream schedule (QOL_ONE_STREAM=1) gives the four-stream result
+    and the one-stream schedule (QOL_ONE_STREAM=1) gives the three-stream result
     bit for bit."""
     print(f"solve variants and one-stream schedule, VMEC surface 12x30, p={p}", flush=True)
     geo = W.geometry(12, 30, p, modes=W.VMEC_BENCH)
