@@ -1280,10 +1280,11 @@ __global__ void __launch_bounds__(64 * NW) rrq_quat_inverse_kernel(RRQTables T, 
 
 // The rest of G after rrq_quat_inverse_kernel<..., false> has left X in its Ad
 // rows: the same epilogue as the fused form (so the same results), one
-// workgroup of 8 waves per panel (its 3h^2 doubles of LDS leave one workgroup
-// per CU at h = 66 and 78, so the waves hide the LDS latency), X read back one
-// component at a time and the Ad rows scaled in place.
-__global__ void __launch_bounds__(512) rrq_quat_epilogue_kernel(RRQTables T, RRQPanels P, int npan,
+// workgroup of 16 waves per panel (its 3h^2 doubles of LDS leave one workgroup
+// per CU at h = 66 and 78, so the waves hide the LDS latency: 256, 512 and 1024
+// threads took 4.3, 2.8 and 2.0 ms at order 12), X read back one component at a
+// time and the Ad rows scaled in place.
+__global__ void __launch_bounds__(1024) rrq_quat_epilogue_kernel(RRQTables T, RRQPanels P, int npan,
                                                                 const int* __restrict__ nrrq) {
   const int k = blockIdx.x;
   if (nrrq && nrrq[k] == 0) return;
@@ -1498,7 +1499,7 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
   hipLaunchKernelGGL((rrq_real_inverse_kernel<RL_, RA_, CL_, CB_>), dim3(npan), dim3(64), 0, s, T, P, npan, nrrq); \
   hipLaunchKernelGGL((rrq_quat_inverse_kernel<RL_, RA_, CL_, CB_, NW_, false>), dim3(npan), dim3(64 * NW_), 0, s, T, \
                      P, npan, nrrq);                                                                              \
-  hipLaunchKernelGGL(rrq_quat_epilogue_kernel, dim3(npan), dim3(512), el, s, T, P, npan, nrrq)
+  hipLaunchKernelGGL(rrq_quat_epilogue_kernel, dim3(npan), dim3(1024), el, s, T, P, npan, nrrq)
     if (h <= 45) { QOL_QINV_SPLIT(8, 6, 8, 6, 3); }
     else if (h <= 55) { QOL_QINV_SPLIT(9, 7, 7, 8, 4); }
     else if (h <= 66) { QOL_QINV_SPLIT(10, 7, 6, 12, 6); }

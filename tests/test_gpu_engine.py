@@ -106,3 +106,43 @@ def test_fused_corrections_match_apply(eng4, geo4):
     u_ref, val_ref = eng4.apply(ub, ubn, far)
     assert np.array_equal(val_f, val_ref)
     assert np.array_equal(u_f, u_ref)
+
+
+def test_config1_interior_targets_vs_oracle(oracle):
+    """The CPU reference configuration on the GPU: VMEC surface 16x40 charts,
+    order 4, 4,000 interior targets at h 10^U, U ~ U[-6,-1], point-charge
+    density.  The potential equals the oracle's (near list, labels, close
+    corrections, far field) to 1e-11 of max |u|, and the Green's identity holds
+    to the order-4 level (u = the charges' field inside)."""
+    say("config 1: 16x40 p=4, 4000 interior targets")
+    geo = W.geometry(16, 40, 4, modes=W.VMEC_BENCH)
+    tx, tri, dist = W.targets(geo, 4000, -6.0, -1.0, sides=-1, seed=1)
+    assert (dist < 0).all()
+    pos, q = W.point_charges(geo, nchg=8, offset=0.4, seed=2)
+    ub, gu = W.charge_field(geo["sx"], pos, q)
+    ubn = (geo["snx"] * gu).sum(1)
+    e = Engine(geo)
+    e.set_targets(tx)
+    nnz = e.nearlist()
+    offs, idx = e.pairs()
+    o_offs, o_idx = oracle.nearlist(geo["h"], geo["sx"], geo["sw"], tx, nthreads=8)
+    assert np.array_equal(offs, o_offs) and np.array_equal(idx, o_idx)
+    far = e.far(ub, ubn)
+    e.eval(exterior=True, side_tol=1e-10)
+    u, _ = e.apply(ub, ubn, far)
+    lab = e.weights()["label"]
+    e.close()
+    uo = oracle.far_direct(tx, geo["sx"], geo["snx"], geo["sw"], ub, ubn, nthreads=8)
+    oracle.set_side_tol(1e-10)
+    try:
+        oracle.close_apply(4, geo, tx, o_offs, o_idx, ub, ubn, exterior=True, u=uo, nthreads=8)
+    finally:
+        oracle.set_side_tol(1e300)
+    uex, _ = W.charge_field(tx, pos, q)
+    diff = np.abs(u - uo).max() / np.abs(uo).max()
+    grf = np.abs(u - uex).max() / np.abs(uex).max()
+    say(f"  {nnz} pairs, labels L1..L4, RRQ {np.bincount(lab, minlength=6)[1:].tolist()}; "
+        f"GPU vs oracle {diff:.2e}; GRF {grf:.3e}")
+    assert nnz == 40357
+    assert diff < 1e-11
+    assert grf < 5e-2

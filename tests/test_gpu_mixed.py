@@ -56,13 +56,54 @@ def test_mixed_orders_grf():
     assert res[("p8", -8)] < 1e-5
 
 
-def test_mixed_full_size():
+# density action of sampled RRQ weights against the oracle, by order (the
+# quaternion system's conditioning grows with the order; as in the per-order tests)
+MIXED_ACT_TOL = {4: 1e-10, 6: 1e-10, 8: 1e-10, 10: 1e-9, 12: 4e-8}
+
+
+def _check_sample_vs_oracle(oracle, p, g, tx, e, pos, q):
+    """labels, level nodes and weights of the three busiest panels of one order"""
+    h = g["h"]
+    offs, idx = e.pairs()
+    w = e.weights()
+    lg, bl = e.level_data()
+    ub, gu = W.charge_field(g["sx"], pos, q)
+    ubn = (g["snx"] * gu).sum(1)
+    oracle.set_side_tol(1e-10)
+    worst = [0.0, 0.0]
+    try:
+        for k in np.argsort(np.diff(offs))[-3:]:
+            a, b = offs[k], offs[k + 1]
+            sl = slice(k * h, (k + 1) * h)
+            L = oracle.patch_levels(p, g["sx"][sl], g["rts"][sl], g["rps"][sl])
+            assert np.array_equal(lg[k, 0:3].T, L["sx"]) and np.array_equal(bl[k, :, 3], L["qradii2"]), (p, k)
+            r = oracle.rrq(p, tx[idx[a:b] - 1], g["sx"][sl], g["snx"][sl], g["sw"][sl], g["rts"][sl], g["rps"][sl],
+                           exterior=True)
+            assert np.array_equal(w["label"][a:b], r["label"]), (p, k)
+            lv = r["label"] <= 4
+            for key, dens in (("As", ubn[sl]), ("Ad", ub[sl])):
+                gk = w[key][a:b]
+                if lv.any():
+                    err = np.abs(gk[lv] - r[key][lv]) / np.abs(r[key][lv]).max(axis=1, keepdims=True)
+                    worst[0] = max(worst[0], err.max())
+                if (~lv).any():
+                    act = np.abs(gk[~lv] @ dens - r[key][~lv] @ dens) / (np.abs(r[key][~lv]) @ np.abs(dens))
+                    worst[1] = max(worst[1], act.max())
+    finally:
+        oracle.set_side_tol(1e300)
+    print(f"    vs oracle on 3 panels: level weights {worst[0]:.2e}, RRQ action {worst[1]:.2e}", flush=True)
+    assert worst[0] < 1e-13 and worst[1] < MIXED_ACT_TOL[p]
+
+
+def test_mixed_full_size(oracle):
     """The mixed-order workload at full size: 64x160 charts of the bench surface,
     orders 4, 6, 8, 10, 12 drawn per chart, 1,000,000 targets at h 10^U,
     U in [-8,-1], both sides.  Every order's near list, levels and RRQ run on the
-    device; the per-order evaluation is timed (inputs resident) and the GRF error
-    of the whole surface is bounded by the uniform order-4 figure measured on the
-    16x40 surface."""
+    device; the per-order evaluation is timed (inputs resident), each order's
+    labels, level nodes and weights are checked against the oracle on its three
+    busiest panels (a check that sees quadrature errors far below the seams'
+    effect on the identity), and the GRF error of the whole surface is bounded by
+    the uniform order-4 figure measured on the 16x40 surface."""
     import time
     from qotential_amd.engine import Engine
     print("mixed orders 4..12, 64x160 charts, 1M targets", flush=True)
@@ -87,6 +128,7 @@ def test_mixed_full_size():
         npairs += nnz
         t_eval += dt
         print(f"  order {p}: {g['ntri']} triangles, {nnz} pairs, eval {1e3 * dt:.1f} ms", flush=True)
+        _check_sample_vs_oracle(oracle, p, g, tx, e, pos, q)
     # every order launched before any is waited for: the orders overlap
     t0 = time.perf_counter()
     for e in engines.values():
