@@ -56,9 +56,11 @@ def test_mixed_orders_grf():
     assert res[("p8", -8)] < 1e-5
 
 
-# density action of sampled RRQ weights against the oracle, by order (the
-# quaternion system's conditioning grows with the order; as in the per-order tests)
-MIXED_ACT_TOL = {4: 1e-10, 6: 1e-10, 8: 1e-10, 10: 1e-9, 12: 4e-8}
+# density action of the busiest panels' RRQ weights against the oracle, by order:
+# measured 4.6e-13, 2.7e-12, 2.5e-11, 5.3e-10, 4.1e-8 at p = 4..12 (hundreds of
+# targets per panel down to h = 1e-8; the quaternion system's condition number,
+# about 2e10 at p = 12, times the FP64 unit roundoff is 2e-6 per weight); bounds 10x
+MIXED_ACT_TOL = {4: 5e-12, 6: 3e-11, 8: 3e-10, 10: 5e-9, 12: 4e-7}
 
 
 def _check_sample_vs_oracle(oracle, p, g, tx, e, pos, q):
