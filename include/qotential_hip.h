@@ -21,6 +21,16 @@ extern "C" {
 
 const char* qol_last_error(void);
 int64_t qol_last_status(void);
+/* RRQ pairs of this thread's last qol_rrq_mex_, qol_rrq_noalloc_ or
+ * qol_getnearquad_lap_rrq_mex_ call whose Gauss-Newton projection onto the patch
+ * had not stopped by its 40th and last step.  It stops when the step |du| + |dv|
+ * falls below 1e-15 or below eight times the step that rounding of the residual
+ * makes (8 eps |x| / the smallest singular value of the surface Jacobian, bounded
+ * above); such a pair keeps its last iterate as its foot point.  *max_step (if not
+ * NULL) receives the largest of those last steps, in units of the reference
+ * triangle.  No counterpart in the reference, whose drop-ins report nothing of
+ * the kind. */
+int64_t qol_last_projection_unconverged(double* max_step);
 
 /* ---------------- workload generation (host; qol_geo_build_gpu on the GPU) ----------------
  * Replaces stellarator_geo_ntri / stellarator_geo / stellarator_geo_charts
@@ -223,6 +233,11 @@ int qol_engine_sync(void* engine);
  * rrq moments + recursion, rrq solid angle, rrq weights.  Stages on different
  * streams overlap. */
 int qol_engine_stage_ms(void* engine, double* ms);
+/* RRQ pairs of the last eval whose Gauss-Newton projection had not stopped by
+ * its 40th and last step (they keep that last iterate), and in *max_step (if not
+ * NULL) the largest such last step, as for qol_last_projection_unconverged;
+ * waits for the engine's stream.  No counterpart in the reference. */
+int qol_engine_projection_unconverged(void* engine, int64_t* count, double* max_step);
 int qol_engine_get_weights(void* engine, double* As, double* Ad, int64_t* label, double* Omega,
                            double* Ias);
 /* near-correction apply + fixed-order scatter into u (in: far field) */

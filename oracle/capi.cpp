@@ -19,6 +19,10 @@ extern "C" {
 
 void orc_set_side_tol(double tol) { set_side_tol(tol); }
 
+/* Gauss-Newton projections that had not stopped by the step cap since the last reset,
+   and the largest last step among them */
+i64 orc_projection_unconverged(i64 reset, double* max_step) { return projection_unconverged(reset != 0, max_step); }
+
 /* PR_ADAPTIVE / PR_SQUARE; returns 0, or 1 (nothing changed) for both set */
 i64 orc_set_level_mode(i64 adaptive, i64 square) { return set_level_mode(adaptive != 0, square != 0) ? 0 : 1; }
 

@@ -13,6 +13,9 @@ import numpy as np
 
 _HERE = Path(__file__).resolve().parent
 _LIB = None
+_LIB_FMA = None
+_SIDE_TOL = 1e300          # the side tolerance and level form, applied to both builds
+_LEVEL_MODE = (False, True)
 
 i64 = C.c_int64
 dbl = C.c_double
@@ -42,15 +45,47 @@ def _f(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def lib_fma():
+    """the oracle compiled with fused multiply-adds (liboracle_fma.so): the same
+    algorithm rounded differently, for measuring a result's rounding sensitivity"""
+    global _LIB_FMA
+    if _LIB_FMA is None:
+        path = _HERE / "liboracle_fma.so"
+        if not path.exists():
+            build()
+        _LIB_FMA = C.CDLL(str(path))
+        _LIB_FMA.orc_set_side_tol(dbl(_SIDE_TOL))
+        _LIB_FMA.orc_set_level_mode(i64(1 if _LEVEL_MODE[0] else 0), i64(1 if _LEVEL_MODE[1] else 0))
+    return _LIB_FMA
+
+
 def set_side_tol(tol):
+    global _SIDE_TOL
+    _SIDE_TOL = tol
     lib().orc_set_side_tol(dbl(tol))
+    if _LIB_FMA is not None:
+        _LIB_FMA.orc_set_side_tol(dbl(tol))
+
+
+def projection_unconverged(reset=True):
+    """(count, largest last step): Gauss-Newton projections that had not
+    stopped by their 40th step, since the last reset"""
+    f = lib().orc_projection_unconverged
+    f.restype = C.c_int64
+    st = C.c_double()
+    n = int(f(C.c_int64(1 if reset else 0), C.byref(st)))
+    return n, st.value
 
 
 def set_level_mode(adaptive=False, square=True):
     """PR_ADAPTIVE / PR_SQUARE for every later call (both true is refused)"""
+    global _LEVEL_MODE
     lib().orc_set_level_mode.restype = i64
     if lib().orc_set_level_mode(i64(1 if adaptive else 0), i64(1 if square else 0)) != 0:
         raise ValueError("PR_SQUARE and PR_ADAPTIVE cannot be combined")
+    _LEVEL_MODE = (adaptive, square)
+    if _LIB_FMA is not None:
+        _LIB_FMA.orc_set_level_mode(i64(1 if adaptive else 0), i64(1 if square else 0))
 
 
 def gauss(n):
@@ -128,7 +163,9 @@ def levels_eval(order, tx, sx, rts, rps, dist=1.4):
 
 
 def rrq(order, tx, sx, snx, sw, rts, rps, exterior=True, ichart=0, sxbd_chart=None, rv_chart=None,
-        timeinfo=None):
+        timeinfo=None, fma=False):
+    """rrq_r64 for m targets against one panel; fma=True runs the build with fused
+    multiply-adds (lib_fma)"""
     tx = _f(tx); m = tx.shape[0]; h = order * (order + 1) // 2; nq = order + 2
     pre = simplex_precomp(nq, order - 1)
     As = np.zeros((m, h), order="F"); Ad = np.zeros((m, h), order="F")
@@ -136,7 +173,8 @@ def rrq(order, tx, sx, snx, sw, rts, rps, exterior=True, ichart=0, sxbd_chart=No
     ti = np.zeros(20) if timeinfo is None else timeinfo
     sxbd = np.zeros((3 * nq, 3)) if sxbd_chart is None else _f(sxbd_chart)
     rv = np.zeros((3, 3)) if rv_chart is None else _f(rv_chart)
-    lib().orc_rrq(i64(m), _p(tx), i64(h), _p(_f(sx)), _p(_f(snx)), _p(_f(sw)), _p(_f(rts)), _p(_f(rps)),
+    (lib_fma() if fma else lib()).orc_rrq(
+                  i64(m), _p(tx), i64(h), _p(_f(sx)), _p(_f(snx)), _p(_f(sw)), _p(_f(rts)), _p(_f(rps)),
                   i64(order), i64(nq), i64(order + 2), dbl(1.4), i64(1 if exterior else 0), i64(ichart),
                   _p(sxbd), _p(rv), _p(pre["tgl"]), _p(pre["wgl"]), _p(pre["Dgl"]), _p(pre["w_bclag"]),
                   _p(pre["Legmat"]), _p(pre["umatr"]), _p(pre["vmatr"]), _p(As), _p(Ad), _p(Om), _p(Ia),

@@ -111,5 +111,8 @@ void rrq(i64 m, const double* tx, i64 n, const double* sx, const double* snx,
          const double* w_bclag, const double* Legmat, const double* umatr,
          const double* vmatr, double* As, double* Ad, double* Omega, double* Ias,
          double* timeinfo, i64* label);
+/* projections since the last reset that had not stopped by their 40th step,
+   and (max_step) the largest last step |du| + |dv| among them */
+i64 projection_unconverged(bool reset, double* max_step);
 
 }  // namespace orc

@@ -27,6 +27,7 @@
  */
 #include "qol_oracle.h"
 #include <algorithm>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -367,7 +368,11 @@ static void setup_panel(Panel& P, i64 n, const double* sx, const double* snx, co
 
 /* ---------- per-target pieces ---------- */
 
-/* complex root of |X_e(s) - x|^2 near [-1,1]; returns false if Newton fails */
+/* complex root of |X_e(s) - x|^2 near [-1,1]; returns false if Newton fails.
+   The start is the root of the chord's |C + s D|^2: s = (-C.D + i |C x D|) / D.D.
+   Its imaginary part is taken from the cross product: the equal form
+   sqrt(|C|^2 |D|^2 - (C.D)^2) cancels for a target near the chord's line (near a
+   vertex), can round to 0 there, and Newton from a real start stays real. */
 static bool edge_root(const Edge& E, i64 nq, const double* x, cplx& s0) {
   double A[3], B[3], D[3], C[3];
   for (int c = 0; c < 3; ++c) {
@@ -379,8 +384,9 @@ static bool edge_root(const Edge& E, i64 nq, const double* x, cplx& s0) {
   }
   double dd = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
   double cd = C[0] * D[0] + C[1] * D[1] + C[2] * D[2];
-  double cc = C[0] * C[0] + C[1] * C[1] + C[2] * C[2];
-  double im = std::sqrt(std::max(0.0, cc * dd - cd * cd));
+  double cxd[3];
+  cross(C, D, cxd);
+  double im = std::sqrt(cxd[0] * cxd[0] + cxd[1] * cxd[1] + cxd[2] * cxd[2]);
   cplx s(-cd / dd, im / dd);
   for (int it = 0; it < 30; ++it) {
     cplx f = 0.0, fp = 0.0;
@@ -490,7 +496,26 @@ static double solid_angle(const Panel& P, const double* x, const double* q, doub
   return total;
 }
 
-/* Gauss-Newton projection of x (original coordinates) onto the patch */
+/* projections that had not met the stop test by their 40th step, and the
+   largest such last step */
+static std::mutex g_capped_mu;
+static i64 g_unconverged = 0;
+static double g_capped_step = 0.0;
+i64 projection_unconverged(bool reset, double* max_step) {
+  std::lock_guard<std::mutex> g(g_capped_mu);
+  const i64 n = g_unconverged;
+  if (max_step) *max_step = g_capped_step;
+  if (reset) { g_unconverged = 0; g_capped_step = 0.0; }
+  return n;
+}
+
+/* Gauss-Newton projection of x (original coordinates) onto the patch.  It stops
+   when the step |du| + |dv| falls below 1e-15 or below 8 eps |x| sqrt((a11 +
+   a22) / det), eight times the step that rounding of the residual (eps |x|) alone
+   makes (1 / the smallest singular value of the Jacobian is at most
+   sqrt((a11 + a22) / det)); a fixed 1e-15 is below that floor on small panels,
+   where the iteration would cycle to the step cap.  At the cap (40 steps) the
+   last iterate is kept, and counted. */
 static void project(const Panel& P, const double* x, const double* uvn, double* ystar,
                     double* nstar, double* dist) {
   i64 h = P.h;
@@ -502,8 +527,10 @@ static void project(const Panel& P, const double* x, const double* uvn, double* 
     double d = (y[0] - x[0]) * (y[0] - x[0]) + (y[1] - x[1]) * (y[1] - x[1]) + (y[2] - x[2]) * (y[2] - x[2]);
     if (d < best) { best = d; u = uvn[2 * i]; v = uvn[2 * i + 1]; }
   }
-  double y[3], yu[3], yv[3];
-  for (int it = 0; it < 40; ++it) {
+  double y[3], yu[3], yv[3], step = 0.0;
+  const double xn = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  bool done = false;
+  for (int it = 0; it < 40 && !done; ++it) {
     geom_eval(P, u, v, y, yu, yv);
     double r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
     double a11 = yu[0] * yu[0] + yu[1] * yu[1] + yu[2] * yu[2];
@@ -517,9 +544,15 @@ static void project(const Panel& P, const double* x, const double* uvn, double* 
     if (un < 0.0) un = 0.0;
     if (vn < 0.0) vn = 0.0;
     if (un + vn > 1.0) { double ex = 0.5 * (un + vn - 1.0); un -= ex; vn -= ex; if (un < 0) { vn += un; un = 0; } if (vn < 0) { un += vn; vn = 0; } }
-    double step = std::fabs(un - u) + std::fabs(vn - v);
+    step = std::fabs(un - u) + std::fabs(vn - v);
     u = un; v = vn;
-    if (step < 1e-15) break;
+    const double floor8 = 8.0 * 2.220446049250313e-16 * xn * std::sqrt((a11 + a22) / det);
+    done = step < std::max(1e-15, floor8);
+  }
+  if (!done) {
+    std::lock_guard<std::mutex> g(g_capped_mu);
+    ++g_unconverged;
+    g_capped_step = std::max(g_capped_step, step);
   }
   geom_eval(P, u, v, y, yu, yv);
   double nn[3];

@@ -32,6 +32,7 @@ def lib():
         _LIB.qol_geo_charts_gpu.restype = i64
         _LIB.qol_geo_curvature_tail.restype = C.c_double
         _LIB.qol_last_status.restype = i64
+        _LIB.qol_last_projection_unconverged.restype = i64
     return _LIB
 
 

@@ -31,6 +31,7 @@
 namespace qol {
 void set_status(int64_t s);
 void set_error(const std::string& s);
+void set_unconverged(int64_t n, double max_step);
 }  // namespace qol
 
 using namespace qol;
@@ -169,6 +170,10 @@ int qol_engine_sync(void* h) {
 
 int qol_engine_stage_ms(void* h, double* ms) {
   return guarded([&] { E(h)->stage_ms(ms); });
+}
+
+int qol_engine_projection_unconverged(void* h, int64_t* count, double* max_step) {
+  return guarded([&] { *count = E(h)->projection_unconverged(max_step); });
 }
 
 int qol_engine_get_weights(void* h, double* As, double* Ad, int64_t* label, double* Omega, double* Ias) {
@@ -451,6 +456,7 @@ void qol_rrq_mex_(const int64_t* m, const double* tx, const int64_t* n, const do
                   double* IalphaAsvestas, double* timeinfo) {
   (void)isimd; (void)tgl; (void)wgl; (void)Dgl; (void)w_bclag; (void)Legmat; (void)umatr; (void)vmatr;
   guarded([&] {
+    set_unconverged(0, 0.0);
     const int64_t p = *order, h = p * (p + 1) / 2;
     if (*n != h) throw std::runtime_error("n must be order(order+1)/2");
     check_orders(p, *nquad, *orderff, *distff);
@@ -467,6 +473,11 @@ void qol_rrq_mex_(const int64_t* m, const double* tx, const int64_t* n, const do
     e.set_pairs(offs.data(), idx.data());
     e.eval(*exterior != 0 ? 1 : 0, 1e-10, true);
     e.get_weights(As, Ad, nullptr, Omega, IalphaAsvestas);
+    {
+      double st = 0.0;
+      const int64_t n = e.projection_unconverged(&st);
+      set_unconverged(n, st);
+    }
     double ms[10];
     e.stage_ms(ms);
     add_timeinfo(ms, timeinfo);
@@ -482,6 +493,7 @@ void qol_getnearquad_lap_rrq_mex_(const int64_t* npan, const int64_t* nterms, co
                                   double* sparsevs, double* sparsevd, double* timeinfo) {
   (void)isimd;
   guarded([&] {
+    set_unconverged(0, 0.0);
     const int64_t p = *nterms, h = *hdim;
     if (h != p * (p + 1) / 2) throw std::runtime_error("hdim must be nterms(nterms+1)/2");
     if (*nsrc != *npan * h) throw std::runtime_error("nsrc must be npan*hdim");
@@ -500,6 +512,11 @@ void qol_getnearquad_lap_rrq_mex_(const int64_t* npan, const int64_t* nterms, co
     e.set_pairs(offs.data(), idx.data());
     e.eval((*iside % 10) == 0 ? 1 : 0, 1e-10, false);
     e.triplets(tcxrow, sparsei, sparsej, sparsevs, sparsevd);
+    {
+      double st = 0.0;
+      const int64_t n = e.projection_unconverged(&st);
+      set_unconverged(n, st);
+    }
     double ms[10];
     e.stage_ms(ms);
     add_timeinfo(ms, timeinfo);

@@ -308,7 +308,7 @@ size_t Engine::one_panel_bytes(i64 p, i64 m, i64 ichart, bool square) {
   if (!rrq_solve_g1_in_lds((int)h)) add(4 * h * h * D);
   // eval: Omega, Ias, the per-pair RRQ data, the level geometry
   for (size_t b : {4 * h * mm * D, mm * D, 4 * mm * D, 3 * nq * mm * D, 3 * mm * D, 9 * mm * D,
-                   rrq_edge_order_ints((int64_t)mm) * I, 5 * h * mm * D, 8 * ntot * D, 4 * nsubtot * D})
+                   rrq_edge_order_ints((int64_t)mm) * I, (size_t)16, 5 * h * mm * D, 8 * ntot * D, 4 * nsubtot * D})
     add(b);
   return tot;
 }
@@ -582,6 +582,7 @@ void Engine::eval_levels_only() {
   HIPCHK(hipGetLastError());
   evaluated = true;
   have_omega = false;
+  projected = false;
 }
 
 // Stages and streams (all at the default priority).  The per-panel RRQ set-up
@@ -616,6 +617,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
     val_current = false;
     evaluated = true;
     have_omega = want_omega;
+    projected = false;
     return;
   }
   alloc_rrq();
@@ -630,6 +632,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   d_oe.alloc(sizeof(double) * 3 * (size_t)nnz);
   d_root.alloc(sizeof(double) * 9 * (size_t)nnz);
   d_eorder.alloc(sizeof(int) * rrq_edge_order_ints(nnz));
+  d_pfail.alloc(16);
   d_vt.alloc(sizeof(double) * 5 * (size_t)h * nnz);
   alloc_levels();  // every buffer is in place before the first launch
   HIPCHK(hipEventRecord(ev_start, stream));
@@ -666,7 +669,7 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   rec(5, 0, s3);
   launch_rrq_project(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
                      (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_sx.ptr, exterior, side_tol,
-                     (double*)d_pq.ptr, (int*)d_eorder.ptr, nnz, s3);
+                     (double*)d_pq.ptr, (int*)d_eorder.ptr, (int*)d_pfail.ptr, nnz, s3);
   rec(5, 1, s3);
   HIPCHK(hipEventRecord(ev_proj, s3));
   // s1: direct levels, after the projection: the projection heads the chain the
@@ -710,6 +713,18 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   HIPCHK(hipGetLastError());
   evaluated = true;
   have_omega = want_omega;
+  projected = true;
+}
+
+i64 Engine::projection_unconverged(double* max_step) {
+  if (!evaluated) throw std::runtime_error("not evaluated");
+  if (max_step) *max_step = 0.0;
+  if (!projected) return 0;
+  HIPCHK(hipStreamSynchronize(stream));
+  int stat[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpy(stat, d_pfail.ptr, sizeof(stat), hipMemcpyDeviceToHost));
+  if (max_step) std::memcpy(max_step, stat + 2, sizeof(double));
+  return stat[0];
 }
 
 void Engine::stage_ms(double* ms) {

@@ -24,6 +24,7 @@
 
 namespace qol {
 void set_status(int64_t s);
+void set_unconverged(int64_t n, double max_step);
 void set_error(const std::string& s);
 }  // namespace qol
 
@@ -83,6 +84,7 @@ extern "C" void qol_rrq_noalloc_(const int64_t* m, const double* tx, const int64
                                  double* Ad, double* Omega, double* IalphaAsvestas, double* timeinfo) {
   (void)isimd; (void)tgl; (void)wgl; (void)Dgl; (void)w_bclag; (void)Legmat; (void)umatr; (void)vmatr;
   try {
+    set_unconverged(0, 0.0);
     const int64_t p = *order, h = p * (p + 1) / 2;
     if (p < 2 || p > 12) throw std::runtime_error("qol_rrq_noalloc_: order must be 2..12");
     if (*n != h) throw std::runtime_error("n must be order(order+1)/2");
@@ -116,6 +118,11 @@ extern "C" void qol_rrq_noalloc_(const int64_t* m, const double* tx, const int64
       e.set_pairs(offs.data(), idx.data());
       e.eval(*exterior != 0 ? 1 : 0, 1e-10, true);
       e.get_weights(As, Ad, nullptr, Omega, IalphaAsvestas);
+      {
+      double st = 0.0;
+      const int64_t n = e.projection_unconverged(&st);
+      set_unconverged(n, st);
+    }
       double ms[10];
       e.stage_ms(ms);
       add_timeinfo(ms, timeinfo);

@@ -86,6 +86,8 @@ struct Engine {
   DevBuf d_label, d_ppanel, d_nrrq, d_lcount, d_lstart, d_ltot, d_lpairs, d_wout, d_val, d_omega, d_ias;
   DevBuf d_abucket, d_apairs, d_acnt;  // adaptive levels: tree cut per pair, pairs sorted by cut, cut counts
   DevBuf d_lgeo, d_balls, d_pq, d_w1, d_oe, d_root, d_eorder, d_vt;
+  DevBuf d_pfail;              // projections ended at the step cap: count, largest last step
+  bool projected = false;      // the last eval ran the projection (it had RRQ pairs to run)
   DevBuf d_frame, d_edge, d_bX, d_cx, d_basis, d_Fq, d_Msl, d_Fn, d_G, d_Binv, d_scr;
   DevBuf d_ub, d_ubn, d_u, d_tcount, d_toffs, d_tcur, d_tpairs;
   RRQPanels RP{};
@@ -117,6 +119,10 @@ struct Engine {
   void classify_lists();
   void direct(hipStream_t s);
   void stage_ms(double* ms);
+  // RRQ pairs of the last eval whose Gauss-Newton projection had not stopped by its
+  // last (40th) step, and the largest such last step |du| + |dv|; their foot
+  // point is the last iterate (waits for the stream)
+  i64 projection_unconverged(double* max_step = nullptr);
   void get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias);
   void pair_features(float* feat);
   void apply(const double* ub, const double* ubn, double* u, double* val, bool device_ptrs);

@@ -175,7 +175,7 @@ void launch_rrq_panel_solve(RRQTables T, RRQPanels P, int npan, const int* nrrq,
 bool rrq_solve_g1_in_lds(int h);
 void launch_rrq_project(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
                         const int* tidx, const double* tx, const double* sx, int exterior, double side_tol,
-                        double* pq, int* order, int64_t nnz, hipStream_t s);
+                        double* pq, int* order, int* nfail, int64_t nnz, hipStream_t s);
 // root + SSQ kernel, then (after event `mid`, if given) the solid-angle kernel
 size_t rrq_edge_order_ints(int64_t nnz);
 void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,

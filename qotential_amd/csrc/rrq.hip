@@ -1576,7 +1576,11 @@ __device__ __forceinline__ void leg3_c(const double* cf, cd s, cd* val, cd* der)
   }
 }
 
-// complex root of |X_e(s) - x|^2 near [-1,1] by Newton from the chord guess
+// complex root of |X_e(s) - x|^2 near [-1,1] by Newton from the chord's root
+// s = (-C.D + i |C x D|) / D.D.  The imaginary part comes from the cross product:
+// the equal form sqrt(|C|^2 |D|^2 - (C.D)^2) cancels for a target near the
+// chord's line (near a vertex), can round to 0, and Newton from a real start
+// stays on the real axis.
 template <int NQ>
 __device__ bool edge_root(const double* cf, const double* x, cd& s0) {
   double A[3], B[3], dA[3], dB[3], D[3], C[3];
@@ -1589,8 +1593,9 @@ __device__ bool edge_root(const double* cf, const double* x, cd& s0) {
   }
   const double dd = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
   const double cdv = C[0] * D[0] + C[1] * D[1] + C[2] * D[2];
-  const double cc = C[0] * C[0] + C[1] * C[1] + C[2] * C[2];
-  cd s = cmk(-cdv / dd, sqrt(fmax(0.0, cc * dd - cdv * cdv)) / dd);
+  double cxd[3];
+  cross3(C, D, cxd);
+  cd s = cmk(-cdv / dd, sqrt(cxd[0] * cxd[0] + cxd[1] * cxd[1] + cxd[2] * cxd[2]) / dd);
   for (int it = 0; it < 30; ++it) {
     cd v[3], d[3];
     leg3_c<NQ>(cf, s, v, d);
@@ -1718,9 +1723,13 @@ __device__ void project_start(const double* ssx, const double* uvn, int h, const
 }
 
 // One Gauss-Newton update of (u, v) from the point y and its tangents, clamped
-// to the triangle; true when the step is below 1e-15
-__device__ __forceinline__ bool gn_step(const double* x, const double* y, const double* yu, const double* yv,
-                                        double& u, double& v) {
+// to the triangle; true when the step |du| + |dv| (also returned in st) is below
+// 1e-15 or below 8 eps |x| sqrt((a11 + a22) / det) (xn = |x|): eight times the
+// step that rounding of the residual alone makes, since 1 / the Jacobian's
+// smallest singular value is at most sqrt((a11 + a22) / det).  A fixed 1e-15 is
+// below that floor on small panels, where the iteration would cycle to the cap.
+__device__ __forceinline__ bool gn_step(const double* x, double xn, const double* y, const double* yu,
+                                        const double* yv, double& u, double& v, double& st) {
   const double r[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
   const double a11 = yu[0] * yu[0] + yu[1] * yu[1] + yu[2] * yu[2];
   const double a12 = yu[0] * yv[0] + yu[1] * yv[1] + yu[2] * yv[2];
@@ -1737,19 +1746,28 @@ __device__ __forceinline__ bool gn_step(const double* x, const double* y, const 
     if (un < 0.0) { vn += un; un = 0.0; }
     if (vn < 0.0) { un += vn; vn = 0.0; }
   }
-  const double step = fabs(un - u) + fabs(vn - v);
+  st = fabs(un - u) + fabs(vn - v);
   u = un; v = vn;
-  return step < 1e-15;
+  return st < fmax(1e-15, 8.0 * 2.220446049250313e-16 * xn * sqrt((a11 + a22) / det));
 }
 
 __device__ bool project_steps(const double* cx, const double* __restrict__ kt, int h, int korder, const double* x,
                               double& u, double& v, int it0, int it1) {
-  double y[3], yu[3], yv[3];
+  const double xn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  double y[3], yu[3], yv[3], st;
   for (int it = it0; it < it1; ++it) {
     geom_eval(cx, kt, h, korder, u, v, y, yu, yv);
-    if (gn_step(x, y, yu, yv, u, v)) return true;
+    if (gn_step(x, xn, y, yu, yv, u, v, st)) return true;
   }
   return false;
+}
+
+// A projection that ends at the step cap: stat[0] counts them, the 8 bytes at
+// stat + 2 hold the largest last step (a non-negative double's bits order as its
+// value, so an unsigned maximum of the bits is the maximum of the values).
+__device__ __forceinline__ void note_capped(int* stat, double st) {
+  atomicAdd(stat, 1);
+  atomicMax(reinterpret_cast<unsigned long long*>(stat + 2), (unsigned long long)__double_as_longlong(st));
 }
 
 __device__ double project_end(const double* cx, const double* __restrict__ kt, int h, int korder, const double* x,
@@ -1785,7 +1803,10 @@ __global__ void zero_ints_kernel(int* __restrict__ p, int n) {
 // the few that have not converged (a list built with an atomic counter) continue
 // in the second launch from the saved (u, v), so a wave no longer runs as long
 // as its slowest pair.  The step sequence of every pair is the single loop's,
-// so the results are the same.
+// so the results are the same.  A pair that has not met the stop test of
+// gn_step after kPMax steps keeps its last iterate, as in the single loop, and is
+// counted in `nfail` (with the largest such last step) so that the caller can
+// see it.
 constexpr int kPI = 8, kPMax = 40;
 
 __device__ void project_write(RRQTables T, RRQPanels P, int k, const double* x, double u, double v, int exterior,
@@ -1836,7 +1857,8 @@ __global__ void __launch_bounds__(256) rrq_project_rest_kernel(RRQTables T, RRQP
                                                                const double* __restrict__ tx, int exterior,
                                                                double side_tol, double* __restrict__ pq,
                                                                const int* __restrict__ rest,
-                                                               const int* __restrict__ nrest) {
+                                                               const int* __restrict__ nrest,
+                                                               int* __restrict__ nfail) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= *nrest) return;
   const int r = rest[gid];
@@ -1846,7 +1868,15 @@ __global__ void __launch_bounds__(256) rrq_project_rest_kernel(RRQTables T, RRQP
   const double x[3] = {tx[3 * g], tx[3 * g + 1], tx[3 * g + 2]};
   double* o = pq + 4 * (size_t)r;
   double u = o[0], v = o[1];
-  (void)project_steps(P.cx + (size_t)k * 3 * h, T.ktab, h, T.p - 1, x, u, v, kPI, kPMax);
+  const double* cx = P.cx + (size_t)k * 3 * h;
+  const double xn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  double y[3], yu[3], yv[3], st = 0.0;
+  bool done = false;
+  for (int it = kPI; it < kPMax && !done; ++it) {
+    geom_eval(cx, T.ktab, h, T.p - 1, u, v, y, yu, yv);
+    done = gn_step(x, xn, y, yu, yv, u, v, st);
+  }
+  if (!done) note_capped(nfail, st);
   project_write(T, P, k, x, u, v, exterior, side_tol, o);
 }
 
@@ -1912,7 +1942,8 @@ __global__ void __launch_bounds__(64) rrq_project_rest_reg_kernel(RRQTables T, R
                                                                   const double* __restrict__ tx, int exterior,
                                                                   double side_tol, double* __restrict__ pq,
                                                                   const int* __restrict__ rest,
-                                                                  const int* __restrict__ nrest) {
+                                                                  const int* __restrict__ nrest,
+                                                                  int* __restrict__ nfail) {
   constexpr int H = N * (N + 1) / 2;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= *nrest) return;
@@ -1929,34 +1960,38 @@ __global__ void __launch_bounds__(64) rrq_project_rest_reg_kernel(RRQTables T, R
     for (int i = 0; i < H; ++i) cr[c][i] = cx[c * H + i];
   double* o = pq + 4 * (size_t)r;
   double u = o[0], v = o[1];
-  double y[3], yu[3], yv[3];
-  for (int it = kPI; it < kPMax; ++it) {
+  const double xn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  double y[3], yu[3], yv[3], st = 0.0;
+  bool done = false;
+  for (int it = kPI; it < kPMax && !done; ++it) {
     geom_eval_reg<N>(cr, T.ktab, u, v, y, yu, yv);
-    if (gn_step(x, y, yu, yv, u, v)) break;
+    done = gn_step(x, xn, y, yu, yv, u, v, st);
   }
+  if (!done) note_capped(nfail, st);
   project_write(T, P, k, x, u, v, exterior, side_tol, o);
 }
 
 void launch_rrq_project(RRQTables T, RRQPanels P, const int* lpairs, const int* ltot, const int* ppanel,
                         const int* tidx, const double* tx, const double* sx, int exterior, double side_tol,
-                        double* pq, int* order, int64_t nnz, hipStream_t s) {
+                        double* pq, int* order, int* nfail, int64_t nnz, hipStream_t s) {
   int* nrest = order;      // [1]
   int* rest = order + 1;   // [nnz]
   const dim3 grid((unsigned)((nnz + 255) / 256)), blk(256);
   hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, nrest, 1);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, nfail, 4);
   hipLaunchKernelGGL(rrq_project_kernel, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, sx, exterior,
                      side_tol, pq, rest, nrest);
   const dim3 rgrid((unsigned)((nnz + 63) / 64)), rblk(64);
 #define QOL_PROJ(NN)                                                                                            \
   case NN:                                                                                                      \
     hipLaunchKernelGGL(rrq_project_rest_reg_kernel<NN>, rgrid, rblk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, \
-                       exterior, side_tol, pq, (const int*)rest, (const int*)nrest);                             \
+                       exterior, side_tol, pq, (const int*)rest, (const int*)nrest, nfail);                      \
     break;
   switch (T.p) {
     QOL_PROJ(2) QOL_PROJ(3) QOL_PROJ(4) QOL_PROJ(5) QOL_PROJ(6) QOL_PROJ(7) QOL_PROJ(8)
     default:
       hipLaunchKernelGGL(rrq_project_rest_kernel, grid, blk, 0, s, T, P, lpairs, ltot, ppanel, tidx, tx, exterior,
-                         side_tol, pq, (const int*)rest, (const int*)nrest);
+                         side_tol, pq, (const int*)rest, (const int*)nrest, nfail);
   }
 #undef QOL_PROJ
 }

@@ -143,6 +143,15 @@ class Engine:
         return dict(levels=ms[0], classify=ms[1], direct=ms[2], rrq_setup=ms[3], rrq_solve=ms[4], rrq_project=ms[5],
                     rrq_root_ssq=ms[6], rrq_moments=ms[7], rrq_solid_angle=ms[8], rrq_weights=ms[9])
 
+    def projection_unconverged(self):
+        """(count, largest last step): RRQ pairs of the last eval whose
+        Gauss-Newton projection had not stopped by its 40th and last step (they
+        keep that last iterate), and the largest such step |du| + |dv|"""
+        n, st = C.c_int64(), C.c_double()
+        check(lib().qol_engine_projection_unconverged(self._h, C.byref(n), C.byref(st)),
+              "qol_engine_projection_unconverged")
+        return n.value, st.value
+
     def weights(self, omega=False):
         n = max(self.nnz, 1)
         As = np.zeros((n, self.h), order="F"); Ad = np.zeros((n, self.h), order="F")

@@ -78,7 +78,9 @@ def rrq(tx, sx, snx, sw, rts, rps, order, exterior=True, ichart=0, sxbd_chart=No
                        ("tgl", "wgl", "Dgl", "w_bclag", "Legmat", "umatr", "vmatr")],
                        ptr(As), ptr(Ad), ptr(Om), ptr(Ia), ptr(ti))
     _status("qol_rrq_noalloc" if noalloc else "qol_rrq_mex")
-    return dict(As=As, Ad=Ad, Omega=Om, Ias=Ia, timeinfo=ti)
+    st = C.c_double()
+    n = lib().qol_last_projection_unconverged(C.byref(st))
+    return dict(As=As, Ad=Ad, Omega=Om, Ias=Ia, timeinfo=ti, unconverged=(n, st.value))
 
 
 def getnearquad_lap_rrq(geo, tcx, tcxrow, tcxi, iside=30):

@@ -176,3 +176,99 @@ def test_engine_reuse_new_targets_same_nnz(geo):
     assert np.array_equal(u_b, u_f)
     assert np.abs(u_b[:len(tx)] - u_a[perm]).max() <= 1e-12 * np.abs(u_a).max()
     assert not u_b[len(tx):].any()
+
+
+# Targets beyond vertices and edges.  Near the patch boundary the result depends
+# on rounding like eps / (distance to the boundary point): the oracle's plain build
+# and its build with fused multiply-adds (the GPU compiler's contraction) differ
+# there by up to 10 eps / distance in the action on a smooth density.  Bounds: the
+# action within max(EDGE_ACT_BASE[p], 500 eps / distance) (500: about 10x the
+# largest such spread of two other roundings); per weight, for targets 1e-3 or
+# more from the boundary point, EDGE_RRQ_TOL[p] (p = 12's weights move by up to
+# 1e-3 between roundings even there, so it is checked on the action only).  The
+# bases are those of the mixed-order test: the order-12 action differs from the
+# oracle's by up to 4.1e-8 on the busiest panels there
+EDGE_ACT_BASE = {4: 1e-10, 8: 1e-10, 12: 4e-7}
+EDGE_RRQ_TOL = {4: 1e-10, 8: 2e-7}
+
+
+@pytest.mark.parametrize("p", [4, 8, 12])
+def test_projection_beyond_vertices_and_edges(oracle, p):
+    """Targets beyond the vertices and edges of a sample of panels, each against
+    its own panel: labels equal the oracle's, the RRQ weights' action on a smooth
+    density agrees with the oracle's to the bound above (per weight too, away from
+    the boundary, at p = 4 and 8), and the Gauss-Newton projections stop before
+    the 40-step cap (at most one in 1000 reaches it, within 1e-12 of stopping; the
+    oracle's reach it none)"""
+    from panel_points import action_bound, boundary_targets
+    say(f"p={p}: targets beyond the vertices and edges of every 73rd panel")
+    geo = W.geometry(8, 20, p, modes=W.VMEC_BENCH)
+    h = geo["h"]
+    pos, q = W.point_charges(geo, seed=3)
+    ub, gu = W.charge_field(geo["sx"], pos, q)
+    ubn = (geo["snx"] * gu).sum(1)
+    panels = list(range(0, geo["ntri"], 73))
+    txs, abs_, tris = [], [], []
+    for k in panels:
+        sl = slice(k * h, (k + 1) * h)
+        t, ab = boundary_targets(oracle, p, geo["sx"][sl], np.sqrt(geo["sw"][sl].sum()))
+        txs.append(t)
+        abs_.append(ab)
+        tris.append(np.full(len(t), k))
+    tx, ab_all, tri = np.vstack(txs), np.vstack(abs_), np.concatenate(tris)
+    offs, idx1 = _own_panel_pairs(geo["ntri"], tri)
+    e = Engine(geo)
+    e.set_targets(tx)
+    e.set_pairs(offs, idx1)
+    e.eval(exterior=True, side_tol=1e-10)
+    n, step = e.projection_unconverged()
+    w = e.weights()
+    e.close()
+    oracle.set_side_tol(1e-10)
+    oracle.projection_unconverged(True)
+    worst = {"act": 0.0, "w": 0.0, "act_away": 0.0, "k_near": 0.0}
+    try:
+        for k in panels:
+            a, b = offs[k], offs[k + 1]
+            sl = slice(k * h, (k + 1) * h)
+            ab = ab_all[idx1[a:b] - 1]
+            r = oracle.rrq(p, tx[idx1[a:b] - 1], geo["sx"][sl], geo["snx"][sl], geo["sw"][sl], geo["rts"][sl],
+                           geo["rps"][sl], exterior=True)
+            lab = w["label"][a:b]
+            assert np.array_equal(lab, r["label"]), k
+            rrq = lab == 5
+            for key in ("As", "Ad"):
+                err = np.abs(w[key][a:b] - r[key]).max(axis=1) / np.abs(r[key]).max(axis=1)
+                assert err[~rrq].max(initial=0.0) < 1e-13, (k, key)
+                away = rrq & (np.hypot(ab[:, 0], ab[:, 1]) >= 1e-3)
+                if p in EDGE_RRQ_TOL:
+                    worst["w"] = max(worst["w"], err[away].max(initial=0.0))
+            act = np.abs((w["As"][a:b] - r["As"]) @ ubn[sl] - (w["Ad"][a:b] - r["Ad"]) @ ub[sl])
+            act /= np.abs(r["As"]) @ np.abs(ubn[sl]) + np.abs(r["Ad"]) @ np.abs(ub[sl])
+            ratio = act[rrq] / action_bound(p, ab[rrq], EDGE_ACT_BASE[p])
+            worst["act"] = max(worst["act"], ratio.max(initial=0.0))
+            dist = np.hypot(ab[:, 0], ab[:, 1])
+            worst["act_away"] = max(worst["act_away"], act[rrq & (dist >= 1e-3)].max(initial=0.0))
+            near = rrq & (dist < 1e-3)
+            worst["k_near"] = max(worst["k_near"], (act[near] * dist[near]).max(initial=0.0) / np.finfo(float).eps)
+    finally:
+        oracle.set_side_tol(1e300)
+    no, so = oracle.projection_unconverged(True)
+    nrrq = int((w["label"] == 5).sum())
+    say(f"  {nrrq} RRQ pairs; capped projections: GPU {n} (largest last step {step:.2e}), oracle {no} ({so:.2e})")
+    say(f"  RRQ action vs oracle: largest error / bound {worst['act']:.2e} (1e-3 or more from the boundary "
+        f"point {worst['act_away']:.2e}, nearer {worst['k_near']:.1f} eps / distance); weights away from the "
+        f"boundary {worst['w']:.2e}")
+    assert nrrq > 1000 and worst["act"] < 1.0
+    if p in EDGE_RRQ_TOL:
+        assert worst["w"] < EDGE_RRQ_TOL[p]
+    assert no == 0 and n <= 1e-3 * nrrq and step < 1e-12
+    # the drop-ins report the same through qol_last_projection_unconverged
+    k = panels[1]
+    sl = slice(k * h, (k + 1) * h)
+    t = tx[tri == k]
+    for noalloc in (False, True):
+        g = R.rrq(t, geo["sx"][sl], geo["snx"][sl], geo["sw"][sl], geo["rts"][sl], geo["rps"][sl], p,
+                  exterior=True, noalloc=noalloc)
+        nd, sd = g["unconverged"]
+        assert 0 <= nd <= 1e-3 * len(t) and sd < 1e-12 and (nd == 0) == (sd == 0.0)

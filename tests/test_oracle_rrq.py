@@ -78,7 +78,30 @@ def test_ssq_straight_edge_exact(oracle):
             g = lambda u: f(s0 + e * np.sinh(u)) / bl
             u0, u1 = np.arcsinh((-1 - s0) / e), np.arcsinh((1 - s0) / e)
             exact = quad(g, u0, u1, epsabs=1e-15, epsrel=1e-14, limit=500)[0]
-            assert abs(approx - exact) < 1e-12 * max(1.0, abs(exact)), (s0, d, deg, approx, exact)
+            # the target's own rounding (eps |x|) fixes the root's imaginary part
+            # d / |b| only to eps |x| / d relative, which moves the log term by
+            # about 2 eps |x| / d
+            cond = 10 * np.finfo(float).eps * np.linalg.norm(x) / d
+            assert abs(approx - exact) < 1e-12 * max(1.0, abs(exact)) + cond, (s0, d, deg, approx, exact)
+
+
+@pytest.mark.parametrize("x0,d", [(1.0 + 1e-8, 1e-9), (1.0, 1e-9), (-1.0 - 3e-9, 2e-9), (0.999999, 1e-10)])
+def test_ssq_root_near_an_endpoint(oracle, x0, d):
+    """A target 1e-9 off a straight edge's line near its endpoint: the root of
+    |X(s) - x|^2 is x0 + i d / |b| exactly.  The chord start's imaginary part
+    must come out positive (from the cross product: sqrt(|C|^2 |D|^2 - (C.D)^2)
+    rounds to 0 here, and Newton from a real start never leaves the real axis,
+    fails, and the edge loses its singularity swap)."""
+    nq = 10
+    a, b = np.array([0.2, -0.1, 0.4]), np.array([0.7, 0.3, -0.2])
+    t, coef = _curve_coefs(oracle, nq, lambda s: a + s * b)
+    n = np.cross(b, [0.0, 0.0, 1.0])
+    x = a + x0 * b + d * n / np.linalg.norm(n)
+    w, so = _ssq(oracle, coef, nq, x)
+    bl = np.linalg.norm(b)
+    assert so[2] == 1.0
+    # the target itself carries rounding of about 1e-16 in each coordinate
+    assert abs(so[0] - x0) < 1e-14 and abs(abs(so[1]) - d / bl) < 1e-15
 
 
 def _flat_panel(oracle, p):
