@@ -60,9 +60,31 @@ __device__ __forceinline__ void chunk_balls(const LevelArgs& A, const double* cn
     const int s_all = A.sub_base[l] + s;
     const int i0 = A.suboff[A.sub_base[l] + l + s] - c0, i1 = A.suboff[A.sub_base[l] + l + s + 1] - c0;
     const double* L = cn + (size_t)pp * kCP;
-    // four independent sums, each in node order (the oracle's), in one pass
+    // four independent sums, each in node order (the oracle's), in one pass; the
+    // LDS loads go in batches of kB nodes ahead of the additions, which keep
+    // their order (one thread serialises a sub-patch, so load latency, not the
+    // adds, bounded the unbatched loop)
+    constexpr int kB = 8;
     double swsum = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int j = i0; j < i1; ++j) {
+    int j = i0;
+    for (; j + kB <= i1; j += kB) {
+      double bx[kB], by[kB], bz[kB], bw[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        bx[u] = L[j + u];
+        by[u] = L[kChunk + j + u];
+        bz[u] = L[2 * kChunk + j + u];
+        bw[u] = L[3 * kChunk + j + u];
+      }
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        swsum += bw[u];
+        a0 += bx[u] * bw[u];
+        a1 += by[u] * bw[u];
+        a2 += bz[u] * bw[u];
+      }
+    }
+    for (; j < i1; ++j) {
       const double w = L[3 * kChunk + j];
       swsum += w;
       a0 += L[j] * w;
@@ -70,20 +92,25 @@ __device__ __forceinline__ void chunk_balls(const LevelArgs& A, const double* cn
       a2 += L[2 * kChunk + j] * w;
     }
     const double qp[3] = {a0 / swsum, a1 / swsum, a2 / swsum};
-    // a maximum is exact in any order: two interleaved partial maxima
-    double rm0 = 0.0, rm1 = 0.0;
-    int j = i0;
-    for (; j + 1 < i1; j += 2) {
-      const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
-      const double e0 = L[j + 1] - qp[0], e1 = L[kChunk + j + 1] - qp[1], e2 = L[2 * kChunk + j + 1] - qp[2];
-      rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
-      rm1 = fmax(rm1, e0 * e0 + e1 * e1 + e2 * e2);
+    // a maximum is exact in any order: kB partial maxima over batches
+    double rm[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) rm[u] = 0.0;
+    j = i0;
+    for (; j + kB <= i1; j += kB) {
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const double d0 = L[j + u] - qp[0], d1 = L[kChunk + j + u] - qp[1], d2 = L[2 * kChunk + j + u] - qp[2];
+        rm[u] = fmax(rm[u], d0 * d0 + d1 * d1 + d2 * d2);
+      }
     }
-    if (j < i1) {
+    for (; j < i1; ++j) {
       const double d0 = L[j] - qp[0], d1 = L[kChunk + j] - qp[1], d2 = L[2 * kChunk + j] - qp[2];
-      rm0 = fmax(rm0, d0 * d0 + d1 * d1 + d2 * d2);
+      rm[0] = fmax(rm[0], d0 * d0 + d1 * d1 + d2 * d2);
     }
-    const double rmax2 = fmax(rm0, rm1);
+    double rmax2 = rm[0];
+#pragma unroll
+    for (int u = 1; u < kB; ++u) rmax2 = fmax(rmax2, rm[u]);
     const double r2 = A.subtyp[s_all] == 1 ? A.dist2 * fmax(rmax2, 0.5 * swsum) : A.dist2 * fmax(rmax2, swsum);
     double* B = balls + (size_t)(k0 + pp) * 4 * A.nsubtot + 4 * s_all;
     B[0] = qp[0];
