@@ -22,6 +22,9 @@ in HBM before timing starts.  Three paths are timed on the same workload:
 metric's condition (GRF error within 1e-9 of plain FP64 -- it is plain FP64).
 Every path's Green's identity error (S[du/dn] - D[u] = u inside, 0 outside, on
 every target) and its potential's difference from the plain path are reported.
+Beside them (not part of `value`) the plain path runs on the configuration's
+"radius 2h" near list: every target within twice the panel's largest corner
+distance of the panel's node mean.
 """
 import argparse
 import json
@@ -297,6 +300,28 @@ def main():
             r = runs[path]
             r["eng"] = None
 
+    # the configuration's "radius 2h" near list (h: the panel's largest corner
+    # distance), timed beside the reference ball rule; value stays on the ball rule
+    near2h = None
+    if not a.no_compare:
+        eng = Engine(geo)
+        eng.set_targets(tx)
+        eng.set_near_radius(2.0)
+        t0 = time.perf_counter()
+        nnz2 = eng.nearlist()
+        t_near2 = time.perf_counter() - t0
+        far2 = eng.far(ub, ubn)
+        eng.eval(exterior=True, side_tol=1e-10)
+        eng.apply(ub, ubn, far2)
+        elapsed2, _ = timed_steps(eng, a.steps, a.warmup)
+        u2, _ = eng.apply(ub, ubn, far2)
+        eng.close()
+        near2h = {"pairs": int(nnz2), "near_list_ms": 1e3 * t_near2, "ms_per_step": 1e3 * elapsed2 / a.steps,
+                  "pairs_per_s": nnz2 * a.steps / elapsed2, "grf_rel_err": float(np.abs(u2 - uref).max() / scale)}
+        print(f"[bench] plain, radius-2h near list: {nnz2} pairs ({nnz2 / runs['plain']['nnz']:.2f}x the ball "
+              f"rule, {1e3 * t_near2:.1f} ms incl. host sync); {a.steps} steps {near2h['ms_per_step']:.2f} ms/step, "
+              f"{near2h['pairs_per_s']:.4g} pairs/s; GRF rel err {near2h['grf_rel_err']:.3e}", flush=True)
+
     nnz = runs["plain"]["nnz"]
     grf = {k: float(np.abs(r["u"] - uref).max() / scale) for k, r in runs.items()}
     diff = {k: float(np.abs(r["u"] - runs["plain"]["u"]).max() / scale) for k, r in runs.items()}
@@ -364,7 +389,7 @@ def main():
                    "split_levels": a.split_levels,
                    "stage_ms": {k: r_["stage"] for k, r_ in runs.items()}, "kernels": kernels,
                    "grf_rel_err": grf, "rel_err_vs_plain_fp64": diff[a.path], "paths_rel_err_vs_plain_fp64": diff,
-                   "repeat_rel_diff": repeat},
+                   "repeat_rel_diff": repeat, "plain_near_radius_2h": near2h},
         "roofline": roof,
         "cpu_baseline": base,
     }

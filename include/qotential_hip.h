@@ -221,6 +221,13 @@ int qol_engine_nearlist(void* engine, int64_t* nnz);
  * caller's k-d tree, test/torus_fmm3dbie_rrq_comparison.m:81-99), then sorts
  * its list into ascending target order */
 int qol_engine_nearlist_grid(void* engine, int64_t* nnz);
+/* the radius the next qol_engine_nearlist / _grid uses: diam_factor = 0 (the
+ * default) is the reference's ball, 1.75 sqrt(sum sw) about the mean of the
+ * panel's nodes; f > 0 is f h about the same centre, h the largest distance
+ * between the corners of the panel's interpolant (f = 2: the "kd-tree radius 2h"
+ * near list of the benchmark's order-8 configuration).  No counterpart in the
+ * reference's entry points, whose callers build the list themselves. */
+int qol_engine_set_near_radius(void* engine, double diam_factor);
 /* caller's CSR by panel: offs(npan+1) 0-based, idx(nnz) 1-based target indices */
 int qol_engine_set_pairs(void* engine, const int64_t* offs, const int64_t* idx);
 int qol_engine_get_pairs(void* engine, int64_t* offs, int64_t* idx);

@@ -6,6 +6,7 @@
  *                   (stellarator_grf_core_mod.f90:759-888) and the fixed-order
  *                   scatter (:895-899)
  */
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -106,7 +107,37 @@ void orc_rrq(i64 m, const double* tx, i64 n, const double* sx, const double* snx
 /* CSR near list by the ball rule: for each panel k (hdim nodes each),
    offs(npan+1) 0-based, idx 1-based target indices ascending.  Two passes;
    panels are independent, so each pass runs over panels in nthreads threads
-   (the result does not depend on the thread count). */
+   (the result does not depend on the thread count).  With a diameter factor
+   f > 0 (orc_set_near_radius) the radius is f h, h the largest distance between
+   the corners (0,0), (1,0), (0,1) of the panel's degree-(p-1) interpolant. */
+static double g_near_diam = 0.0;
+
+/* rows interpolating a panel of hdim nodes at its three corners: Koornwinder
+   values at the corner times umatr (the interpolant's coefficients from values) */
+static const std::vector<double>& corner_rows(i64 hdim) {
+  static std::vector<double> rows[16];
+  i64 p = 1;
+  while (p * (p + 1) / 2 < hdim) ++p;
+  std::vector<double>& r = rows[p];
+  if (r.empty()) {
+    const i64 korder = p - 1, nq = p + 2;
+    std::vector<double> t(nq), w(nq), D(nq * nq), wb(nq), L(nq * nq), um(hdim * hdim), vm(hdim * hdim), pol(hdim);
+    simplex_precomp(nq, korder, hdim, t.data(), w.data(), D.data(), wb.data(), L.data(), um.data(), vm.data());
+    const double cuv[3][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}};
+    std::vector<double> out(3 * hdim);
+    for (int c = 0; c < 3; ++c) {
+      koorn_pols(cuv[c][0], cuv[c][1], korder, pol.data());
+      for (i64 j = 0; j < hdim; ++j) {
+        double a = 0.0;
+        for (i64 q = 0; q < hdim; ++q) a += pol[q] * um[q + hdim * j];
+        out[c * hdim + j] = a;
+      }
+    }
+    r = out;
+  }
+  return r;
+}
+
 static void panel_ball(i64 k, i64 hdim, const double* sx, const double* sw, double qp[3], double* qr2) {
   double s = 0.0;
   qp[0] = qp[1] = qp[2] = 0.0;
@@ -114,13 +145,33 @@ static void panel_ball(i64 k, i64 hdim, const double* sx, const double* sw, doub
     for (int c = 0; c < 3; ++c) qp[c] += sx[3 * (k * hdim + i) + c];
   for (i64 i = 0; i < hdim; ++i) s += sw[k * hdim + i];
   double qr = 1.75 * std::sqrt(s);
+  if (g_near_diam > 0.0) {
+    const std::vector<double>& cr = corner_rows(hdim);
+    double v[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) {
+        double a = 0.0;
+        for (i64 i = 0; i < hdim; ++i) a += cr[c * hdim + i] * sx[3 * (k * hdim + i) + d];
+        v[c][d] = a;
+      }
+    double hd = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b) {
+        const double e0 = v[a][0] - v[b][0], e1 = v[a][1] - v[b][1], e2 = v[a][2] - v[b][2];
+        hd = std::max(hd, std::sqrt(e0 * e0 + e1 * e1 + e2 * e2));
+      }
+    qr = g_near_diam * hd;
+  }
   for (int c = 0; c < 3; ++c) qp[c] /= (double)hdim;
   *qr2 = qr * qr;
 }
 
+void orc_set_near_radius(double diam_factor) { g_near_diam = diam_factor; }
+
 i64 orc_nearlist_count_mt(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
                           const double* tx, i64* offs, i64 nthreads) {
   std::vector<i64> cnt(npan, 0);
+  if (g_near_diam > 0.0) (void)corner_rows(hdim);  // filled before the threads read it
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? (int)nthreads : 1)
   for (i64 k = 0; k < npan; ++k) {
     double qp[3], qr2;
@@ -139,6 +190,7 @@ i64 orc_nearlist_count_mt(i64 npan, i64 hdim, const double* sx, const double* sw
 
 void orc_nearlist_fill_mt(i64 npan, i64 hdim, const double* sx, const double* sw, i64 nt,
                           const double* tx, const i64* offs, i64* idx, i64 nthreads) {
+  if (g_near_diam > 0.0) (void)corner_rows(hdim);
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? (int)nthreads : 1)
   for (i64 k = 0; k < npan; ++k) {
     double qp[3], qr2;

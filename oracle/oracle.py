@@ -182,6 +182,12 @@ def rrq(order, tx, sx, snx, sw, rts, rps, exterior=True, ichart=0, sxbd_chart=No
     return dict(As=As, Ad=Ad, Omega=Om, Ias=Ia, label=lab, timeinfo=ti)
 
 
+def set_near_radius(diam_factor=0.0):
+    """radius rule of later nearlist calls: 0 the reference ball, f > 0 f times
+    the panel's largest corner distance"""
+    lib().orc_set_near_radius(dbl(diam_factor))
+
+
 def nearlist(hdim, sx, sw, tx, nthreads=1):
     sx = _f(sx); sw = _f(sw); tx = _f(tx)
     npan = sw.shape[0] // hdim

@@ -150,6 +150,14 @@ int qol_engine_nearlist_grid(void* h, int64_t* nnz) {
   return guarded([&] { *nnz = E(h)->nearlist_grid(); });
 }
 
+int qol_engine_set_near_radius(void* h, double diam_factor) {
+  return guarded([&] {
+    if (!(diam_factor >= 0.0) || !std::isfinite(diam_factor))
+      throw std::runtime_error("near radius: the diameter factor must be 0 (the reference ball) or positive");
+    E(h)->near_diam = diam_factor;
+  });
+}
+
 int qol_engine_set_pairs(void* h, const int64_t* offs, const int64_t* idx) {
   return guarded([&] { E(h)->set_pairs(offs, idx); });
 }

@@ -413,7 +413,8 @@ i64 Engine::nearlist() {
   d_offs.alloc(sizeof(int) * (npan + 1));
   HIPCHK(hipMemsetAsync(d_cnt.ptr, 0, sizeof(int) * npan, stream));
   hipLaunchKernelGGL(nearlist_prep_kernel, dim3(blocks(npan, 256)), dim3(256), 0, stream, npan, h,
-                     (const double*)d_sx.ptr, (const double*)d_sw.ptr, (double*)d_nball.ptr);
+                     (const double*)d_sx.ptr, (const double*)d_sw.ptr, (const double*)tab->d_corner.ptr, near_diam,
+                     (double*)d_nball.ptr);
   const int pb = 16;
   const int gy = nt > 0 ? std::min(64, blocks(nt, 256)) : 1;
   if (nt > 0)
@@ -442,7 +443,8 @@ i64 Engine::nearlist_grid() {
   d_cnt.alloc(sizeof(int) * npan);
   d_offs.alloc(sizeof(int) * (npan + 1));
   hipLaunchKernelGGL(nearlist_prep_kernel, dim3(blocks(npan, 256)), dim3(256), 0, stream, npan, h,
-                     (const double*)d_sx.ptr, (const double*)d_sw.ptr, (double*)d_nball.ptr);
+                     (const double*)d_sx.ptr, (const double*)d_sw.ptr, (const double*)tab->d_corner.ptr, near_diam,
+                     (double*)d_nball.ptr);
   std::vector<double> bl(4 * (size_t)npan);
   HIPCHK(hipMemcpyAsync(bl.data(), d_nball.ptr, sizeof(double) * bl.size(), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));

@@ -2,7 +2,10 @@
 //
 // Reference: stellarator_grf_core_mod.f90:652-695 (count, then fill) and
 // test/stellarator_bench.f90:88-94: ball centre = mean of the panel's nodes,
-// radius 1.75*sqrt(sum sw), target kept when |x - c|^2 < r^2.
+// radius 1.75*sqrt(sum sw), target kept when |x - c|^2 < r^2.  With a diameter
+// factor f > 0 the radius is f h instead, h the largest distance between the
+// corners of the panel's interpolant (the "radius 2h" near list, f = 2; h as the
+// target generator defines the patch diameter).
 // Brute force over (panel, target) like the reference, but kPB panels per
 // workgroup share every target load, and the fill keeps ascending target order
 // with wavefront ballots so the CSR is bit-identical to the serial loop.
@@ -12,9 +15,10 @@ namespace qol {
 
 constexpr int kPB = 16;  // panels per workgroup
 
-// ball[k] = (c0, c1, c2, r^2), summed in the reference's order
+// ball[k] = (c0, c1, c2, r^2), summed in the reference's order; corner: the 3 x h
+// rows interpolating a panel at its corners (used when diam > 0)
 __global__ void nearlist_prep_kernel(int npan, int h, const double* __restrict__ sx, const double* __restrict__ sw,
-                                     double* __restrict__ ball) {
+                                     const double* __restrict__ corner, double diam, double* __restrict__ ball) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npan) return;
   double q0 = 0.0, q1 = 0.0, q2 = 0.0, s = 0.0;
@@ -25,7 +29,23 @@ __global__ void nearlist_prep_kernel(int npan, int h, const double* __restrict__
     q2 += sx[3 * g + 2];
   }
   for (int i = 0; i < h; ++i) s += sw[(size_t)k * h + i];
-  const double qr = 1.75 * sqrt(s);
+  double qr = 1.75 * sqrt(s);
+  if (diam > 0.0) {
+    double v[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) {
+        double a = 0.0;
+        for (int i = 0; i < h; ++i) a += corner[c * h + i] * sx[3 * ((size_t)k * h + i) + d];
+        v[c][d] = a;
+      }
+    double hd = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b) {
+        const double e0 = v[a][0] - v[b][0], e1 = v[a][1] - v[b][1], e2 = v[a][2] - v[b][2];
+        hd = fmax(hd, sqrt(e0 * e0 + e1 * e1 + e2 * e2));
+      }
+    qr = diam * hd;
+  }
   ball[4 * k] = q0 / (double)h;
   ball[4 * k + 1] = q1 / (double)h;
   ball[4 * k + 2] = q2 / (double)h;

@@ -76,6 +76,7 @@ struct Engine {
   int solve_variant = 0;      // RRQ solve kernels: 0 default, 1 fused elimination, 2 matrix-core elimination
   bool square = true;         // level form: square regroup (PR_SQUARE) or triangle sub-patches
   bool adaptive = false;      // triangle levels served by the adaptive walk (PR_ADAPTIVE)
+  double near_diam = 0.0;     // near list: 0 the reference ball, f > 0 radius f x (largest corner distance)
   hipStream_t stream = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr;
   hipEvent_t tev[10][2] = {};
   hipEvent_t ev_start = nullptr, ev_lists = nullptr, ev_direct = nullptr, ev_setup = nullptr, ev_system = nullptr, ev_solve = nullptr,

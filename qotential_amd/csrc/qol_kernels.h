@@ -138,7 +138,8 @@ void launch_direct_split(LevelArgs A, const int* lpairs, const int* ltot, const 
                          const double* tx, const double* lgeo, double* wout, int64_t nnz, hipStream_t s, int lo,
                          int hi);
 
-__global__ void nearlist_prep_kernel(int npan, int h, const double* sx, const double* sw, double* ball);
+__global__ void nearlist_prep_kernel(int npan, int h, const double* sx, const double* sw, const double* corner,
+                                     double diam, double* ball);
 __global__ void nearlist_count_kernel(int npan, int nt, const double* tx, const double* ball, int* count);
 __global__ void nearlist_fill_kernel(int npan, int nt, const double* tx, const double* ball, const int* offs,
                                      int* idx);

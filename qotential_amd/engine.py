@@ -77,9 +77,15 @@ class Engine:
         self.nt = tx.shape[0]
         check(lib().qol_engine_set_targets(self._h, i64(self.nt), ptr(tx), i64(0)), "qol_engine_set_targets")
 
+    def set_near_radius(self, diam_factor=0.0):
+        """radius of the next near list: 0 the reference's ball rule, f > 0 a
+        ball of f times the panel's largest corner distance (f = 2: the 2h rule)"""
+        check(lib().qol_engine_set_near_radius(self._h, dbl(diam_factor)), "qol_engine_set_near_radius")
+
     def nearlist(self, grid=False):
-        """near list by the reference's ball rule (brute force, or through a
-        uniform grid of the targets: the same CSR)"""
+        """near list by the reference's ball rule, or the radius set_near_radius
+        chose (brute force, or through a uniform grid of the targets: the same
+        CSR)"""
         n = i64(0)
         fn = lib().qol_engine_nearlist_grid if grid else lib().qol_engine_nearlist
         check(fn(self._h, C.byref(n)), "qol_engine_nearlist")
