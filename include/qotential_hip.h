@@ -245,6 +245,10 @@ int qol_engine_stage_ms(void* engine, double* ms);
  * NULL) the largest such last step, as for qol_last_projection_unconverged;
  * waits for the engine's stream.  No counterpart in the reference. */
 int qol_engine_projection_unconverged(void* engine, int64_t* count, double* max_step);
+/* per-pair weights As, Ad (nnz x h, column-major), labels (1-4 the level, 5
+ * RRQ, 0 the panel's own nodes), and, when the eval asked for them, Omega and
+ * IalphaAsvestas; any output pointer may be NULL, and with As and Ad both NULL
+ * the weights are not copied at all */
 int qol_engine_get_weights(void* engine, double* As, double* Ad, int64_t* label, double* Omega,
                            double* Ias);
 /* near-correction apply + fixed-order scatter into u (in: far field) */

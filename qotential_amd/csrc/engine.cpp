@@ -766,14 +766,16 @@ void Engine::pair_features(float* feat) {
 void Engine::get_weights(double* As, double* Ad, i64* label, double* Omega, double* Ias) {
   if (!evaluated) throw std::runtime_error("weights not evaluated");
   HIPCHK(hipStreamSynchronize(stream));
-  std::vector<double> w(2 * (size_t)h * (nnz > 0 ? nnz : 1));
-  if (nnz > 0) HIPCHK(hipMemcpy(w.data(), d_wout.ptr, sizeof(double) * 2 * h * (size_t)nnz, hipMemcpyDeviceToHost));
-  // column-major (nnz, h)
-  for (i64 j = 0; j < nnz; ++j)
-    for (int c = 0; c < h; ++c) {
-      if (As) As[j + (size_t)nnz * c] = w[(size_t)j * 2 * h + c];
-      if (Ad) Ad[j + (size_t)nnz * c] = w[(size_t)j * 2 * h + h + c];
-    }
+  if ((As || Ad) && nnz > 0) {
+    std::vector<double> w(2 * (size_t)h * nnz);
+    HIPCHK(hipMemcpy(w.data(), d_wout.ptr, sizeof(double) * 2 * h * (size_t)nnz, hipMemcpyDeviceToHost));
+    // column-major (nnz, h)
+    for (i64 j = 0; j < nnz; ++j)
+      for (int c = 0; c < h; ++c) {
+        if (As) As[j + (size_t)nnz * c] = w[(size_t)j * 2 * h + c];
+        if (Ad) Ad[j + (size_t)nnz * c] = w[(size_t)j * 2 * h + h + c];
+      }
+  }
   if (label && nnz > 0) {
     std::vector<int8_t> l(nnz);
     HIPCHK(hipMemcpy(l.data(), d_label.ptr, nnz, hipMemcpyDeviceToHost));

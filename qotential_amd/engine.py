@@ -158,6 +158,13 @@ class Engine:
               "qol_engine_projection_unconverged")
         return n.value, st.value
 
+    def labels(self):
+        """the label of every pair (1-4 the serving level, 5 RRQ, 0 the panel's own
+        nodes), without copying the weights"""
+        lab = np.zeros(max(self.nnz, 1), dtype=np.int64)
+        check(lib().qol_engine_get_weights(self._h, None, None, ptr(lab), None, None), "qol_engine_get_weights")
+        return lab[:self.nnz]
+
     def weights(self, omega=False):
         n = max(self.nnz, 1)
         As = np.zeros((n, self.h), order="F"); Ad = np.zeros((n, self.h), order="F")

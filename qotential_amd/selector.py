@@ -248,7 +248,7 @@ def ablation(selector, geo, nt=20000, seed=99, threshold=0.9, steps=5):
             e.set_selector(selector, threshold)
         e.eval()
         u, _ = e.apply(ub, ubn, far)
-        lab = e.weights()["label"]
+        lab = e.labels()
         err = np.abs(u - uref) / np.abs(uex).max()
         e.apply(ub, ubn, far)
         e.sync()
