@@ -169,6 +169,24 @@ DeviceTables::DeviceTables(i64 p, bool square, hipStream_t s) {
   RT.tgl = (const double*)d_tgl.ptr;
   RT.wgl = (const double*)d_wgl.ptr;
   RT.Legmat = (const double*)d_Leg.ptr;
+  // [k * nq + j]: the coefficient of s^j in the Legendre polynomial P_k,
+  // (-1)^m C(k, m) C(2k - 2m, k) / 2^k for j = k - 2m: integers over a power of
+  // two, exact in double for the orders here (k <= 13)
+  {
+    const int nq = (int)T.nq;
+    std::vector<double> l2m((size_t)nq * nq, 0.0);
+    auto binom = [](int n, int r) {
+      double b = 1.0;
+      for (int i = 1; i <= r; ++i) b = b * (double)(n - r + i) / (double)i;
+      return b;
+    };
+    for (int kq = 0; kq < nq; ++kq)
+      for (int m = 0; 2 * m <= kq; ++m)
+        l2m[(size_t)kq * nq + kq - 2 * m] =
+            ((m & 1) ? -1.0 : 1.0) * binom(kq, m) * binom(2 * kq - 2 * m, kq) * std::ldexp(1.0, -kq);
+    upload(d_l2m, l2m.data(), l2m.size(), s);
+    RT.leg2mon = (const double*)d_l2m.ptr;
+  }
   RT.umatr = (const double*)d_umatr.ptr;
   RT.uvn = (const double*)d_uvn.ptr;
   const std::vector<double> kt = koorn_eval_table((int)T.p);
@@ -288,7 +306,8 @@ size_t Engine::one_panel_bytes(i64 p, i64 m, i64 ichart, bool square) {
   // per-order tables (DeviceTables)
   for (size_t b : {pad * ntot * D, pad * ntot * sizeof(float), h * ntot * D, ntot * D, suboff.size() * I, subtyp.size() * I,
                    level_chunks(LA, suboff.data()).size() * I, T.tgl.size() * D, T.wgl.size() * D,
-                   T.Legmat.size() * D, T.umatr.size() * D, T.uvn.size() * D, koorn_eval_table((int)p).size() * D,
+                   T.Legmat.size() * D, T.Legmat.size() * D, T.umatr.size() * D, T.uvn.size() * D,
+                   koorn_eval_table((int)p).size() * D,
                    3 * h * D})
     add(b);
   if (!square) add((size_t)kAdaptBuckets * (kAdaptSegs + 1) * I);

@@ -46,6 +46,7 @@ struct DeviceTables {
   DevBuf d_interp, d_imp, d_impf, d_wt, d_suboff, d_subtyp, d_chunks, d_tgl, d_wgl, d_Leg, d_umatr, d_uvn, d_ktab;
   DevBuf d_corner;  // 3 x h rows interpolating a panel at its corners (selector features)
   DevBuf d_aseg;    // triangle levels: the adaptive tree cuts' leaf cells
+  DevBuf d_l2m;     // Legendre to power-basis coefficients, nq x nq
   DeviceTables(i64 p, bool square, hipStream_t s);
 };
 

@@ -42,6 +42,7 @@ struct RRQTables {
   const double* tgl;     // nq
   const double* wgl;     // nq
   const double* Legmat;  // nq x nq (coef = Legmat * vals)
+  const double* leg2mon; // nq x nq: [k * nq + j] = coefficient of s^j in P_k(s)
   const double* umatr;   // h x h
   const double* uvn;     // 2 x h VR nodes
   const double* ktab;    // Koornwinder evaluation constants (koorn_eval_table(p))
@@ -51,7 +52,7 @@ std::vector<double> koorn_eval_table(int N);
 // per-panel RRQ data on the device (one slot per panel)
 struct RRQPanels {
   double* frame;   // [npan][16]: R(9) c(3) alpha sigma L pad
-  double* edge;    // [npan][3][3][nq] Legendre coefficients of the frame boundary
+  double* edge;    // [npan][3][3][nq] power-basis coefficients of the frame boundary (s in [-1, 1])
   double* bX;      // [npan][3*nq][6] boundary X, dX (frame)
   double* cx;      // [npan][3][h] Koornwinder coefficients of the geometry (original coords)
   double* basis;   // [npan][3*nq][ncoef][12] grad H (3 complex) and V (3 complex)

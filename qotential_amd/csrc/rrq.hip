@@ -286,9 +286,17 @@ __global__ void __launch_bounds__(256) rrq_panel_frame_kernel(RRQTables T, RRQPa
     double s = 0.0;
     for (int q = 0; q < nq; ++q) s += T.Legmat[kk + (size_t)nq * q] * bx[6 * (e * nq + q) + c];
     ecf[i] = s;
-    P.edge[(size_t)k * 9 * nq + i] = s;
   }
   __syncthreads();
+  // the edges in the power basis for the per-pair kernels (Horner's rule costs
+  // about half the Legendre recurrence per point): coefficient j of edge
+  // component ec is sum_k ecf_k [s^j] P_k, over k = j, j+2, ...
+  for (int i = threadIdx.x; i < 9 * nq; i += blockDim.x) {
+    const int ec = i / nq, j = i % nq;
+    double m = 0.0;
+    for (int q = j; q < nq; q += 2) m += ecf[ec * nq + q] * T.leg2mon[(size_t)q * nq + j];
+    P.edge[(size_t)k * 9 * nq + i] = m;
+  }
   for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
     const int e = ib / nq, i = ib % nq;
     for (int c = 0; c < 3; ++c) {
@@ -1553,7 +1561,41 @@ __device__ __forceinline__ void leg3_r(const double* cf, double s, double* val, 
   }
 }
 
+// The edges' power-basis coefficients (cf[c * NQ + j] multiplies s^j): value and
+// derivative of the three components by Horner's rule with fused multiply-adds
+template <int NQ>
+__device__ __forceinline__ void pow3_r(const double* cf, double s, double* val, double* der) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double v = cf[c * NQ + NQ - 1], d = 0.0;
+#pragma unroll
+    for (int k = NQ - 2; k >= 0; --k) {
+      d = fma(d, s, v);
+      v = fma(v, s, cf[c * NQ + k]);
+    }
+    val[c] = v;
+    der[c] = d;
+  }
+}
+
 // the same at complex s
+template <int NQ>
+__device__ __forceinline__ void pow3_c(const double* cf, cd s, cd* val, cd* der) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    cd v = cmk(cf[c * NQ + NQ - 1], 0.0), d = cmk(0.0, 0.0);
+#pragma unroll
+    for (int k = NQ - 2; k >= 0; --k) {
+      // d = d s + v, v = v s + cf_k
+      d = cmk(fma(d.re, s.re, fma(-d.im, s.im, v.re)), fma(d.re, s.im, fma(d.im, s.re, v.im)));
+      v = cmk(fma(v.re, s.re, fma(-v.im, s.im, cf[c * NQ + k])), fma(v.re, s.im, v.im * s.re));
+    }
+    val[c] = v;
+    der[c] = d;
+  }
+}
+
+// the Legendre series at complex s
 template <int NQ>
 __device__ __forceinline__ void leg3_c(const double* cf, cd s, cd* val, cd* der) {
   cd p0 = cmk(1.0, 0.0), p1 = s, d0 = cmk(0.0, 0.0), d1 = cmk(1.0, 0.0);
@@ -1584,8 +1626,8 @@ __device__ __forceinline__ void leg3_c(const double* cf, cd s, cd* val, cd* der)
 template <int NQ>
 __device__ bool edge_root(const double* cf, const double* x, cd& s0) {
   double A[3], B[3], dA[3], dB[3], D[3], C[3];
-  leg3_r<NQ>(cf, -1.0, A, dA);
-  leg3_r<NQ>(cf, 1.0, B, dB);
+  pow3_r<NQ>(cf, -1.0, A, dA);
+  pow3_r<NQ>(cf, 1.0, B, dB);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     D[c] = 0.5 * (B[c] - A[c]);
@@ -1598,7 +1640,7 @@ __device__ bool edge_root(const double* cf, const double* x, cd& s0) {
   cd s = cmk(-cdv / dd, sqrt(cxd[0] * cxd[0] + cxd[1] * cxd[1] + cxd[2] * cxd[2]) / dd);
   for (int it = 0; it < 30; ++it) {
     cd v[3], d[3];
-    leg3_c<NQ>(cf, s, v, d);
+    pow3_c<NQ>(cf, s, v, d);
     cd f = cmk(0, 0), fp = cmk(0, 0);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1691,7 +1733,7 @@ __device__ double solid_angle_edge(const double* cf, const double* tgl, const do
     for (int i = 0; i < NQ; ++i) {
       const double s = c + hh * tgl[i];
       double X[3], dX[3];
-      leg3_r<NQ>(cf, s, X, dX);
+      pow3_r<NQ>(cf, s, X, dX);
       const double z[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
       double zx[3];
       cross3(z, dX, zx);
