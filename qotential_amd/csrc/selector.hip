@@ -142,6 +142,85 @@ __device__ __forceinline__ void selector_logits(const uint16_t* __restrict__ W, 
   selector_forward(W, B, xin, h1, h2, out);
 }
 
+typedef float sqf4 __attribute__((ext_vector_type(4)));
+
+// per-wave LDS of the classification's network: the features of 64 pairs (row
+// stride kSelFS), a 16-pair hidden layer (row stride kSelHS = 4 mod 32, so the
+// A-operand reads of rows l%16, columns 4s + l/16 spread over the 32 banks) and
+// the 64 pairs' logits (row stride 8)
+constexpr int kSelFS = 13, kSelHS = kSelHid + 4;
+constexpr int kSelWave = 64 * kSelFS + 16 * kSelHS + 64 * 8;  // floats
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The network for the 64 pairs of one wave on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32), ngroups groups of 16 pairs: F holds the normalised
+// features (columns 10 and 11 zero), the logits go to L[pair][8].  BF16 weights
+// (exact in FP32) as the B operand, FP32 accumulation, the bias added after the
+// sum.  Every lane of the wave takes part.  Lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] of a 4-deep step and holds D[4(l/16)+r][l%16] of a tile.
+__device__ void selector_mlp_wave(const uint16_t* __restrict__ W, const float* __restrict__ Bi, const float* F,
+                                  float* Hb, float* L, int ngroups) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  for (int g = 0; g < ngroups; ++g) {
+    const float* Fg = F + 16 * g * kSelFS;
+    sqf4 acc[4];
+    // layer 1: K = 10 in three steps (the last one padded), N = 64 in four tiles
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = sqf4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const int kq = 4 * s + lk;
+      const float a = Fg[li * kSelFS + kq];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float b = kq < kSelIn ? bf16_to_f32(W[(16 * t + li) * kSelIn + kq]) : 0.0f;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Hb[(4 * lk + r) * kSelHS + 16 * t + li] = fmaxf(acc[t][r] + Bi[16 * t + li], 0.0f);
+    wave_sync();
+    // layer 2: K = 64 in 16 steps, N = 64
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = sqf4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < kSelHid / 4; ++s) {
+      const int kq = 4 * s + lk;
+      const float a = Hb[li * kSelHS + kq];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float b = bf16_to_f32(W[kSelW1 + (16 * t + li) * kSelHid + kq]);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+      }
+    }
+    wave_sync();  // every lane has read the first hidden layer
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Hb[(4 * lk + r) * kSelHS + 16 * t + li] = fmaxf(acc[t][r] + Bi[kSelHid + 16 * t + li], 0.0f);
+    wave_sync();
+    // layer 3: K = 64, N = 7 in one tile (columns 7..15 zero)
+    sqf4 o = sqf4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < kSelHid / 4; ++s) {
+      const int kq = 4 * s + lk;
+      const float a = Hb[li * kSelHS + kq];
+      const float b = li < kSelOut ? bf16_to_f32(W[kSelW1 + kSelW2 + li * kSelHid + kq]) : 0.0f;
+      o = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, o, 0, 0, 0);
+    }
+    if (li < kSelOut)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) L[(16 * g + 4 * lk + r) * 8 + li] = o[r] + Bi[2 * kSelHid + li];
+    wave_sync();  // the hidden tile is rewritten by the next group
+  }
+}
+
 size_t classify_smem(const LevelArgs& A, const SelectorArgs& S) {
   size_t b = sizeof(double) * 4 * (size_t)A.nsubtot;
   if (S.mode == 2) {
@@ -149,6 +228,7 @@ size_t classify_smem(const LevelArgs& A, const SelectorArgs& S) {
     b += sizeof(float) * (kSelNB + 2 * kSelIn);                    // biases, normalisation
     b += sizeof(uint16_t) * kSelNW;                                // weights
     b += 64 + sizeof(PanelFeat);                                   // panel features (8-byte aligned)
+    b += sizeof(float) * 4 * kSelWave;                             // the network's per-wave tiles
   }
   return b;
 }
@@ -223,7 +303,61 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
     panel_features(h, ynod, snx + 3 * (size_t)k * h, sw + (size_t)k * h, S.corner, sb + 4 * A.sub_base[0], pf[0]);
   __syncthreads();
   const int j0 = offs[k], j1 = offs[k + 1];
-  for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+  if (LEARNED) {
+    // the learned selector (launched for S.mode == 2 only): each wave takes 64
+    // pairs at a time (a loop uniform over the wave, since the network runs on
+    // the matrix cores with every lane)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float* F = (float*)(pf + 1) + (size_t)wv * kSelWave;
+    float* Hb = F + 64 * kSelFS;
+    float* LB = Hb + 16 * kSelHS;
+    for (int base = j0 + 64 * wv; base < j1; base += blockDim.x) {
+      const int j = base + lane;
+      const bool live = j < j1;
+      float f[kSelIn];
+#pragma unroll
+      for (int i = 0; i < kSelIn; ++i) f[i] = 0.0f;
+      int lab = 5;
+      if (live) {
+        const int t = tidx[j];
+        const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
+        for (int l = 0; l < 4 && lab == 5; ++l) {
+          bool outside = true;
+          for (int s = A.sub_base[l]; s < A.sub_base[l] + A.lev_nsub[l]; ++s) {
+            const double d0 = sb[4 * s] - x0, d1 = sb[4 * s + 1] - x1, d2 = sb[4 * s + 2] - x2;
+            if (d0 * d0 + d1 * d1 + d2 * d2 < sb[4 * s + 3]) { outside = false; break; }
+          }
+          if (outside) lab = l + 1;
+        }
+        pair_feature_vector(h, pf[0], ynod, x0, x1, x2, f);
+#pragma unroll
+        for (int i = 0; i < kSelIn; ++i) f[i] = (f[i] - fb[kSelNB + i]) * fb[kSelNB + kSelIn + i];
+      }
+#pragma unroll
+      for (int i = 0; i < kSelIn; ++i) F[lane * kSelFS + i] = f[i];
+      F[lane * kSelFS + kSelIn] = 0.0f;
+      F[lane * kSelFS + kSelIn + 1] = 0.0f;
+      wave_sync();
+      selector_mlp_wave(wl, fb, F, Hb, LB, min(4, (j1 - base + 15) / 16));
+      if (live) {
+        const float* lg = LB + lane * 8;
+        int best = 0;
+        float mx = lg[0];
+#pragma unroll
+        for (int c = 1; c < kSelOut; ++c)
+          if (lg[c] > mx) { mx = lg[c]; best = c; }
+        float z = 0.0f;
+#pragma unroll
+        for (int c = 0; c < kSelOut; ++c) z += __expf(lg[c] - mx);
+        if (1.0f / z >= S.threshold) lab = selector_label(best);
+        label[j] = (int8_t)lab;
+        ppanel[j] = k;
+        if (lab > 0) atomicAdd(&cnt[lab - 1], 1);
+      }
+      wave_sync();  // the feature and logit tiles are rewritten next round
+    }
+  }
+  for (int j = j0 + threadIdx.x; j < j1 && !LEARNED; j += blockDim.x) {
     const int t = tidx[j];
     const double x0 = tx[3 * (size_t)t], x1 = tx[3 * (size_t)t + 1], x2 = tx[3 * (size_t)t + 2];
     int lab = 5;
@@ -245,21 +379,6 @@ __global__ void __launch_bounds__(256) classify_kernel(LevelArgs A, SelectorArgs
       lab = S.forced;
     } else if (S.mode == 3) {
       if (lab <= 4) lab = min(4, lab + S.forced);
-    } else if (LEARNED) {
-      float f[kSelIn], lg[kSelOut];
-      pair_feature_vector(h, pf[0], ynod, x0, x1, x2, f);
-#pragma unroll
-      for (int i = 0; i < kSelIn; ++i) f[i] = (f[i] - fb[kSelNB + i]) * fb[kSelNB + kSelIn + i];
-      selector_logits(wl, fb, f, lg);
-      int best = 0;
-      float mx = lg[0];
-#pragma unroll
-      for (int c = 1; c < kSelOut; ++c)
-        if (lg[c] > mx) { mx = lg[c]; best = c; }
-      float z = 0.0f;
-#pragma unroll
-      for (int c = 0; c < kSelOut; ++c) z += __expf(lg[c] - mx);
-      if (1.0f / z >= S.threshold) lab = selector_label(best);
     }
     label[j] = (int8_t)lab;
     ppanel[j] = k;

@@ -109,3 +109,42 @@ def test_selector_training_lowers_loss(small_set):
     say(f"loss {loss[0]:.3f} -> {loss[-1]:.3f}, train accuracy {acc:.3f}")
     assert loss[-1] < 0.5 * loss[0]
     assert acc > 0.8
+
+
+def test_served_labels_follow_the_network():
+    """The classification runs the network on the matrix cores, 64 pairs per
+    wave in groups of 16 (ragged at each panel's end).  Every pair's label must
+    be the FP32 network's (PyTorch, same BF16 weights) on the engine's own
+    features of the pair: the class of the largest probability where that
+    probability reaches the threshold (class 6 served by level 4), else the
+    ball rule's.  Pairs within 1e-5 of a tie or of the threshold can fall
+    either way under another summation order."""
+    geo = W.geometry(12, 36, 8, modes=W.VMEC_BENCH)
+    tx, _, _ = W.targets(geo, 20000, -7.0, -0.5, sides=0, seed=5)
+    sel = S.Selector.load()
+    params, norm = sel.params()
+    thr = 0.9
+    e = Engine(geo)
+    e.set_targets(tx)
+    e.nearlist()
+    feat = e.pair_features()
+    e.eval()
+    rule = e.labels()
+    e.set_selector(sel, thr)
+    e.eval()
+    lab = e.labels()
+    e.set_selector(None)
+    e.close()
+    sel.close()
+    _, _, pr = torch_reference(params, norm, feat, np.zeros(len(feat), dtype=np.int64),
+                               np.ones(S.N_CLASS, dtype=np.float32))
+    top = np.sort(pr, 1)
+    best = pr.argmax(1)
+    want = np.where(pr.max(1) >= thr, np.where(best == 6, 4, best), rule)
+    tie = (top[:, -1] - top[:, -2] < 1e-5) | (np.abs(top[:, -1] - thr) < 1e-5)
+    diff = lab != want
+    say(f"{len(lab)} pairs, {int((lab != rule).sum())} moved off the ball rule, "
+        f"{int(tie.sum())} within 1e-5 of a tie or the threshold, {int(diff.sum())} differ from torch")
+    assert len(lab) > 20000 and (lab != rule).mean() > 0.01
+    assert not diff[~tie].any()
+    assert tie.mean() < 1e-3
