@@ -84,6 +84,7 @@ def kernel_flops(p, npan, counts, nrrq, nrrq_panels, split_levels):
     info = order_info(p)
     h, nq, ncoef = info["h"], info["nq"], info["ncoef"]
     nb = 3 * nq
+    nc1, nc2 = p * (p + 1) // 2, (p - 1) * p // 2
     ntot = int(sum(info["nl"]))
     lf = level_flops(p, counts)
     ns = max(0, min(4, split_levels))
@@ -93,8 +94,11 @@ def kernel_flops(p, npan, counts, nrrq, nrrq_panels, split_levels):
         # real Gauss-Jordan 2h^3; quaternion Gauss-Jordan 32h^3, C = B Fn^T 2h^3, As rows 8h^3
         "rrq_solve": ("rrq_real_inverse_kernel+rrq_quat_inverse_kernel", 44.0 * h ** 3 * nrrq_panels,
                       PEAK_FP64_TFLOPS),
-        # boundary moments 48 nb ncoef, degree recursion ~200 ncoef
-        "rrq_moments": ("rrq_target_kernel", float(nrrq) * (48 * nb * ncoef + 200 * ncoef), PEAK_FP64_TFLOPS),
+        # raw moments 4 nb (3 nc1 + 9 nc2) (twelve real node weights against the boundary
+        # harmonics of degree < p, resp. < p - 1), their combination into the boundary
+        # moments ~200 ncoef, degree recursion ~200 ncoef
+        "rrq_moments": ("rrq_target_kernel", float(nrrq) * (4 * nb * (3 * nc1 + 9 * nc2) + 400 * ncoef),
+                        PEAK_FP64_TFLOPS),
         # W = G V: 2 (2h)(5h) per RRQ pair
         "rrq_weights": ("rrq_weights_kernel", float(nrrq) * 20 * h * h, PEAK_FP64_TFLOPS),
     }

@@ -153,7 +153,7 @@ void qol_rrq_mex_(const int64_t* m, const double* tx, const int64_t* n, const do
  * qol_rrq_mex_, with every device buffer of the call taken from a fixed 64 MiB
  * device array per GPU (set up on the first call on that device) and no
  * hipMalloc / hipFree per call.  Calls needing more than the array -- about
- * 16,000 targets at order 8, 7,900 at order 12 -- fail with qol_last_status() = 1
+ * 17,800 targets at order 8, 7,700 at order 12 -- fail with qol_last_status() = 1
  * and a message, before any device work.  Calls on one GPU share the array and
  * are serialised by the library. */
 void qol_rrq_noalloc_(const int64_t* m, const double* tx, const int64_t* n, const double* sx,

@@ -321,7 +321,7 @@ size_t Engine::one_panel_bytes(i64 p, i64 m, i64 ichart, bool square) {
                    mm, mm * I, 3 * (size_t)kAdaptBuckets * I})
     add(b);
   // alloc_rrq
-  for (size_t b : {16 * D, 9 * nq * D, 6 * nb * D, 3 * h * D, 12 * nb * ncoef * D, 3 * h * h * D, h * h * D,
+  for (size_t b : {16 * D, 9 * nq * D, 6 * nb * D, 3 * h * D, 2 * nb * (size_t)(p * (p + 1) / 2) * D, 3 * h * h * D, h * h * D,
                    h * h * D, 10 * h * h * D, h * h * D})
     add(b);
   if (!rrq_solve_g1_in_lds((int)h)) add(4 * h * h * D);
@@ -579,7 +579,7 @@ void Engine::alloc_rrq() {
   d_edge.alloc(sizeof(double) * 9 * nq * np_);
   d_bX.alloc(sizeof(double) * 6 * nb * np_);
   d_cx.alloc(sizeof(double) * 3 * (size_t)h * np_);
-  d_basis.alloc(sizeof(double) * 12 * nb * ncoef * np_);
+  d_basis.alloc(sizeof(double) * 2 * nb * (size_t)(p * (p + 1) / 2) * np_);  // boundary harmonics of degree < p
   d_Fq.alloc(sizeof(double) * 3 * (size_t)h * h * np_);
   d_Msl.alloc(sizeof(double) * (size_t)h * h * np_);
   d_Fn.alloc(sizeof(double) * (size_t)h * h * np_);
@@ -709,12 +709,12 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                   (double*)d_root.ptr, (double*)d_oe.ptr, (int*)d_eorder.ptr, nnz, s3, timing ? tev[8][0] : nullptr);
   rec(6, 1, s3);
   HIPCHK(hipEventRecord(ev_edge, s3));
-  // main: moments once the edges and the basis data are in, weights once the
+  // main: moments once the edges and the boundary harmonics are in, weights once the
   // solve is in too
   HIPCHK(hipStreamWaitEvent(stream, ev_edge, 0));
   HIPCHK(hipStreamWaitEvent(stream, ev_system, 0));
   rec(7, 0, stream);
-  const size_t lds3 = rrq_target_smem(h, nq, ncoef);
+  const size_t lds3 = rrq_target_smem(p, h, nq, ncoef);
   set_lds((const void*)rrq_target_kernel, lds3);
   hipLaunchKernelGGL(rrq_target_kernel, dim3(npan), dim3(256), lds3, stream, tab->RT, RP, (const int*)d_lpairs.ptr,
                      (const int*)d_lstart.ptr, (const int*)d_ltot.ptr, npan, (const int*)d_tidx.ptr,

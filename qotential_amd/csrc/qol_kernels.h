@@ -55,7 +55,7 @@ struct RRQPanels {
   double* edge;    // [npan][3][3][nq] power-basis coefficients of the frame boundary (s in [-1, 1])
   double* bX;      // [npan][3*nq][6] boundary X, dX (frame)
   double* cx;      // [npan][3][h] Koornwinder coefficients of the geometry (original coords)
-  double* basis;   // [npan][3*nq][ncoef][12] grad H (3 complex) and V (3 complex)
+  double* basis;   // [npan][3*nq][p(p+1)/2] complex: H'_n^m (n < p) at the boundary nodes
   double* Fq;      // [npan][h][h][3] pure quaternions grad F_k(y_i)
   double* Msl;     // [npan][h][h] Msl^T (column-major)
   double* Fn;      // [npan][h][h] F_k at the nodes, F(i,k)
@@ -190,6 +190,6 @@ void launch_rrq_weights(RRQTables T, RRQPanels P, const int* lpairs, const int* 
                         const int* nrrq, const double* vt, double* wout, hipStream_t s);
 size_t rrq_frame_smem(int h, int nq);
 size_t rrq_system_smem(int h, int nq, int ncoef);
-size_t rrq_target_smem(int h, int nq, int ncoef);
+size_t rrq_target_smem(int p, int h, int nq, int ncoef);
 
 }  // namespace qol

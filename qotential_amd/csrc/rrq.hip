@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(256) rrq_panel_frame_kernel(RRQTables T, RRQPa
   __syncthreads();
 }
 
-// One workgroup per panel: basis data at the boundary nodes, and the quaternion
+// One workgroup per panel: harmonic values at the boundary nodes, and the quaternion
 // matrix, Msl^T and Fn at the panel nodes, from the frame and boundary nodes
 // written by rrq_panel_frame_kernel.
 __global__ void __launch_bounds__(256) rrq_panel_system_kernel(RRQTables T, RRQPanels P, int npan,
@@ -334,42 +334,14 @@ __global__ void __launch_bounds__(256) rrq_panel_system_kernel(RRQTables T, RRQP
   for (int i = threadIdx.x; i < 16; i += blockDim.x) fr[i] = P.frame[(size_t)k * 16 + i];
   for (int i = threadIdx.x; i < 6 * nb; i += blockDim.x) bx[i] = P.bX[(size_t)k * nb * 6 + i];
   __syncthreads();
-  // basis data at the boundary nodes: grad H'_n^m and V = grad(dX . grad H'_n^m)
+  // harmonic values H'_n^m of degree n < p at the boundary nodes: the moments
+  // of every pair are contracted against these (their gradients and V are
+  // fixed combinations of them, applied after the contraction)
   for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) harm_vals(bx + 6 * ib, p, Hs + (size_t)ib * ncoef);
   __syncthreads();
-  for (int it = threadIdx.x; it < nb * ncoef; it += blockDim.x) {
-    const int ib = it / ncoef, kk = it % ncoef;
-    int n = 0;
-    while (hidx(n + 1, 0) <= kk) ++n;
-    const int m = kk - hidx(n, 0);
-    const cd* H = Hs + (size_t)ib * ncoef;
-    const double* dX = bx + 6 * ib + 3;
-    cd g[3];
-    harm_grad(H, n, m, g);
-    // component-major: o[c * ncoef] is component c of coefficient kk
-    double* o = P.basis + ((size_t)k * nb + ib) * 12 * ncoef + kk;
-    for (int c = 0; c < 3; ++c) { o[(2 * c) * ncoef] = g[c].re; o[(2 * c + 1) * ncoef] = g[c].im; }
-    // V = n [dX_Z grad H'_{n-1}^m + i/2 (dW grad H'_{n-1}^{m-1} + conj(dW) grad H'_{n-1}^{m+1})]
-    cd V[3] = {cmk(0, 0), cmk(0, 0), cmk(0, 0)};
-    if (n >= 2) {
-      cd gl[3], gm[3], gh[3];
-      const int n1 = n - 1;
-      // gradients of degree n-1 at m-1, m, m+1 (negative orders through conjugation)
-      auto gradm = [&](int mm, cd* gg) {
-        if (mm > n1 || mm < -n1) { gg[0] = gg[1] = gg[2] = cmk(0, 0); return; }
-        if (mm >= 0) { harm_grad(H, n1, mm, gg); return; }
-        harm_grad(H, n1, -mm, gg);
-        for (int c = 0; c < 3; ++c) { gg[c] = cconj(gg[c]); if ((-mm) & 1) gg[c] = cmk(-gg[c].re, -gg[c].im); }
-      };
-      gradm(m - 1, gl); gradm(m, gm); gradm(m + 1, gh);
-      const cd dW = cmk(dX[1], dX[2]);
-      for (int c = 0; c < 3; ++c) {
-        const cd t = cadd(cmul(dW, gl[c]), cmul(cconj(dW), gh[c]));
-        V[c] = cscale(cadd(cscale(gm[c], dX[0]), cscale(cmuli(t), 0.5)), (double)n);
-      }
-    }
-    for (int c = 0; c < 3; ++c) { o[(6 + 2 * c) * ncoef] = V[c].re; o[(6 + 2 * c + 1) * ncoef] = V[c].im; }
-  }
+  const int nc1 = p * (p + 1) / 2;
+  cd* Ho = (cd*)P.basis + (size_t)k * nb * nc1;
+  for (int it = threadIdx.x; it < nb * nc1; it += blockDim.x) Ho[it] = Hs[(size_t)(it / nc1) * ncoef + it % nc1];
   __syncthreads();
   // quaternion system at the nodes.  F_k = Im((-i)^m H'_n^m), n=1..p, m=1..n.
   // Fq(i,k) = grad F_k(y_i) as a pure quaternion; the real 4h x 4h matrix of the
@@ -2192,25 +2164,101 @@ void launch_rrq_edge(RRQTables T, RRQPanels P, const int* lpairs, const int* lto
 #undef QOL_EDGE
 }
 
-// LDS plan of rrq_target_kernel: fixed panel data plus per-target slots
+// LDS plan of rrq_target_kernel: fixed panel data plus per-target slots.  Per
+// target: the frame point (6), w1 (nb), the raw moments, later the moment vector
+// (rstr doubles), solid angle and Lambda (4), and the node weights, later the
+// boundary moments (mstr complex).
+__host__ __device__ inline int rrq_raw_doubles(int p) { return 2 * (3 * (p * (p + 1) / 2) + 9 * ((p - 1) * p / 2)); }
+__host__ __device__ inline int rrq_rstr(int p, int h) { const int r = rrq_raw_doubles(p); return r > 5 * h ? r : 5 * h; }
+__host__ __device__ inline int rrq_mstr(int nb, int ncoef) { return 6 * nb > 5 * ncoef ? 6 * nb : 5 * ncoef; }
 __host__ __device__ inline size_t rrq_target_fixed(int h, int nq) {
   return (size_t)(16 + 6 * 3 * nq) * 8;
 }
-__host__ __device__ inline size_t rrq_target_per(int h, int nq, int ncoef) {
-  const int nb = 3 * nq, big = 6 * nb > 5 * h ? 6 * nb : 5 * h;
-  return (size_t)(6 + nb + big + 4) * 8 + (size_t)ncoef * 5 * 16;
+__host__ __device__ inline size_t rrq_target_per(int p, int h, int nq, int ncoef) {
+  const int nb = 3 * nq;
+  return (size_t)(6 + nb + rrq_rstr(p, h) + 4) * 8 + (size_t)rrq_mstr(nb, ncoef) * 16;
 }
-__host__ __device__ int rrq_tile(int h, int nq, int ncoef) {
+__host__ __device__ int rrq_tile(int p, int h, int nq, int ncoef) {
   // about 48 KB per workgroup keeps three workgroups on a CU
-  const size_t budget = 48 * 1024, fixed = rrq_target_fixed(h, nq), per = rrq_target_per(h, nq, ncoef);
+  const size_t budget = 48 * 1024, fixed = rrq_target_fixed(h, nq), per = rrq_target_per(p, h, nq, ncoef);
   int t = budget > fixed + per ? (int)((budget - fixed) / per) : 1;
   return t > 16 ? 16 : (t < 1 ? 1 : t);
 }
-size_t rrq_target_smem(int h, int nq, int ncoef) {
-  return rrq_target_fixed(h, nq) + rrq_target_per(h, nq, ncoef) * rrq_tile(h, nq, ncoef);
+size_t rrq_target_smem(int p, int h, int nq, int ncoef) {
+  return rrq_target_fixed(h, nq) + rrq_target_per(p, h, nq, ncoef) * rrq_tile(p, h, nq, ncoef);
 }
 
-constexpr int kTG = 2;  // targets per thread in the moment and weight loops
+// entry (n, m) of a raw-moment array of real weights, any m: conjugation and the
+// sign (-1)^m for m < 0, zero outside |m| <= n (selects, no branches)
+__device__ __forceinline__ cd rget(const cd* __restrict__ A, int n, int m) {
+  const int am = m < 0 ? -m : m;
+  const bool ok = n >= 0 && am <= n;
+  const cd v = A[ok ? hidx(n, am) : 0];
+  const double sr = (m < 0 && (am & 1)) ? -1.0 : 1.0, si = m < 0 ? -sr : sr;
+  return ok ? cmk(sr * v.re, si * v.im) : cmk(0.0, 0.0);
+}
+
+// The boundary moments of one target, chi_n^m = sum_b a . grad H'_n^m and
+// alpha_n^m = sum_b z (x) V_n^m with V_n^m = n [dX_Z grad H'_{n-1}^m + i/2 (dW
+// grad H'_{n-1}^{m-1} + conj(dW) grad H'_{n-1}^{m+1})] (a = (w1/R)(z x dX), z
+// scaled by w1/R, (x) the quaternion product (-z.V, z x V)), from raw moments
+// of real node weights: R_c = sum_b a_c H'_{n'}^{m'} (n' < p) and Z_cj = sum_b
+// z_c dX_j H'_{n'}^{m'} (n' < p - 1).  The gradient is linear with constant
+// coefficients in the harmonics of one degree lower (harm_grad), and the
+// weights are real, so the conjugation rule for negative orders carries over to
+// the raw moments.
+__device__ __forceinline__ void moments_from_raw(const cd* __restrict__ R, int p, int n, int m, cd* M) {
+  const int nc1 = p * (p + 1) / 2, nc2 = (p - 1) * p / 2;
+  for (int c = 0; c < 5; ++c) M[c] = cmk(0.0, 0.0);
+  if (n < 1) return;
+  // chi = sum_c (grad of R_c at degree n)_c
+  {
+    const cd a = rget(R, n - 1, m), lo = rget(R + nc1, n - 1, m - 1), hi = rget(R + nc1, n - 1, m + 1);
+    const cd lo2 = rget(R + 2 * nc1, n - 1, m - 1), hi2 = rget(R + 2 * nc1, n - 1, m + 1);
+    M[0] = cscale(cadd(cadd(a, cscale(cmuli(cadd(lo, hi)), 0.5)), cscale(csub(hi2, lo2), 0.5)), (double)n);
+  }
+  if (n < 2) return;
+  // U_cd = sum_b z_c V_d; alpha = (-(U_00 + U_11 + U_22), U_12 - U_21, U_20 - U_02, U_01 - U_10)
+  // added into M (LDS) term by term
+  const int n2 = n - 2;
+  const double g = 0.5 * (double)(n - 1);
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    const cd* Z = R + 3 * nc1 + 3 * c * nc2;
+    cd z0[3], z1[5], z2[5];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) z0[j] = rget(Z, n2, m - 1 + j);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { z1[j] = rget(Z + nc2, n2, m - 2 + j); z2[j] = rget(Z + 2 * nc2, n2, m - 2 + j); }
+    // with G the gradient at degree n-1 (harm_grad) and g = (n-1)/2:
+    // U_cd = n (G(Z_c0)^m + i/2 [G(Z_c1)^{m-1} + G(Z_c1)^{m+1} + i (G(Z_c2)^{m-1} -
+    // G(Z_c2)^{m+1})])_d = n g (t0 + i/2 a - b/2)_d, the components collected:
+    //   t0 = (2 z0_m, i (z0_{m-1} + z0_{m+1}), z0_{m+1} - z0_{m-1})
+    //   a  = (2 (z1_{m-1} + z1_{m+1}), i (z1_{m-2} + 2 z1_m + z1_{m+2}), z1_{m+2} - z1_{m-2})
+    //   b  = (2 (z2_{m-1} - z2_{m+1}), i (z2_{m-2} - z2_{m+2}), 2 z2_m - z2_{m-2} - z2_{m+2})
+    const double f = (double)n * g;
+    cd u[3];
+    {
+      const cd t0 = cscale(z0[1], 2.0), a = cscale(cadd(z1[1], z1[3]), 2.0), b = cscale(csub(z2[1], z2[3]), 2.0);
+      u[0] = cscale(cadd(t0, csub(cscale(cmuli(a), 0.5), cscale(b, 0.5))), f);
+    }
+    {
+      const cd t0 = cmuli(cadd(z0[0], z0[2])), a = cmuli(cadd(cadd(z1[0], z1[4]), cscale(z1[2], 2.0)));
+      const cd b = cmuli(csub(z2[0], z2[4]));
+      u[1] = cscale(cadd(t0, csub(cscale(cmuli(a), 0.5), cscale(b, 0.5))), f);
+    }
+    {
+      const cd t0 = csub(z0[2], z0[0]), a = csub(z1[4], z1[0]), b = csub(cscale(z2[2], 2.0), cadd(z2[0], z2[4]));
+      u[2] = cscale(cadd(t0, csub(cscale(cmuli(a), 0.5), cscale(b, 0.5))), f);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d == c) M[1] = csub(M[1], u[d]);
+      else if ((d - c + 3) % 3 == 1) M[1 + (c + 2) % 3 + 1] = cadd(M[1 + (c + 2) % 3 + 1], u[d]);  // U_01, U_12, U_20
+      else M[1 + (c + 1) % 3 + 1] = csub(M[1 + (c + 1) % 3 + 1], u[d]);                            // U_02, U_10, U_21
+    }
+  }
+}
 
 __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels P, const int* __restrict__ lpairs,
                                                          const int* __restrict__ lstart, const int* __restrict__ ltot,
@@ -2223,22 +2271,21 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
   const int nmine = nrrq[k];
   if (nmine == 0) return;
   const int h = T.h, nq = T.nq, p = T.p, nb = 3 * nq, ncoef = T.ncoef, H4 = 4 * h, H5 = 5 * h;
-  const int TT = rrq_tile(h, nq, ncoef);
-  const int big = 6 * nb > H5 ? 6 * nb : H5;
+  const int nc1 = p * (p + 1) / 2, nc2 = (p - 1) * p / 2;
+  const int TT = rrq_tile(p, h, nq, ncoef), rstr = rrq_rstr(p, h), mstr = rrq_mstr(nb, ncoef);
   extern __shared__ double sm[];
   double* fr = sm;                        // 16
   double* bx = fr + 16;                   // [nb][6]
   double* tX = bx + 6 * nb;               // [TT][6]   frame X, original x
   double* w1 = tX + 6 * TT;               // [TT][nb]
-  double* ab = w1 + (size_t)nb * TT;      // [TT][big]: a,b per node, later the moment vector v
-  double* tq = ab + (size_t)big * TT;     // [TT][4]: solid angle, Lambda
-  cd* mom = (cd*)(tq + 4 * TT);           // [TT][ncoef][5]
+  double* rv = w1 + (size_t)nb * TT;      // [TT][rstr]: raw moments, later the moment vector v
+  double* tq = rv + (size_t)rstr * TT;    // [TT][4]: solid angle, Lambda
+  cd* mom = (cd*)(tq + 4 * TT);           // [TT][mstr]: node weights [nb][12], later moments [ncoef][5]
   __shared__ int plist[16];
   for (int i = threadIdx.x; i < 16; i += blockDim.x) fr[i] = P.frame[(size_t)k * 16 + i];
   for (int i = threadIdx.x; i < 6 * nb; i += blockDim.x) bx[i] = P.bX[(size_t)k * nb * 6 + i];
   const double sigma = P.frame[(size_t)k * 16 + 13], alpha = P.frame[(size_t)k * 16 + 12];
-  const double* G = P.G + (size_t)k * 2 * h * H5;
-  const double* basis = P.basis + (size_t)k * nb * ncoef * 12;
+  const cd* Hb = (const cd*)P.basis + (size_t)k * nb * nc1;
   const int r0 = lstart[4 * (size_t)npan + k];
   const int* mine = lpairs + rrq_list_base(ltot) + r0;
   for (int tile0 = 0; tile0 < nmine; tile0 += TT) {
@@ -2259,7 +2306,7 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       tX[6 * t + 3] = x[0]; tX[6 * t + 4] = x[1]; tX[6 * t + 5] = x[2];
     }
     __syncthreads();
-    // a = (w1/R)(z x dX), b = (w1/R) z
+    // node weights: a = (w1/R)(z x dX) and (w1/R) z_c dX_j
     for (int it = threadIdx.x; it < nt * nb; it += blockDim.x) {
       const int t = it / nb, ib = it % nb;
       const double* Y = bx + 6 * ib;
@@ -2268,8 +2315,10 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       w1[(size_t)t * nb + ib] = wr;  // w1/R from here on
       double zx[3];
       cross3(z, Y + 3, zx);
-      double* o = ab + (size_t)t * big + 6 * ib;
-      for (int c = 0; c < 3; ++c) { o[c] = wr * zx[c]; o[3 + c] = wr * z[c]; }
+      double* o = (double*)(mom + (size_t)t * mstr) + 12 * ib;
+      for (int c = 0; c < 3; ++c) o[c] = wr * zx[c];
+      for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < 3; ++j) o[3 + 3 * c + j] = (wr * z[c]) * Y[3 + j];
     }
     __syncthreads();
     // Lambda = sum (w1/R) dX per (target, component), into the spare slots of tq
@@ -2279,67 +2328,62 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       for (int ib = 0; ib < nb; ++ib) l += w1[(size_t)t * nb + ib] * bx[6 * ib + 3 + c];
       tq[4 * t + 1 + c] = l;
     }
-    // boundary moments chi (1 complex) and alpha (4 complex) per coefficient;
-    // a thread serves one coefficient for kTG targets, so each basis value is
-    // read once per group (basis is component-major: consecutive kk coalesce)
-    {
-      const int ngrp = (nt + kTG - 1) / kTG;
-      for (int it = threadIdx.x; it < ngrp * ncoef; it += blockDim.x) {
-        const int kk = it % ncoef, t0 = kTG * (it / ncoef);
-        double acc[kTG][10];
+    // raw moments, one thread per (target, harmonic of degree < p): the three
+    // a-weights always, the nine z dX weights for degrees < p - 1; consecutive
+    // threads read consecutive harmonics of a node
+    for (int it = threadIdx.x; it < nt * nc1; it += blockDim.x) {
+      const int t = it / nc1, kk = it % nc1;
+      const bool zp = kk < nc2;
+      const double* wt = (const double*)(mom + (size_t)t * mstr);
+      double acc[24];
 #pragma unroll
-        for (int g = 0; g < kTG; ++g)
+      for (int c = 0; c < 24; ++c) acc[c] = 0.0;
+      // the harmonics of kHB nodes are loaded together (loads in flight)
+      constexpr int kHB = 6;
+      for (int ib0 = 0; ib0 < nb; ib0 += kHB) {
+        cd hb[kHB];
 #pragma unroll
-          for (int c = 0; c < 10; ++c) acc[g][c] = 0.0;
-        // the basis of kIB nodes is loaded together (more loads in flight)
-        constexpr int kIB = 3;
-        for (int ib0 = 0; ib0 < nb; ib0 += kIB) {
-          double Bv[kIB][12];
+        for (int u = 0; u < kHB; ++u) hb[u] = Hb[(size_t)min(ib0 + u, nb - 1) * nc1 + kk];
 #pragma unroll
-          for (int u = 0; u < kIB; ++u) {
-            const double* Bp = basis + (size_t)min(ib0 + u, nb - 1) * 12 * ncoef + kk;
+        for (int u = 0; u < kHB; ++u) {
+          if (ib0 + u >= nb) break;
+          const cd hv = hb[u];
+          const double* w = wt + 12 * (ib0 + u);
 #pragma unroll
-            for (int c = 0; c < 12; ++c) Bv[u][c] = Bp[c * ncoef];
+          for (int q = 0; q < 3; ++q) {
+            acc[2 * q] = fma(w[q], hv.re, acc[2 * q]);
+            acc[2 * q + 1] = fma(w[q], hv.im, acc[2 * q + 1]);
           }
+          if (zp) {
 #pragma unroll
-          for (int u = 0; u < kIB; ++u) {
-          const int ib = ib0 + u;
-          if (ib >= nb) break;
-          const double* B = Bv[u];
-#pragma unroll
-          for (int g = 0; g < kTG; ++g) {
-            if (t0 + g >= nt) break;
-            const double* o = ab + (size_t)(t0 + g) * big + 6 * ib;
-            const double o0 = o[0], o1 = o[1], o2 = o[2], z0 = o[3], z1 = o[4], z2 = o[5];
-            double* A = acc[g];
-            // chi += a . gradH ; z (x) V = (-z.V, z x V)
-            A[0] = fma(o0, B[0], fma(o1, B[2], fma(o2, B[4], A[0])));
-            A[1] = fma(o0, B[1], fma(o1, B[3], fma(o2, B[5], A[1])));
-            A[2] = fma(-z0, B[6], fma(-z1, B[8], fma(-z2, B[10], A[2])));
-            A[3] = fma(-z0, B[7], fma(-z1, B[9], fma(-z2, B[11], A[3])));
-            A[4] = fma(z1, B[10], fma(-z2, B[8], A[4]));
-            A[5] = fma(z1, B[11], fma(-z2, B[9], A[5]));
-            A[6] = fma(z2, B[6], fma(-z0, B[10], A[6]));
-            A[7] = fma(z2, B[7], fma(-z0, B[11], A[7]));
-            A[8] = fma(z0, B[8], fma(-z1, B[6], A[8]));
-            A[9] = fma(z0, B[9], fma(-z1, B[7], A[9]));
+            for (int q = 3; q < 12; ++q) {
+              acc[2 * q] = fma(w[q], hv.re, acc[2 * q]);
+              acc[2 * q + 1] = fma(w[q], hv.im, acc[2 * q + 1]);
+            }
           }
-          }
-        }
-#pragma unroll
-        for (int g = 0; g < kTG; ++g) {
-          if (t0 + g >= nt) break;
-          cd* M = mom + ((size_t)(t0 + g) * ncoef + kk) * 5;
-#pragma unroll
-          for (int c = 0; c < 5; ++c) M[c] = cmk(acc[g][2 * c], acc[g][2 * c + 1]);
         }
       }
+      cd* R = (cd*)(rv + (size_t)t * rstr);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) R[q * nc1 + kk] = cmk(acc[2 * q], acc[2 * q + 1]);
+      if (zp) {
+#pragma unroll
+        for (int q = 3; q < 12; ++q) R[3 * nc1 + (q - 3) * nc2 + kk] = cmk(acc[2 * q], acc[2 * q + 1]);
+      }
+    }
+    __syncthreads();
+    // boundary moments chi (component 0) and alpha (1..4) per coefficient
+    for (int it = threadIdx.x; it < nt * ncoef; it += blockDim.x) {
+      const int t = it / ncoef, kk = it % ncoef;
+      int n = 0;
+      while (hidx(n + 1, 0) <= kk) ++n;
+      moments_from_raw((const cd*)(rv + (size_t)t * rstr), p, n, kk - hidx(n, 0), mom + (size_t)t * mstr + 5 * kk);
     }
     __syncthreads();
     // seeds: W~_00 = solid angle, degree-1 Q = Lambda (x) grad H'_1^m with
     // Lambda = -omega + sum (w1/R) dX
     for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-      cd* M = mom + (size_t)t * ncoef * 5;
+      cd* M = mom + (size_t)t * mstr;
       const double om = tq[4 * t];
       const double lam[3] = {tq[4 * t + 1], tq[4 * t + 2], tq[4 * t + 3]};
       M[0] = cmk(om, 0.0);
@@ -2363,7 +2407,7 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       for (int it = threadIdx.x; it < nt * (n + 1) * ncomp; it += blockDim.x) {
         const int c = it % ncomp, tm = it / ncomp;
         const int t = tm / (n + 1), m = tm % (n + 1);
-        cd* M = mom + (size_t)t * ncoef * 5;
+        cd* M = mom + (size_t)t * mstr;
         const double* X = tX + 6 * t;
         const double xZ = X[0];
         const cd w = cmk(X[1], X[2]), wc = cmk(X[1], -X[2]);
@@ -2383,14 +2427,14 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       }
     }
     __syncthreads();
-    // moment vector v = [rhs (4h); W (h)] into the (now free) a,b slots
+    // moment vector v = [rhs (4h); W (h)] into the (now free) raw-moment slots
     for (int it = threadIdx.x; it < nt * h; it += blockDim.x) {
       const int t = it / h, kk = it % h;
       int n = 1;
       while ((n + 1) * n / 2 <= kk) ++n;
       const int m = kk - (n - 1) * n / 2 + 1;
-      const cd* E = mom + ((size_t)t * ncoef + hidx(n, m)) * 5;
-      double* v = ab + (size_t)t * big;
+      const cd* E = mom + (size_t)t * mstr + (size_t)hidx(n, m) * 5;
+      double* v = rv + (size_t)t * rstr;
       v[kk] = sigma * imrot(m, E[1]);
       v[h + kk] = -sigma * imrot(m, E[2]);
       v[2 * h + kk] = -sigma * imrot(m, E[3]);
@@ -2402,14 +2446,14 @@ __global__ void __launch_bounds__(256) rrq_target_kernel(RRQTables T, RRQPanels 
       const double f = 1.0 / (16.0 * atan(1.0)) / alpha;
       for (int it = threadIdx.x; it < nt * H4; it += blockDim.x) {
         const int t = it / H4, i = it % H4;
-        omega_out[(size_t)plist[t] * H4 + i] = ab[(size_t)t * big + i] * f;
+        omega_out[(size_t)plist[t] * H4 + i] = rv[(size_t)t * rstr + i] * f;
       }
       for (int t = threadIdx.x; t < nt; t += blockDim.x) ias_out[plist[t]] = sigma * tq[4 * t];
     }
     // moment vectors to HBM, per panel transposed: vt[r0*5h + i*nrrq + t]
     for (int it = threadIdx.x; it < nt * H5; it += blockDim.x) {
       const int t = it % nt, i = it / nt;
-      vt[(size_t)r0 * H5 + (size_t)i * nmine + tile0 + t] = ab[(size_t)t * big + i];
+      vt[(size_t)r0 * H5 + (size_t)i * nmine + tile0 + t] = rv[(size_t)t * rstr + i];
     }
   }
 }
