@@ -662,7 +662,9 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   rec(3, 0, s2);
   const size_t lds1 = rrq_frame_smem(h, nq);
   set_lds((const void*)rrq_panel_frame_kernel, lds1);
-  hipLaunchKernelGGL(rrq_panel_frame_kernel, dim3(npan), dim3(256), lds1, s2, tab->RT, RP, npan, (const int*)nullptr,
+  // one wave per panel: the kernel's loops run over the h nodes or the boundary
+  // nodes, so wider workgroups leave lanes idle while fewer panels share a CU
+  hipLaunchKernelGGL(rrq_panel_frame_kernel, dim3(npan), dim3(64), lds1, s2, tab->RT, RP, npan, (const int*)nullptr,
                      (const double*)d_sx.ptr, (const double*)d_snx.ptr, (const double*)d_sw.ptr, ichart,
                      (const double*)d_sxbd.ptr, (const double*)d_rv.ptr);
   HIPCHK(hipEventRecord(ev_setup, s2));
