@@ -146,8 +146,8 @@ typedef float sqf4 __attribute__((ext_vector_type(4)));
 
 // per-wave LDS of the classification's network: the features of 64 pairs (row
 // stride kSelFS), a 16-pair hidden layer (row stride kSelHS = 4 mod 32, so the
-// A-operand reads of rows l%16, columns 4s + l/16 spread over the 32 banks) and
-// the 64 pairs' logits (row stride 8)
+// A-operand reads of rows l%16 start on different banks) and the 64 pairs'
+// logits (row stride 8)
 constexpr int kSelFS = 13, kSelHS = kSelHid + 4;
 constexpr int kSelWave = 64 * kSelFS + 16 * kSelHS + 64 * 8;  // floats
 
@@ -157,29 +157,65 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The network for the 64 pairs of one wave on the FP32 matrix cores
-// (v_mfma_f32_16x16x4f32), ngroups groups of 16 pairs: F holds the normalised
-// features (columns 10 and 11 zero), the logits go to L[pair][8].  BF16 weights
-// (exact in FP32) as the B operand, FP32 accumulation, the bias added after the
-// sum.  Every lane of the wave takes part.  Lane l supplies A[l%16][l/16] and
-// B[l/16][l%16] of a 4-deep step and holds D[4(l/16)+r][l%16] of a tile.
+typedef __bf16 sbf8 __attribute__((ext_vector_type(8)));
+
+// x = hi + mid + lo exactly, each part a BF16 (8 significant bits each, so the
+// three carry the FP32 significand; each residual is exact in FP32)
+__device__ __forceinline__ void split_bf16x3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
+  hi = (__bf16)x;
+  const float r1 = x - (float)hi;
+  mid = (__bf16)r1;
+  lo = (__bf16)(r1 - (float)mid);
+}
+
+__device__ __forceinline__ __bf16 bf16_bits(uint16_t u) { return __builtin_bit_cast(__bf16, u); }
+
+// One 32-deep step of D += A B for the three BF16 parts of A: lane l supplies
+// a[0..7] = A[l%16][8(l/16) + j] (FP32, split here) and b = B[8(l/16) + j][l%16].
+__device__ __forceinline__ sqf4 mfma_bf16x3(const float* a, sbf8 b, sqf4 acc) {
+  sbf8 ah, am, al;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    __bf16 x, y, z;
+    split_bf16x3(a[j], x, y, z);
+    ah[j] = x; am[j] = y; al[j] = z;
+  }
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, b, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b, acc, 0, 0, 0);
+}
+
+// The network for the 64 pairs of one wave on the BF16 matrix cores
+// (v_mfma_f32_16x16x32_bf16), ngroups groups of 16 pairs: F holds the normalised
+// features (row stride kSelFS), the logits go to L[pair][8].  Every FP32
+// activation enters as the sum of three BF16 parts and the weights are BF16, so
+// every product is exact and the sums are FP32 (the order of the additions
+// inside a step is the hardware's); the bias is added after the sum.  Every lane
+// of the wave takes part.  Lane l supplies A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16]
+// of a 32-deep step and holds D[4(l/16)+r][l%16] of a tile.
 __device__ void selector_mlp_wave(const uint16_t* __restrict__ W, const float* __restrict__ Bi, const float* F,
                                   float* Hb, float* L, int ngroups) {
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   for (int g = 0; g < ngroups; ++g) {
     const float* Fg = F + 16 * g * kSelFS;
     sqf4 acc[4];
-    // layer 1: K = 10 in three steps (the last one padded), N = 64 in four tiles
+    // layer 1: K = 10 in one step (columns 10..31 zero), N = 64 in four tiles
+    {
+      float a[8];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = sqf4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      const int kq = 4 * s + lk;
-      const float a = Fg[li * kSelFS + kq];
+      for (int j = 0; j < 8; ++j) {
+        const int kq = 8 * lk + j;
+        a[j] = kq < kSelIn ? Fg[li * kSelFS + kq] : 0.0f;
+      }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const float b = kq < kSelIn ? bf16_to_f32(W[(16 * t + li) * kSelIn + kq]) : 0.0f;
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        sbf8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int kq = 8 * lk + j;
+          b[j] = bf16_bits(kq < kSelIn ? W[(16 * t + li) * kSelIn + kq] : (uint16_t)0);
+        }
+        acc[t] = mfma_bf16x3(a, b, sqf4{0.0f, 0.0f, 0.0f, 0.0f});
       }
     }
 #pragma unroll
@@ -187,16 +223,20 @@ __device__ void selector_mlp_wave(const uint16_t* __restrict__ W, const float* _
 #pragma unroll
       for (int r = 0; r < 4; ++r) Hb[(4 * lk + r) * kSelHS + 16 * t + li] = fmaxf(acc[t][r] + Bi[16 * t + li], 0.0f);
     wave_sync();
-    // layer 2: K = 64 in 16 steps, N = 64
+    // layer 2: K = 64 in two steps, N = 64
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = sqf4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int s = 0; s < kSelHid / 4; ++s) {
-      const int kq = 4 * s + lk;
-      const float a = Hb[li * kSelHS + kq];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      float a[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = Hb[li * kSelHS + 32 * s + 8 * lk + j];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const float b = bf16_to_f32(W[kSelW1 + (16 * t + li) * kSelHid + kq]);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        sbf8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = bf16_bits(W[kSelW1 + (16 * t + li) * kSelHid + 32 * s + 8 * lk + j]);
+        acc[t] = mfma_bf16x3(a, b, acc[t]);
       }
     }
     wave_sync();  // every lane has read the first hidden layer
@@ -206,13 +246,19 @@ __device__ void selector_mlp_wave(const uint16_t* __restrict__ W, const float* _
       for (int r = 0; r < 4; ++r)
         Hb[(4 * lk + r) * kSelHS + 16 * t + li] = fmaxf(acc[t][r] + Bi[kSelHid + 16 * t + li], 0.0f);
     wave_sync();
-    // layer 3: K = 64, N = 7 in one tile (columns 7..15 zero)
+    // layer 3: K = 64 in two steps, N = 7 in one tile (columns 7..15 zero)
     sqf4 o = sqf4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int s = 0; s < kSelHid / 4; ++s) {
-      const int kq = 4 * s + lk;
-      const float a = Hb[li * kSelHS + kq];
-      const float b = li < kSelOut ? bf16_to_f32(W[kSelW1 + kSelW2 + li * kSelHid + kq]) : 0.0f;
-      o = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, o, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      float a[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = Hb[li * kSelHS + 32 * s + 8 * lk + j];
+      sbf8 b;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        b[j] = bf16_bits(li < kSelOut ? W[kSelW1 + kSelW2 + min(li, kSelOut - 1) * kSelHid + 32 * s + 8 * lk + j]
+                                      : (uint16_t)0);
+      o = mfma_bf16x3(a, b, o);
     }
     if (li < kSelOut)
 #pragma unroll
