@@ -695,9 +695,16 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
                      (double*)d_pq.ptr, (int*)d_eorder.ptr, (int*)d_pfail.ptr, nnz, s3);
   rec(5, 1, s3);
   HIPCHK(hipEventRecord(ev_proj, s3));
-  // s1: direct levels, after the projection: the projection heads the chain the
-  // moments wait for, and the direct kernel's 121k waves would crowd it out
-  HIPCHK(hipStreamWaitEvent(s1, ev_proj, 0));
+  rec(6, 0, s3);
+  launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
+                  (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
+                  (double*)d_root.ptr, (double*)d_oe.ptr, (int*)d_eorder.ptr, nnz, s3, timing ? tev[8][0] : nullptr);
+  rec(6, 1, s3);
+  HIPCHK(hipEventRecord(ev_edge, s3));
+  // s1: direct levels, after the edge chain (projection, roots, solid angle): that
+  // chain heads the path the moments wait for, the direct kernel's 121k waves
+  // would crowd it out, and the direct kernel then runs beside the moments
+  HIPCHK(hipStreamWaitEvent(s1, ev_edge, 0));
   rec(2, 0, s1);
   direct(s1);
   rec(2, 1, s1);
@@ -705,12 +712,6 @@ void Engine::eval(int exterior, double side_tol, bool want_omega) {
   const bool fuse = density_set;
   if (fuse) launch_correction(0, 3, s1);
   HIPCHK(hipEventRecord(ev_direct, s1));
-  rec(6, 0, s3);
-  launch_rrq_edge(tab->RT, RP, (const int*)d_lpairs.ptr, (const int*)d_ltot.ptr, (const int*)d_ppanel.ptr,
-                  (const int*)d_tidx.ptr, (const double*)d_tx.ptr, (const double*)d_pq.ptr, (double*)d_w1.ptr,
-                  (double*)d_root.ptr, (double*)d_oe.ptr, (int*)d_eorder.ptr, nnz, s3, timing ? tev[8][0] : nullptr);
-  rec(6, 1, s3);
-  HIPCHK(hipEventRecord(ev_edge, s3));
   // main: moments once the edges and the boundary harmonics are in, weights once the
   // solve is in too
   HIPCHK(hipStreamWaitEvent(stream, ev_edge, 0));
